@@ -1,0 +1,165 @@
+/*
+ * mela.h -- C ABI of the MI355X (gfx950) GF(2) linear-algebra library.
+ *
+ * Drop-in for the six MEX functions of HCPhy/mex_linear_algebra.  Each of the
+ * reference's gateways (void mexFunction(int nlhs, mxArray *plhs[], int nrhs,
+ * const mxArray *prhs[])) validates its mxArray arguments, bit-packs them,
+ * runs a CPU core and materialises a logical/double result.  Here the
+ * validation order and message text are kept, the packing, elimination and
+ * output materialisation run as HIP kernels, and errors come back as status
+ * codes plus a thread-local message instead of a longjmp out of the call.
+ *
+ * Matrices handed to the six entry points are MATLAB column-major arrays:
+ * element (i,j) of an m x n array sits at index i + j*m.  A "logical" array
+ * is one byte per element (nonzero = true); a "double" array is read as
+ * ((int)x) & 1 (x86-64 semantics for NaN/Inf/out-of-range: parity 0).
+ *
+ * All functions are safe to call from several host threads on distinct
+ * contexts; one context serialises its own calls.
+ */
+#ifndef MELA_H
+#define MELA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum mela_status {
+    MELA_OK = 0,
+    MELA_ERR_SPARSE = 1,    /* sparse input (reference rejects, matmul_gf2.c:78-79) */
+    MELA_ERR_DIM = 2,       /* inner dimensions differ (matmul_gf2.c:84) */
+    MELA_ERR_TYPE = 3,      /* neither logical nor double (matmul_gf2.c:86-88) */
+    MELA_ERR_NOMEM = 4,     /* host or device allocation failed */
+    MELA_ERR_INVALID = 5,   /* null pointer / size overflow / bad descriptor */
+    MELA_ERR_DEVICE = 6,    /* HIP runtime error (message names the call) */
+    MELA_ERR_ADMISSION = 7, /* request larger than the server's HBM budget */
+    MELA_ERR_CANCELLED = 8, /* request cancelled or server shut down */
+    MELA_ERR_TIMEOUT = 9,   /* wait deadline passed */
+    MELA_ERR_ARGS = 10      /* wrong number of inputs/outputs (MEX gateways) */
+} mela_status;
+
+/* Input class: the two mxClassIDs the reference accepts, plus OTHER for any
+ * other class.  OR with MELA_SPARSE for a sparse array. */
+enum {
+    MELA_LOGICAL = 0,
+    MELA_DOUBLE = 1,
+    MELA_OTHER = 2,
+    MELA_SPARSE = 0x100
+};
+
+typedef struct mela_ctx mela_ctx;
+
+/* ----------------------------------------------------------- context ---- */
+mela_status mela_ctx_create(int device, mela_ctx **ctx);
+void mela_ctx_destroy(mela_ctx *ctx);
+/* HIP stream (hipStream_t) every kernel of this context runs on */
+void *mela_ctx_stream(mela_ctx *ctx);
+mela_status mela_ctx_sync(mela_ctx *ctx);
+/* message and MATLAB identifier of the last failure on this thread */
+const char *mela_last_error(void);
+const char *mela_last_error_id(void);
+const char *mela_version(void);
+
+/* ------------------------------------------- the six reference entry points
+ *
+ * mela_matmul_gf2 / mela_matmul_m4ri replace mexFunction at
+ * src/mela_matmul_gf2.c:72 and src/mela_matmul_m4ri.c:140:
+ *   C (m x n logical, caller-allocated m*n bytes) = A (m x k) * B (k2 x n)
+ * Checks in reference order: sparse ("Sparse matrices not supported. Convert
+ * to full using full()."), k == k2 ("Inner dimensions must agree."), classes
+ * ("Inputs must be logical or double matrices."). */
+mela_status mela_matmul_gf2(mela_ctx *ctx, const void *A, int a_class,
+                            size_t m, size_t k, const void *B, int b_class,
+                            size_t k2, size_t n, uint8_t *C);
+mela_status mela_matmul_m4ri(mela_ctx *ctx, const void *A, int a_class,
+                             size_t m, size_t k, const void *B, int b_class,
+                             size_t k2, size_t n, uint8_t *C);
+
+/* mela_rank_gf2 replaces src/mela_rank_gf2.c:126 (class message "Input must
+ * be a logical or double matrix.", id bitrank_mex:inputNotLogical);
+ * mela_rank_m4ri replaces src/mela_rank_m4ri.c:286 ("Input must be double or
+ * logical.").  The rank is returned as a double, like mxCreateDoubleScalar.
+ * Sparse input is rejected with the matmul message (the reference gateways do
+ * not check and would read a sparse array's storage as dense data). */
+mela_status mela_rank_gf2(mela_ctx *ctx, const void *A, int a_class,
+                          size_t m, size_t n, double *rank);
+mela_status mela_rank_m4ri(mela_ctx *ctx, const void *A, int a_class,
+                           size_t m, size_t n, double *rank);
+
+/* mela_null_gf2 / mela_null_m4ri replace src/mela_null_gf2.c:142 and
+ * src/mela_null_m4ri.c:251.  The n x (n - rank) logical basis Z is the
+ * canonical one (Z(free,:) = I, A*Z = 0, free columns ascending); its size is
+ * known only after elimination, so the library calls alloc(user, n, nullity)
+ * once and fills the column-major bytes it returns (the MEX gateway passes a
+ * function that creates the mxArray, as null_gf2.c:208 does).  Messages:
+ * sparse (as matmul), "H must be double or logical.", "Out of memory.". */
+typedef uint8_t *(*mela_alloc_logical_fn)(void *user, size_t rows, size_t cols);
+mela_status mela_null_gf2(mela_ctx *ctx, const void *H, int h_class,
+                          size_t m, size_t n, mela_alloc_logical_fn alloc,
+                          void *user, size_t *nullity);
+mela_status mela_null_m4ri(mela_ctx *ctx, const void *H, int h_class,
+                           size_t m, size_t n, mela_alloc_logical_fn alloc,
+                           void *user, size_t *nullity);
+
+/* ------------------------------------------- device-resident packed forms
+ *
+ * A packed matrix lives in device memory, row-major: row i is ld 64-bit
+ * words, bit j of row i is bit (j & 63) of word (j >> 6) (the reference's
+ * packed layout, matmul_gf2.c:33-36).  Bits past column cols must be zero. */
+typedef struct mela_bitmat {
+    uint64_t *data; /* device pointer */
+    size_t rows, cols, ld;
+} mela_bitmat;
+
+enum {
+    MELA_ALGO_AUTO = 0,
+    MELA_ALGO_ANDPOP = 1, /* AND + parity over a transposed B (v_bitop3) */
+    MELA_ALGO_M4RM = 2,   /* Method of Four Russians, Gray-code tables in LDS */
+    MELA_ALGO_MFMA_I8 = 3 /* exact int8 MFMA, i32 accumulate, & 1 */
+};
+
+mela_status mela_dev_alloc(mela_ctx *ctx, size_t bytes, void **dptr);
+mela_status mela_dev_free(mela_ctx *ctx, void *dptr);
+mela_status mela_copy_h2d(mela_ctx *ctx, void *dst, const void *src, size_t bytes);
+mela_status mela_copy_d2h(mela_ctx *ctx, void *dst, const void *src, size_t bytes);
+mela_status mela_bitmat_alloc(mela_ctx *ctx, size_t rows, size_t cols, mela_bitmat *out);
+void mela_bitmat_free(mela_ctx *ctx, mela_bitmat *m);
+
+/* Bernoulli(1/2) bits: word (i,w) = splitmix64(splitmix64(seed) + i*ceil(cols/64) + w) */
+mela_status mela_gen_bernoulli(mela_ctx *ctx, uint64_t seed, mela_bitmat *M);
+
+/* pack a column-major logical/double host-or-device array into M (device) */
+mela_status mela_pack_dev(mela_ctx *ctx, const void *src_dev, int src_class,
+                          mela_bitmat *M);
+/* unpack M into a column-major logical device array of rows*cols bytes */
+mela_status mela_unpack_dev(mela_ctx *ctx, const mela_bitmat *M, uint8_t *dst_dev);
+
+/* C = A * B over GF(2) (C->rows == A->rows, C->cols == B->cols,
+ * A->cols == B->rows).  Enqueued on the context stream; returns at once. */
+mela_status mela_matmul_packed(mela_ctx *ctx, const mela_bitmat *A,
+                               const mela_bitmat *B, mela_bitmat *C, int algo);
+
+/* Elimination variants: GE (pivot rows applied one by one) and M4RI (pivot
+ * rows applied through 2^k-entry tables).  Outputs are identical. */
+enum { MELA_ELIM_GE = 0, MELA_ELIM_M4RI = 1 };
+
+/* rank of A (A is left unchanged; work memory comes from the context) */
+mela_status mela_rank_packed(mela_ctx *ctx, const mela_bitmat *A, int variant,
+                             size_t *rank);
+/* reduced row echelon form: R (min(m,n) x cols, device, may be NULL) gets the
+ * rank nonzero rows in pivot order; pivots (host, min(m,n) ints, may be NULL)
+ * the pivot columns ascending. */
+mela_status mela_rref_packed(mela_ctx *ctx, const mela_bitmat *A, int variant,
+                             mela_bitmat *R, int *pivots, size_t *rank);
+/* canonical null basis, packed: Z is cols x nullity, allocated by the call
+ * (free with mela_bitmat_free). */
+mela_status mela_null_packed(mela_ctx *ctx, const mela_bitmat *A, int variant,
+                             mela_bitmat *Z, size_t *nullity);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MELA_H */

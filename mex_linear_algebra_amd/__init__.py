@@ -1,0 +1,327 @@
+"""MI355X-native GF(2) linear algebra: drop-in for HCPhy/mex_linear_algebra.
+
+The six functions below mirror the reference's MATLAB entry points (README.md:
+38-71): same names, same argument meaning (logical or double matrices, double
+values taken as ((int)x) & 1), same outputs (logical matrix, double rank,
+logical n x (n - rank) null basis) and the same error messages.  They call the
+HIP library libmela.so through its C ABI (include/mela.h); there is no CPU
+fallback -- a missing library or GPU raises MelaError.
+
+    C = mela_matmul_gf2(A, B)      C = mela_matmul_m4ri(A, B)
+    r = mela_rank_gf2(A)           r = mela_rank_m4ri(A)
+    Z = mela_null_gf2(H)           Z = mela_null_m4ri(H)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+__all__ = [
+    "MelaError", "Context", "BitMatrix", "lib_path", "load", "default_context",
+    "mela_matmul_gf2", "mela_matmul_m4ri", "mela_rank_gf2", "mela_rank_m4ri",
+    "mela_null_gf2", "mela_null_m4ri",
+    "ALGO_AUTO", "ALGO_ANDPOP", "ALGO_M4RM", "ALGO_MFMA_I8", "ELIM_GE", "ELIM_M4RI",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+MELA_LOGICAL, MELA_DOUBLE, MELA_OTHER, MELA_SPARSE = 0, 1, 2, 0x100
+ALGO_AUTO, ALGO_ANDPOP, ALGO_M4RM, ALGO_MFMA_I8 = 0, 1, 2, 3
+ELIM_GE, ELIM_M4RI = 0, 1
+
+STATUS = {0: "OK", 1: "SPARSE", 2: "DIM", 3: "TYPE", 4: "NOMEM", 5: "INVALID",
+          6: "DEVICE", 7: "ADMISSION", 8: "CANCELLED", 9: "TIMEOUT", 10: "ARGS"}
+
+
+class MelaError(RuntimeError):
+    """Raised with the reference's message text (and MATLAB id where it has one)."""
+
+    def __init__(self, message, identifier="", status=None):
+        super().__init__(message)
+        self.identifier = identifier
+        self.status = status
+
+
+class BitMatrix(C.Structure):
+    """mela_bitmat: device pointer + rows, cols, ld (64-bit words per row)."""
+    _fields_ = [("data", C.c_void_p), ("rows", C.c_size_t), ("cols", C.c_size_t),
+                ("ld", C.c_size_t)]
+
+
+ALLOC_FN = C.CFUNCTYPE(C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t)
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def lib_path() -> str:
+    return os.environ.get("MELA_LIB", os.path.join(_HERE, "libmela.so"))
+
+
+def load():
+    """Loads libmela.so (fails loudly if it is not built)."""
+    global _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
+        path = lib_path()
+        if not os.path.exists(path):
+            raise MelaError(f"libmela.so not built at {path} (run __graft_entry__.build())")
+        L = C.CDLL(path)
+        st, vp, sz = C.c_int, C.c_void_p, C.c_size_t
+        ctxp = C.c_void_p
+        L.mela_ctx_create.argtypes = [C.c_int, C.POINTER(ctxp)]
+        L.mela_ctx_create.restype = st
+        L.mela_ctx_destroy.argtypes = [ctxp]
+        L.mela_ctx_destroy.restype = None
+        L.mela_ctx_stream.argtypes = [ctxp]
+        L.mela_ctx_stream.restype = vp
+        L.mela_ctx_sync.argtypes = [ctxp]
+        L.mela_ctx_sync.restype = st
+        L.mela_last_error.restype = C.c_char_p
+        L.mela_last_error_id.restype = C.c_char_p
+        L.mela_version.restype = C.c_char_p
+        for f in (L.mela_matmul_gf2, L.mela_matmul_m4ri):
+            f.argtypes = [ctxp, vp, C.c_int, sz, sz, vp, C.c_int, sz, sz, vp]
+            f.restype = st
+        for f in (L.mela_rank_gf2, L.mela_rank_m4ri):
+            f.argtypes = [ctxp, vp, C.c_int, sz, sz, C.POINTER(C.c_double)]
+            f.restype = st
+        for f in (L.mela_null_gf2, L.mela_null_m4ri):
+            f.argtypes = [ctxp, vp, C.c_int, sz, sz, ALLOC_FN, vp, C.POINTER(sz)]
+            f.restype = st
+        bm = C.POINTER(BitMatrix)
+        L.mela_dev_alloc.argtypes = [ctxp, sz, C.POINTER(vp)]
+        L.mela_dev_alloc.restype = st
+        L.mela_dev_free.argtypes = [ctxp, vp]
+        L.mela_dev_free.restype = st
+        L.mela_copy_h2d.argtypes = [ctxp, vp, vp, sz]
+        L.mela_copy_h2d.restype = st
+        L.mela_copy_d2h.argtypes = [ctxp, vp, vp, sz]
+        L.mela_copy_d2h.restype = st
+        L.mela_bitmat_alloc.argtypes = [ctxp, sz, sz, bm]
+        L.mela_bitmat_alloc.restype = st
+        L.mela_bitmat_free.argtypes = [ctxp, bm]
+        L.mela_bitmat_free.restype = None
+        L.mela_gen_bernoulli.argtypes = [ctxp, C.c_uint64, bm]
+        L.mela_gen_bernoulli.restype = st
+        L.mela_pack_dev.argtypes = [ctxp, vp, C.c_int, bm]
+        L.mela_pack_dev.restype = st
+        L.mela_unpack_dev.argtypes = [ctxp, bm, vp]
+        L.mela_unpack_dev.restype = st
+        L.mela_matmul_packed.argtypes = [ctxp, bm, bm, bm, C.c_int]
+        L.mela_matmul_packed.restype = st
+        L.mela_rank_packed.argtypes = [ctxp, bm, C.c_int, C.POINTER(sz)]
+        L.mela_rank_packed.restype = st
+        L.mela_rref_packed.argtypes = [ctxp, bm, C.c_int, bm, C.POINTER(C.c_int), C.POINTER(sz)]
+        L.mela_rref_packed.restype = st
+        L.mela_null_packed.argtypes = [ctxp, bm, C.c_int, bm, C.POINTER(sz)]
+        L.mela_null_packed.restype = st
+        _lib = L
+        return L
+
+
+def _raise(status):
+    L = load()
+    raise MelaError(L.mela_last_error().decode(), L.mela_last_error_id().decode(),
+                    STATUS.get(status, status))
+
+
+def _check(status):
+    if status != 0:
+        _raise(status)
+
+
+class Context:
+    """A device context (HIP stream + cached device scratch)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        _check(self.lib.mela_ctx_create(device, C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            self.lib.mela_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self):
+        return self.lib.mela_ctx_stream(self.handle)
+
+    def sync(self):
+        _check(self.lib.mela_ctx_sync(self.handle))
+
+    # ---- device-resident packed helpers ----
+    def bitmat(self, rows: int, cols: int) -> BitMatrix:
+        M = BitMatrix()
+        _check(self.lib.mela_bitmat_alloc(self.handle, rows, cols, C.byref(M)))
+        return M
+
+    def free(self, M: BitMatrix):
+        self.lib.mela_bitmat_free(self.handle, C.byref(M))
+
+    def upload(self, P: np.ndarray, cols: int) -> BitMatrix:
+        P = np.ascontiguousarray(P, dtype=np.uint64)
+        M = self.bitmat(P.shape[0], cols)
+        if P.shape[1] != M.ld:
+            Q = np.zeros((P.shape[0], M.ld), dtype=np.uint64)
+            Q[:, : min(P.shape[1], M.ld)] = P[:, : M.ld]
+            P = Q
+        if P.size:
+            _check(self.lib.mela_copy_h2d(self.handle, M.data, P.ctypes.data, P.nbytes))
+        return M
+
+    def download(self, M: BitMatrix) -> np.ndarray:
+        out = np.zeros((M.rows, M.ld), dtype=np.uint64)
+        if out.size:
+            _check(self.lib.mela_copy_d2h(self.handle, out.ctypes.data, M.data, out.nbytes))
+        return out
+
+    def gen_bernoulli(self, seed: int, M: BitMatrix):
+        _check(self.lib.mela_gen_bernoulli(self.handle, seed, C.byref(M)))
+
+    def matmul_packed(self, A: BitMatrix, B: BitMatrix, Cm: BitMatrix, algo: int = ALGO_AUTO):
+        _check(self.lib.mela_matmul_packed(self.handle, C.byref(A), C.byref(B), C.byref(Cm), algo))
+
+    def rank_packed(self, A: BitMatrix, variant: int = ELIM_M4RI) -> int:
+        r = C.c_size_t()
+        _check(self.lib.mela_rank_packed(self.handle, C.byref(A), variant, C.byref(r)))
+        return r.value
+
+    def rref_packed(self, A: BitMatrix, variant: int = ELIM_M4RI):
+        rmax = min(A.rows, A.cols)
+        R = self.bitmat(rmax, A.cols)
+        piv = (C.c_int * max(rmax, 1))()
+        r = C.c_size_t()
+        _check(self.lib.mela_rref_packed(self.handle, C.byref(A), variant, C.byref(R), piv,
+                                         C.byref(r)))
+        return R, [piv[i] for i in range(r.value)]
+
+    def null_packed(self, A: BitMatrix, variant: int = ELIM_M4RI) -> BitMatrix:
+        Z = BitMatrix()
+        f = C.c_size_t()
+        _check(self.lib.mela_null_packed(self.handle, C.byref(A), variant, C.byref(Z), C.byref(f)))
+        return Z
+
+
+_default = None
+_default_lock = threading.Lock()
+
+
+def default_context() -> Context:
+    global _default
+    with _default_lock:
+        if _default is None:
+            _default = Context(0)
+        return _default
+
+
+def _classify(A):
+    """(column-major buffer, class code, rows, cols) for a MATLAB-style input."""
+    if hasattr(A, "tocsc") and hasattr(A, "nnz"):  # scipy.sparse
+        shape = A.shape
+        code = MELA_DOUBLE if A.dtype == np.float64 else (MELA_LOGICAL if A.dtype == np.bool_ else MELA_OTHER)
+        return None, code | MELA_SPARSE, shape[0], shape[1]
+    A = np.asarray(A)
+    if A.ndim > 2:
+        raise MelaError("matrices must be 2-D")
+    if A.ndim < 2:
+        A = A.reshape((1, -1)) if A.ndim == 1 else A.reshape((1, 1))
+    if A.dtype == np.bool_:
+        buf = np.asfortranarray(A).view(np.uint8)
+        return buf, MELA_LOGICAL, A.shape[0], A.shape[1]
+    if A.dtype == np.float64:
+        return np.asfortranarray(A), MELA_DOUBLE, A.shape[0], A.shape[1]
+    return None, MELA_OTHER, A.shape[0], A.shape[1]
+
+
+def _ptr(buf):
+    return None if buf is None or buf.size == 0 else buf.ctypes.data
+
+
+def _matmul(fn_name, A, B, ctx):
+    ctx = ctx or default_context()
+    a, ac, m, k = _classify(A)
+    b, bc, k2, n = _classify(B)
+    out = np.zeros((m, n), dtype=np.uint8, order="F")
+    fn = getattr(ctx.lib, fn_name)
+    _check(fn(ctx.handle, _ptr(a), ac, m, k, _ptr(b), bc, k2, n, _ptr(out)))
+    return out.view(np.bool_)
+
+
+def mela_matmul_gf2(A, B, *more, ctx: Context | None = None):
+    """C = A*B over GF(2) (mela_matmul_gf2.c:72): logical m x n."""
+    if more:
+        raise MelaError("Need two input matrices A,B.")
+    return _matmul("mela_matmul_gf2", A, B, ctx)
+
+
+def mela_matmul_m4ri(A, B, *more, ctx: Context | None = None):
+    """C = A*B over GF(2) by the Method of Four Russians (mela_matmul_m4ri.c:140)."""
+    if more:
+        raise MelaError("Need two input matrices A,B.")
+    return _matmul("mela_matmul_m4ri", A, B, ctx)
+
+
+def _rank(fn_name, A, ctx):
+    ctx = ctx or default_context()
+    a, ac, m, n = _classify(A)
+    r = C.c_double()
+    _check(getattr(ctx.lib, fn_name)(ctx.handle, _ptr(a), ac, m, n, C.byref(r)))
+    return float(r.value)
+
+
+def mela_rank_gf2(A, *more, ctx: Context | None = None):
+    """rank over GF(2) (mela_rank_gf2.c:126), returned as a double."""
+    if more:
+        raise MelaError("One input required.", "bitrank_mex:nargin")
+    return _rank("mela_rank_gf2", A, ctx)
+
+
+def mela_rank_m4ri(A, *more, ctx: Context | None = None):
+    """rank over GF(2) by block M4RI (mela_rank_m4ri.c:286), returned as a double."""
+    if more:
+        raise MelaError("One input required.")
+    return _rank("mela_rank_m4ri", A, ctx)
+
+
+def _null(fn_name, H, ctx):
+    ctx = ctx or default_context()
+    h, hc, m, n = _classify(H)
+    holder = {}
+
+    def _alloc(_user, rows, cols):
+        arr = np.zeros((rows, cols), dtype=np.uint8, order="F")
+        holder["Z"] = arr
+        return arr.ctypes.data if arr.size else None
+
+    cb = ALLOC_FN(_alloc)
+    f = C.c_size_t()
+    _check(getattr(ctx.lib, fn_name)(ctx.handle, _ptr(h), hc, m, n, cb, None, C.byref(f)))
+    Z = holder.get("Z", np.zeros((n, f.value), dtype=np.uint8, order="F"))
+    return Z.view(np.bool_)
+
+
+def mela_null_gf2(H, *more, ctx: Context | None = None):
+    """canonical null-space basis over GF(2) (mela_null_gf2.c:142): logical n x (n-rank)."""
+    if more:
+        raise MelaError("One input required: H.")
+    return _null("mela_null_gf2", H, ctx)
+
+
+def mela_null_m4ri(H, *more, ctx: Context | None = None):
+    """canonical null-space basis over GF(2) via M4RI (mela_null_m4ri.c:251)."""
+    if more:
+        raise MelaError("One input required: H.")
+    return _null("mela_null_m4ri", H, ctx)

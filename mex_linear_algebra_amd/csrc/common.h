@@ -1,0 +1,95 @@
+// Internal declarations shared by the HIP sources of libmela.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "mela.h"
+
+namespace mela {
+
+// ------------------------------------------------------------- errors ----
+// Thread-local last error (message + MATLAB id), returned by mela_last_error.
+mela_status set_error(mela_status st, const std::string &msg, const char *id = "");
+mela_status hip_fail(hipError_t e, const char *what);
+
+#define MELA_HIP(call)                                             \
+    do {                                                           \
+        hipError_t e__ = (call);                                   \
+        if (e__ != hipSuccess) return ::mela::hip_fail(e__, #call); \
+    } while (0)
+
+inline size_t words64(size_t n) { return (n + 63) / 64; }
+
+// ------------------------------------------------------------ context ----
+// Device scratch is taken from a per-context cache of blocks that grows to the
+// largest request seen and is reused across calls (no hipMalloc on the
+// steady-state path).
+struct Scratch {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace mela
+
+struct mela_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;  // serialises calls on one context
+    std::vector<mela::Scratch> scratch;  // indexed by slot
+    int cu_count = 256;
+    hipEvent_t ev = nullptr;
+};
+
+namespace mela {
+
+// Returns a device buffer of at least `bytes` for `slot` (kept across calls).
+mela_status scratch_get(mela_ctx *ctx, int slot, size_t bytes, void **out);
+
+enum ScratchSlot {
+    SLOT_A = 0, SLOT_B, SLOT_C, SLOT_BT, SLOT_IN_A, SLOT_IN_B, SLOT_OUT,
+    SLOT_ELIM_M, SLOT_ELIM_Q, SLOT_ELIM_META, SLOT_ELIM_ACT, SLOT_ELIM_TAB,
+    SLOT_NULL_Z, SLOT_COUNT
+};
+
+// ------------------------------------------------- kernel launchers ----
+// pack.hip
+mela_status launch_pack(hipStream_t s, const void *src, int cls, size_t rows,
+                        size_t cols, uint64_t *dst, size_t ld);
+mela_status launch_unpack(hipStream_t s, const uint64_t *src, size_t ld,
+                          size_t rows, size_t cols, uint8_t *dst);
+mela_status launch_transpose(hipStream_t s, const uint64_t *src, size_t lds,
+                             size_t rows, size_t cols, uint64_t *dst, size_t ldd);
+mela_status launch_gen_bernoulli(hipStream_t s, uint64_t seed, uint64_t *dst,
+                                 size_t rows, size_t cols, size_t ld);
+mela_status launch_fill_zero(hipStream_t s, void *dst, size_t bytes);
+
+// matmul.hip
+mela_status matmul_device(mela_ctx *ctx, const uint64_t *A, size_t lda,
+                          const uint64_t *B, size_t ldb, uint64_t *C, size_t ldc,
+                          size_t m, size_t k, size_t n, int algo);
+
+// elim.hip
+struct ElimResult {
+    size_t rank = 0;
+    std::vector<int> pivots;  // filled when want_pivots
+};
+// Runs elimination on a packed copy owned by the call.  mode_rref: full RREF
+// (also reduces earlier pivot rows); Q receives pivot rows in order.
+mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m,
+                      size_t n, int variant, bool rref, bool want_pivots,
+                      ElimResult *res, uint64_t **Q_out, size_t *ldq_out);
+// logical column-major null basis (n x f bytes, device) from RREF rows
+mela_status launch_null_logical(hipStream_t s, const uint64_t *Q, size_t ldq,
+                                const int *rowmap_dev, const int *free_dev,
+                                size_t n, size_t f, uint8_t *Z);
+mela_status launch_null_packed(hipStream_t s, const uint64_t *Q, size_t ldq,
+                               const int *rowmap_dev, const int *free_dev,
+                               size_t n, size_t f, uint64_t *Z, size_t ldz);
+
+}  // namespace mela

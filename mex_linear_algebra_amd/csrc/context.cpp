@@ -1,0 +1,172 @@
+// Context, errors and device memory of libmela.
+#include <cstdio>
+#include <cstring>
+
+#include "common.h"
+
+namespace {
+thread_local std::string t_err;
+thread_local std::string t_err_id;
+}  // namespace
+
+namespace mela {
+
+mela_status set_error(mela_status st, const std::string &msg, const char *id) {
+    t_err = msg;
+    t_err_id = id ? id : "";
+    return st;
+}
+
+mela_status hip_fail(hipError_t e, const char *what) {
+    char buf[512];
+    std::snprintf(buf, sizeof buf, "HIP error %d (%s) in %s", (int)e, hipGetErrorString(e), what);
+    if (e == hipErrorOutOfMemory) return set_error(MELA_ERR_NOMEM, buf, "mela:memory");
+    return set_error(MELA_ERR_DEVICE, buf, "mela:device");
+}
+
+mela_status scratch_get(mela_ctx *ctx, int slot, size_t bytes, void **out) {
+    if (slot < 0 || slot >= SLOT_COUNT) return set_error(MELA_ERR_INVALID, "bad scratch slot");
+    if ((int)ctx->scratch.size() < SLOT_COUNT) ctx->scratch.resize(SLOT_COUNT);
+    Scratch &sc = ctx->scratch[slot];
+    if (bytes == 0) bytes = 16;
+    bytes = (bytes + 255) & ~(size_t)255;
+    if (sc.bytes < bytes) {
+        if (sc.ptr) {
+            // the previous block may still be read by queued work
+            hipError_t e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+            (void)hipFree(sc.ptr);
+            sc.ptr = nullptr;
+            sc.bytes = 0;
+        }
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return set_error(MELA_ERR_NOMEM, "Out of memory.", "mela:memory");
+        }
+        sc.ptr = p;
+        sc.bytes = bytes;
+    }
+    *out = sc.ptr;
+    return MELA_OK;
+}
+
+}  // namespace mela
+
+using namespace mela;
+
+extern "C" {
+
+const char *mela_last_error(void) { return t_err.c_str(); }
+const char *mela_last_error_id(void) { return t_err_id.c_str(); }
+const char *mela_version(void) { return "mela-mi355x 0.1 (gfx950)"; }
+
+mela_status mela_ctx_create(int device, mela_ctx **out) {
+    if (!out) return set_error(MELA_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0)
+        return set_error(MELA_ERR_DEVICE, "no HIP device available (libmela needs an MI355X)",
+                         "mela:device");
+    if (device < 0 || device >= ndev) return set_error(MELA_ERR_INVALID, "device index out of range");
+    MELA_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    MELA_HIP(hipGetDeviceProperties(&prop, device));
+    mela_ctx *ctx = new mela_ctx();
+    ctx->device = device;
+    ctx->cu_count = prop.multiProcessorCount;
+    e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return hip_fail(e, "hipStreamCreateWithFlags");
+    }
+    ctx->scratch.resize(SLOT_COUNT);
+    *out = ctx;
+    return MELA_OK;
+}
+
+void mela_ctx_destroy(mela_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &s : ctx->scratch)
+        if (s.ptr) (void)hipFree(s.ptr);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+void *mela_ctx_stream(mela_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+mela_status mela_ctx_sync(mela_ctx *ctx) {
+    if (!ctx) return set_error(MELA_ERR_INVALID, "null context");
+    MELA_HIP(hipStreamSynchronize(ctx->stream));
+    return MELA_OK;
+}
+
+mela_status mela_dev_alloc(mela_ctx *ctx, size_t bytes, void **dptr) {
+    if (!ctx || !dptr) return set_error(MELA_ERR_INVALID, "null argument");
+    MELA_HIP(hipSetDevice(ctx->device));
+    hipError_t e = hipMalloc(dptr, bytes ? bytes : 16);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *dptr = nullptr;
+        return set_error(MELA_ERR_NOMEM, "Out of memory.", "mela:memory");
+    }
+    return MELA_OK;
+}
+
+mela_status mela_dev_free(mela_ctx *ctx, void *dptr) {
+    if (!ctx) return set_error(MELA_ERR_INVALID, "null context");
+    if (dptr) {
+        MELA_HIP(hipStreamSynchronize(ctx->stream));
+        MELA_HIP(hipFree(dptr));
+    }
+    return MELA_OK;
+}
+
+mela_status mela_copy_h2d(mela_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx) return set_error(MELA_ERR_INVALID, "null context");
+    if (bytes == 0) return MELA_OK;
+    MELA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    MELA_HIP(hipStreamSynchronize(ctx->stream));
+    return MELA_OK;
+}
+
+mela_status mela_copy_d2h(mela_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx) return set_error(MELA_ERR_INVALID, "null context");
+    if (bytes == 0) return MELA_OK;
+    MELA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    MELA_HIP(hipStreamSynchronize(ctx->stream));
+    return MELA_OK;
+}
+
+mela_status mela_bitmat_alloc(mela_ctx *ctx, size_t rows, size_t cols, mela_bitmat *out) {
+    if (!ctx || !out) return set_error(MELA_ERR_INVALID, "null argument");
+    const size_t ld = words64(cols) ? words64(cols) : 1;
+    void *p = nullptr;
+    mela_status st = mela_dev_alloc(ctx, (rows ? rows : 1) * ld * 8, &p);
+    if (st != MELA_OK) return st;
+    st = launch_fill_zero(ctx->stream, p, (rows ? rows : 1) * ld * 8);
+    if (st != MELA_OK) return st;
+    out->data = (uint64_t *)p;
+    out->rows = rows;
+    out->cols = cols;
+    out->ld = ld;
+    return MELA_OK;
+}
+
+void mela_bitmat_free(mela_ctx *ctx, mela_bitmat *m) {
+    if (!m || !m->data) return;
+    (void)mela_dev_free(ctx, m->data);
+    m->data = nullptr;
+}
+
+mela_status mela_gen_bernoulli(mela_ctx *ctx, uint64_t seed, mela_bitmat *M) {
+    if (!ctx || !M || (!M->data && M->rows)) return set_error(MELA_ERR_INVALID, "null argument");
+    if (M->ld < words64(M->cols)) return set_error(MELA_ERR_INVALID, "ld smaller than the row width");
+    return launch_gen_bernoulli(ctx->stream, seed, M->data, M->rows, M->cols, M->ld);
+}
+
+}  // extern "C"

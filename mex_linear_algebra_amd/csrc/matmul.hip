@@ -1,0 +1,299 @@
+// GF(2) matrix product C = A * B on packed operands.
+//
+// Two exact kernels, both bit-identical to the reference cores
+// (matmul_gf2.c:110-208 and matmul_m4ri.c:178-248, whose result is A*B mod 2):
+//
+//  * k_andpop  -- C(i,j) = parity(popcount(A_i AND Bt_j)).  Per 32 k-bits one
+//    v_bitop3_b32 does the AND and the XOR-accumulate; the popcount parity is
+//    taken once per output bit at the end and the 64 bits of a word are
+//    assembled by a wave ballot.  Bound: VALU bitwise issue rate.
+//  * k_m4rm    -- Method of Four Russians.  For every 8-row strip of B the
+//    workgroup builds the 256 XOR-combinations of those rows (for its 2048
+//    output columns) in LDS; each output row then XORs one table row selected
+//    by the 8 matching bits of A.  Bound: LDS read bandwidth / VALU issue.
+#include "common.h"
+
+namespace mela {
+namespace {
+
+// ------------------------------------------------------------ AND+parity
+constexpr int AP_MR = 16;   // rows per wave (all lanes share them)
+constexpr int AP_NR = 4;    // 64-column slots per lane
+constexpr int AP_KC = 16;   // k-dwords per LDS chunk
+constexpr int AP_WAVES = 4;
+constexpr int AP_ROWS = AP_MR * AP_WAVES;  // 64 rows per workgroup
+constexpr int AP_COLS = 64 * AP_NR;        // 256 columns per workgroup
+
+// A: m x lda words (packed along k), Bt: n x ldb words (B transposed, packed
+// along k).  kd = ceil(k/32) dwords; bits past k are zero in both.
+__global__ void __launch_bounds__(256)
+k_andpop(const uint64_t *__restrict__ A, size_t lda, const uint64_t *__restrict__ Bt,
+         size_t ldb, uint64_t *__restrict__ C, size_t ldc, int m, int n, int kd) {
+    __shared__ uint32_t sA[2][AP_ROWS][AP_KC];
+    __shared__ uint32_t sB[2][AP_KC][64][AP_NR];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int i0 = blockIdx.y * AP_ROWS;
+    const int j0 = blockIdx.x * AP_COLS;
+    const uint32_t *A32 = reinterpret_cast<const uint32_t *>(A);
+    const uint32_t *B32 = reinterpret_cast<const uint32_t *>(Bt);
+    const size_t lda32 = lda * 2, ldb32 = ldb * 2;
+
+    // loader roles: A tile row = tid/4, 4 dwords at (tid%4)*4; B column = tid.
+    const int la_row = tid >> 2, la_off = (tid & 3) * 4;
+    const int lb_col = tid;
+    const bool a_ok = i0 + la_row < m;
+    const bool b_ok = j0 + lb_col < n;
+    const uint32_t *a_src = A32 + (size_t)(i0 + la_row) * lda32 + la_off;
+    const uint32_t *b_src = B32 + (size_t)(j0 + lb_col) * ldb32;
+
+    uint32_t ra[4], rb[AP_KC];
+    auto gload = [&](int kd0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            int d = kd0 + la_off + q;
+            ra[q] = (a_ok && d < kd) ? a_src[d] : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < AP_KC; ++q) {
+            int d = kd0 + q;
+            rb[q] = (b_ok && d < kd) ? b_src[d] : 0u;
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sA[buf][la_row][la_off + q] = ra[q];
+#pragma unroll
+        for (int q = 0; q < AP_KC; ++q) sB[buf][q][lb_col & 63][lb_col >> 6] = rb[q];
+    };
+
+    uint32_t acc[AP_MR][AP_NR];
+#pragma unroll
+    for (int r = 0; r < AP_MR; ++r)
+#pragma unroll
+        for (int c = 0; c < AP_NR; ++c) acc[r][c] = 0u;
+
+    const int nchunks = (kd + AP_KC - 1) / AP_KC;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int ch = 0; ch < nchunks; ++ch) {
+        const int buf = ch & 1;
+        if (ch + 1 < nchunks) gload((ch + 1) * AP_KC);
+#pragma unroll 1
+        for (int kk = 0; kk < AP_KC; kk += 4) {
+            uint4 bv[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                bv[q] = *reinterpret_cast<const uint4 *>(&sB[buf][kk + q][lane][0]);
+#pragma unroll
+            for (int r = 0; r < AP_MR; ++r) {
+                const uint4 av =
+                    *reinterpret_cast<const uint4 *>(&sA[buf][wave * AP_MR + r][kk]);
+                acc[r][0] ^= (av.x & bv[0].x) ^ (av.y & bv[1].x) ^ (av.z & bv[2].x) ^ (av.w & bv[3].x);
+                acc[r][1] ^= (av.x & bv[0].y) ^ (av.y & bv[1].y) ^ (av.z & bv[2].y) ^ (av.w & bv[3].y);
+                acc[r][2] ^= (av.x & bv[0].z) ^ (av.y & bv[1].z) ^ (av.z & bv[2].z) ^ (av.w & bv[3].z);
+                acc[r][3] ^= (av.x & bv[0].w) ^ (av.y & bv[1].w) ^ (av.z & bv[2].w) ^ (av.w & bv[3].w);
+            }
+        }
+        if (ch + 1 < nchunks) sstore(buf ^ 1);
+        __syncthreads();
+    }
+
+    const int nw = (n + 63) >> 6;
+#pragma unroll
+    for (int r = 0; r < AP_MR; ++r) {
+        const int i = i0 + wave * AP_MR + r;
+#pragma unroll
+        for (int c = 0; c < AP_NR; ++c) {
+            uint64_t word = __ballot(__popc(acc[r][c]) & 1);
+            const int w = (j0 >> 6) + c;
+            if (lane == r * AP_NR + c && i < m && w < nw) C[(size_t)i * ldc + w] = word;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ M4RM
+constexpr int M4_WAVES = 16;
+constexpr int M4_ROWS_PER_WAVE = 64;
+constexpr int M4_ROWS = M4_WAVES * M4_ROWS_PER_WAVE;  // 1024 rows per workgroup
+constexpr int M4_DW = 64;                             // dwords (2048 columns) per workgroup
+
+// Builds this wave's 16 entries (16*wave .. 16*wave+15) of the 256-entry table
+// of XOR-combinations of the 8 B rows held (for this lane's dword) in br[]:
+// the high nibble (= wave) selects rows 4..7 for the base, then a Gray-code
+// walk over the low nibble changes one row per entry (one XOR, one live temp).
+__device__ __forceinline__ void m4_build(uint32_t (*T)[M4_DW], const uint32_t br[8], int wave,
+                                         int lane) {
+    uint32_t cur = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+        if ((wave >> b) & 1) cur ^= br[4 + b];
+    T[wave * 16][lane] = cur;
+#pragma unroll
+    for (int t = 1; t < 16; ++t) {
+        cur ^= br[__builtin_ctz(t)];
+        T[wave * 16 + (t ^ (t >> 1))][lane] = cur;
+    }
+}
+
+// C tile: 1024 rows x 2048 columns; blockIdx.z selects a k-range [kb, ke) of
+// 64-bit A words (split-K partial products are XOR-combined with atomics).
+__global__ void __launch_bounds__(1024)
+k_m4rm(const uint64_t *__restrict__ A, size_t lda, const uint64_t *__restrict__ B, size_t ldb,
+       uint64_t *__restrict__ C, size_t ldc, int m, int n, int k, int kw_per_split, int atomic) {
+    __shared__ uint32_t T[2][256][M4_DW];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int row0 = blockIdx.y * M4_ROWS + wave * M4_ROWS_PER_WAVE;
+    const int dw = blockIdx.x * M4_DW + lane;  // this lane's output dword
+    const int ndw = ((n + 63) >> 6) * 2;  // dwords of the packed row width
+    const int kw_total = (k + 63) >> 6;
+    const int kwb = blockIdx.z * kw_per_split;
+    int kwe = kwb + kw_per_split;
+    if (kwe > kw_total) kwe = kw_total;
+    if (kwb >= kwe) return;
+    const int s_begin = kwb * 8;
+    const int s_end = min(kwe * 8, (k + 7) >> 3);  // 8-row strips
+
+    const uint32_t *B32 = reinterpret_cast<const uint32_t *>(B);
+    const size_t ldb32 = ldb * 2;
+    const bool col_ok = dw < ndw;
+
+    // lane r of this wave holds A row row0+r
+    const int arow = row0 + lane;
+    const bool arow_ok = arow < m;
+    const uint64_t *a_src = A + (size_t)arow * lda;
+
+    uint32_t acc[M4_ROWS_PER_WAVE];
+#pragma unroll
+    for (int r = 0; r < M4_ROWS_PER_WAVE; ++r) acc[r] = 0u;
+
+    // branch-free strip loads: clamp the address into B, then mask the value
+    const size_t dw_ld = col_ok ? (size_t)dw : (size_t)(ndw - 1);
+    const uint32_t col_mask = col_ok ? ~0u : 0u;
+    uint32_t br[8];
+    auto load_strip = [&](int s) {
+        const uint32_t *src = B32 + (size_t)s * 8 * ldb32 + dw_ld;
+        if (s * 8 + 8 <= k) {
+#pragma unroll
+            for (int b = 0; b < 8; ++b) br[b] = src[(size_t)b * ldb32] & col_mask;
+        } else {
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const int kr = s * 8 + b;
+                const int krc = kr < k ? kr : k - 1;
+                const uint32_t v = B32[(size_t)krc * ldb32 + dw_ld];
+                br[b] = (kr < k) ? (v & col_mask) : 0u;
+            }
+        }
+    };
+
+    load_strip(s_begin);
+    m4_build(T[0], br, wave, lane);
+    __syncthreads();
+
+    uint64_t aw = arow_ok ? a_src[kwb] : 0ull;
+    for (int s = s_begin; s < s_end; ++s) {
+        const int buf = (s - s_begin) & 1;
+        const bool more = s + 1 < s_end;
+        if (more) load_strip(s + 1);
+        const int sb = s & 7;
+        const uint32_t half = (sb < 4) ? (uint32_t)aw : (uint32_t)(aw >> 32);
+        const int sh = (sb & 3) * 8;
+        // groups of 8 lookups; the scheduling barrier stops the compiler from
+        // hoisting all 64 table addresses into registers at once
+#pragma unroll
+        for (int r0 = 0; r0 < M4_ROWS_PER_WAVE; r0 += 8) {
+#pragma unroll
+            for (int r = r0; r < r0 + 8; ++r) {
+                const uint32_t ar = __builtin_amdgcn_readlane(half, r);
+                const uint32_t idx = (ar >> sh) & 255u;
+                acc[r] ^= T[buf][idx][lane];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (sb == 7 && more) aw = arow_ok ? a_src[(s + 1) >> 3] : 0ull;
+        if (more) m4_build(T[buf ^ 1], br, wave, lane);
+        __syncthreads();
+    }
+
+    if (!col_ok) return;
+    uint32_t *C32 = reinterpret_cast<uint32_t *>(C);
+    const size_t ldc32 = ldc * 2;
+#pragma unroll
+    for (int r = 0; r < M4_ROWS_PER_WAVE; ++r) {
+        const int i = row0 + r;
+        if (i < m) {
+            if (atomic)
+                atomicXor(&C32[(size_t)i * ldc32 + dw], acc[r]);
+            else
+                C32[(size_t)i * ldc32 + dw] = acc[r];
+        }
+    }
+}
+
+}  // namespace
+
+mela_status matmul_device(mela_ctx *ctx, const uint64_t *A, size_t lda, const uint64_t *B,
+                          size_t ldb, uint64_t *C, size_t ldc, size_t m, size_t k, size_t n,
+                          int algo) {
+    hipStream_t s = ctx->stream;
+    const size_t nw = words64(n);
+    if (m == 0 || n == 0) return MELA_OK;
+    if (k == 0) {
+        // empty inner dimension: C = 0
+        for (size_t i = 0; i < 1; ++i) {
+        }
+        return launch_fill_zero(s, C, m * ldc * sizeof(uint64_t));
+    }
+    if (m > (size_t)INT32_MAX || n > (size_t)INT32_MAX || k > (size_t)INT32_MAX)
+        return set_error(MELA_ERR_INVALID, "matrix dimension exceeds 2^31-1");
+    if (algo == MELA_ALGO_AUTO) algo = MELA_ALGO_M4RM;
+
+    if (algo == MELA_ALGO_ANDPOP) {
+        // Bt = B^T (n x ceil(k/64) words)
+        const size_t ldbt = words64(k);
+        void *bt = nullptr;
+        mela_status st = scratch_get(ctx, SLOT_BT, n * ldbt * sizeof(uint64_t), &bt);
+        if (st != MELA_OK) return st;
+        st = launch_transpose(s, B, ldb, k, n, (uint64_t *)bt, ldbt);
+        if (st != MELA_OK) return st;
+        const int kd = (int)((k + 31) / 32);
+        dim3 grid((unsigned)((n + AP_COLS - 1) / AP_COLS), (unsigned)((m + AP_ROWS - 1) / AP_ROWS));
+        if (grid.y > 65535) return set_error(MELA_ERR_INVALID, "too many rows for one launch");
+        hipLaunchKernelGGL(k_andpop, grid, dim3(256), 0, s, A, lda, (const uint64_t *)bt, ldbt, C,
+                           ldc, (int)m, (int)n, kd);
+        MELA_HIP(hipGetLastError());
+        return MELA_OK;
+    }
+    if (algo == MELA_ALGO_M4RM) {
+        const int kw = (int)words64(k);
+        const unsigned gx = (unsigned)((n + 32 * M4_DW - 1) / (32 * M4_DW));
+        const unsigned gy = (unsigned)((m + M4_ROWS - 1) / M4_ROWS);
+        if (gy > 65535) return set_error(MELA_ERR_INVALID, "too many rows for one launch");
+        // split k until the grid covers the chip twice (XOR partials are exact
+        // and order-free, so the atomics cannot change the result)
+        int splits = 1;
+        const unsigned target = 2u * (unsigned)ctx->cu_count;
+        while (gx * gy * splits < target && splits * 2 <= kw && kw / (splits * 2) >= 8) splits *= 2;
+        const int per = (kw + splits - 1) / splits;
+        int atomic = splits > 1;
+        if (atomic) {
+            mela_status st = launch_fill_zero(s, C, m * ldc * sizeof(uint64_t));
+            if (st != MELA_OK) return st;
+        }
+        hipLaunchKernelGGL(k_m4rm, dim3(gx, gy, splits), dim3(1024), 0, s, A, lda, B, ldb, C, ldc,
+                           (int)m, (int)n, (int)k, per, atomic);
+        MELA_HIP(hipGetLastError());
+        (void)nw;
+        return MELA_OK;
+    }
+    return set_error(MELA_ERR_INVALID, "unknown matmul algorithm");
+}
+
+}  // namespace mela

@@ -1,0 +1,209 @@
+// Bit packing, unpacking, bit-matrix transposition and the seeded generator.
+//
+// Packed layout (shared with the reference, matmul_gf2.c:33-36): row-major,
+// `ld` 64-bit words per row, bit j of a row at word j>>6, bit j&63.
+#include "common.h"
+
+namespace mela {
+namespace {
+
+// ((int)x) & 1 with the x86-64 result for NaN/Inf/out-of-range (the
+// reference's cast, e.g. matmul_gf2.c:66): cvttsd2si returns 0x80000000 there,
+// whose low bit is 0.  v_cvt_i32_f64 would saturate instead (INT_MAX is odd),
+// so the range test is explicit.
+__device__ __forceinline__ uint32_t parity_of(double x) {
+    if (!(x > -2147483649.0 && x < 2147483648.0)) return 0u;
+    return (uint32_t)((int)x) & 1u;
+}
+
+constexpr int PACK_WPT = 4;  // packed words per thread
+
+// One thread per (row, 4 consecutive words).  Reads of a column are
+// coalesced across the rows of a wave; each thread writes 32 contiguous bytes.
+template <typename T>
+__global__ void __launch_bounds__(256) k_pack(const T *__restrict__ src, size_t rows,
+                                              size_t cols, uint64_t *__restrict__ dst,
+                                              size_t ld) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    size_t w0 = (size_t)blockIdx.y * PACK_WPT;
+    if (i >= rows) return;
+#pragma unroll
+    for (int q = 0; q < PACK_WPT; ++q) {
+        size_t w = w0 + q;
+        if (w >= ld) break;
+        uint64_t word = 0;
+        size_t j0 = w * 64;
+        if (j0 < cols) {
+            size_t jn = cols - j0 < 64 ? cols - j0 : 64;
+            const T *col = src + j0 * rows + i;
+            for (size_t b = 0; b < jn; ++b) {
+                uint64_t bit;
+                if constexpr (sizeof(T) == 8)
+                    bit = parity_of(col[b * rows]);
+                else
+                    bit = col[b * rows] != 0;
+                word |= bit << b;
+            }
+        }
+        dst[i * ld + w] = word;
+    }
+}
+
+// One thread per (row, word): 64 byte stores, each coalesced across the
+// consecutive rows of a wave (column-major logical output).
+__global__ void __launch_bounds__(256) k_unpack(const uint64_t *__restrict__ src, size_t ld,
+                                                size_t rows, size_t cols,
+                                                uint8_t *__restrict__ dst) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    size_t w = blockIdx.y;
+    if (i >= rows) return;
+    uint64_t word = src[i * ld + w];
+    size_t j0 = w * 64;
+    size_t jn = cols - j0 < 64 ? cols - j0 : 64;
+    uint8_t *out = dst + j0 * rows + i;
+    for (size_t b = 0; b < jn; ++b) out[b * rows] = (uint8_t)((word >> b) & 1u);
+}
+
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int s) {
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+    lo = __shfl_xor(lo, s, 64);
+    hi = __shfl_xor(hi, s, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// In-wave 64x64 bit transpose: lane l holds row l; afterwards lane c holds
+// column c (bit l = old row l, bit c).  Six butterfly levels swap the
+// off-diagonal s x s sub-blocks.
+__device__ __forceinline__ uint64_t transpose64(uint64_t x, int lane) {
+    const uint64_t masks[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull,
+                               0x00FF00FF00FF00FFull, 0x0F0F0F0F0F0F0F0Full,
+                               0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+    for (int lv = 0; lv < 6; ++lv) {
+        int s = 32 >> lv;
+        uint64_t m = masks[lv];
+        uint64_t y = shfl_xor64(x, s);
+        if ((lane & s) == 0)
+            x = (x & m) | ((y & m) << s);
+        else
+            x = (x & ~m) | ((y & ~m) >> s);
+    }
+    return x;
+}
+
+// dst (cols x rows, packed along rows) = transpose of src (rows x cols).
+// One wave per 64x64 bit block; blockIdx.x = row block, blockIdx.y = word.
+__global__ void __launch_bounds__(64) k_transpose(const uint64_t *__restrict__ src, size_t lds,
+                                                  size_t rows, size_t cols,
+                                                  uint64_t *__restrict__ dst, size_t ldd) {
+    int lane = threadIdx.x;
+    size_t bi = blockIdx.x, bw = blockIdx.y;
+    size_t r = bi * 64 + lane;
+    uint64_t x = r < rows ? src[r * lds + bw] : 0;
+    x = transpose64(x, lane);
+    size_t c = bw * 64 + lane;
+    if (c < cols) dst[c * ldd + bi] = x;
+}
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+    uint64_t z = x + 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+__global__ void __launch_bounds__(256) k_gen(uint64_t key, uint64_t *__restrict__ dst,
+                                             size_t rows, size_t cols, size_t ld) {
+    size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= rows * ld) return;
+    size_t i = idx / ld, w = idx % ld;
+    size_t wc = (cols + 63) / 64;
+    uint64_t v = 0;
+    if (w < wc) {
+        v = splitmix64(key + (uint64_t)(i * wc + w));
+        if (w == wc - 1 && (cols & 63)) v &= (1ull << (cols & 63)) - 1;
+    }
+    dst[idx] = v;
+}
+
+__global__ void __launch_bounds__(256) k_zero(uint4 *__restrict__ p, size_t n16) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * 256;
+    for (; i < n16; i += stride) p[i] = make_uint4(0, 0, 0, 0);
+}
+
+__global__ void k_zero_tail(uint8_t *__restrict__ p, size_t n) {
+    size_t i = threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+uint64_t host_splitmix64(uint64_t x) {
+    uint64_t z = x + 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+}  // namespace
+
+mela_status launch_pack(hipStream_t s, const void *src, int cls, size_t rows, size_t cols,
+                        uint64_t *dst, size_t ld) {
+    if (rows == 0 || ld == 0) return MELA_OK;
+    dim3 grid((unsigned)((rows + 255) / 256), (unsigned)((ld + PACK_WPT - 1) / PACK_WPT));
+    if (cls == MELA_DOUBLE)
+        hipLaunchKernelGGL(k_pack<double>, grid, dim3(256), 0, s, (const double *)src, rows,
+                           cols, dst, ld);
+    else
+        hipLaunchKernelGGL(k_pack<uint8_t>, grid, dim3(256), 0, s, (const uint8_t *)src, rows,
+                           cols, dst, ld);
+    MELA_HIP(hipGetLastError());
+    return MELA_OK;
+}
+
+mela_status launch_unpack(hipStream_t s, const uint64_t *src, size_t ld, size_t rows,
+                          size_t cols, uint8_t *dst) {
+    if (rows == 0 || cols == 0) return MELA_OK;
+    dim3 grid((unsigned)((rows + 255) / 256), (unsigned)words64(cols));
+    hipLaunchKernelGGL(k_unpack, grid, dim3(256), 0, s, src, ld, rows, cols, dst);
+    MELA_HIP(hipGetLastError());
+    return MELA_OK;
+}
+
+mela_status launch_transpose(hipStream_t s, const uint64_t *src, size_t lds, size_t rows,
+                             size_t cols, uint64_t *dst, size_t ldd) {
+    if (rows == 0 || cols == 0) return MELA_OK;
+    // every destination word (c, bi) with c < cols, bi < ceil(rows/64) is written
+    dim3 grid((unsigned)words64(rows), (unsigned)words64(cols));
+    hipLaunchKernelGGL(k_transpose, grid, dim3(64), 0, s, src, lds, rows, cols, dst, ldd);
+    MELA_HIP(hipGetLastError());
+    return MELA_OK;
+}
+
+mela_status launch_gen_bernoulli(hipStream_t s, uint64_t seed, uint64_t *dst, size_t rows,
+                                 size_t cols, size_t ld) {
+    size_t total = rows * ld;
+    if (total == 0) return MELA_OK;
+    hipLaunchKernelGGL(k_gen, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                       host_splitmix64(seed), dst, rows, cols, ld);
+    MELA_HIP(hipGetLastError());
+    return MELA_OK;
+}
+
+// Zero fill by kernel (not hipMemsetAsync, so callers may capture it in a graph).
+mela_status launch_fill_zero(hipStream_t s, void *dst, size_t bytes) {
+    if (bytes == 0) return MELA_OK;
+    size_t n16 = bytes / 16;
+    if (n16) {
+        size_t blocks = (n16 + 255) / 256;
+        if (blocks > 65536) blocks = 65536;
+        hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, s, (uint4 *)dst, n16);
+    }
+    size_t tail = bytes - n16 * 16;
+    if (tail)
+        hipLaunchKernelGGL(k_zero_tail, dim3(1), dim3(16), 0, s, (uint8_t *)dst + n16 * 16,
+                           tail);
+    MELA_HIP(hipGetLastError());
+    return MELA_OK;
+}
+
+}  // namespace mela
