@@ -15,7 +15,9 @@
  * ((int)x) & 1 (x86-64 semantics for NaN/Inf/out-of-range: parity 0).
  *
  * All functions are safe to call from several host threads on distinct
- * contexts; one context serialises its own calls.
+ * contexts; one context serialises its own calls.  Passing ctx == NULL to an
+ * entry point uses the library's default context (device 0, created on first
+ * use, after argument validation has passed).
  */
 #ifndef MELA_H
 #define MELA_H
@@ -54,6 +56,8 @@ typedef struct mela_ctx mela_ctx;
 
 /* ----------------------------------------------------------- context ---- */
 mela_status mela_ctx_create(int device, mela_ctx **ctx);
+/* the shared default context (device 0), created on first call */
+mela_status mela_ctx_default(mela_ctx **ctx);
 void mela_ctx_destroy(mela_ctx *ctx);
 /* HIP stream (hipStream_t) every kernel of this context runs on */
 void *mela_ctx_stream(mela_ctx *ctx);
@@ -158,6 +162,32 @@ mela_status mela_rref_packed(mela_ctx *ctx, const mela_bitmat *A, int variant,
  * (free with mela_bitmat_free). */
 mela_status mela_null_packed(mela_ctx *ctx, const mela_bitmat *A, int variant,
                              mela_bitmat *Z, size_t *nullity);
+
+/* ------------------------------------------------ batched rank / null space
+ *
+ * No reference counterpart (the reference handles one matrix per MEX call):
+ * rank, RREF and canonical null bases of up to millions of uneven matrices
+ * (each at most 4096 x 8192) in one persistent launch.  mela_batch_create
+ * plans the batch (work arenas, largest-first queue); mela_batch_run enqueues
+ * it (one hipGraph launch with MELA_BATCH_GRAPH); mela_batch_results waits and
+ * returns ranks and null-basis views (device memory owned by the batch, valid
+ * until mela_batch_destroy or the next run). */
+typedef struct mela_batch mela_batch;
+enum {
+    MELA_BATCH_NULL = 1,  /* also compute the packed null bases */
+    MELA_BATCH_GE = 2,    /* start every matrix in GE mode (default M4RI) */
+    MELA_BATCH_GRAPH = 4  /* capture the launch in a hipGraph and replay it */
+};
+mela_status mela_batch_create(mela_ctx *ctx, const mela_bitmat *mats, size_t count,
+                              int flags, mela_batch **batch);
+/* per-matrix elimination mode: 0 = GE, 1 = M4RI 4-bit tables */
+mela_status mela_batch_set_modes(mela_batch *batch, const int *modes);
+/* times candidate mode assignments on the batch itself and keeps the fastest;
+ * *best_ms receives its time (may be NULL) */
+mela_status mela_batch_autotune(mela_batch *batch, float *best_ms);
+mela_status mela_batch_run(mela_batch *batch);
+mela_status mela_batch_results(mela_batch *batch, uint32_t *ranks, mela_bitmat *nulls);
+void mela_batch_destroy(mela_batch *batch);
 
 #ifdef __cplusplus
 }

@@ -26,6 +26,12 @@ bool mul_overflows(size_t a, size_t b, size_t *out) {
     return false;
 }
 
+// ctx == NULL selects the default context (after validation has passed)
+mela_status resolve(mela_ctx **ctx) {
+    if (*ctx) return MELA_OK;
+    return mela_ctx_default(ctx);
+}
+
 struct Guard {
     std::lock_guard<std::mutex> lk;
     explicit Guard(mela_ctx *c) : lk(c->mu) { (void)hipSetDevice(c->device); }
@@ -74,12 +80,13 @@ mela_status matmul_entry(mela_ctx *ctx, const void *A, int ac, size_t m, size_t 
     if (k != k2) return set_error(MELA_ERR_DIM, "Inner dimensions must agree.");
     if (!is_supported(ac) || !is_supported(bc))
         return set_error(MELA_ERR_TYPE, "Inputs must be logical or double matrices.");
-    if (!ctx) return set_error(MELA_ERR_INVALID, "null context");
     size_t mn;
     if (mul_overflows(m, n, &mn)) return set_error(MELA_ERR_INVALID, "output too large");
     if ((m * k && !A) || (k * n && !B) || (mn && !C))
         return set_error(MELA_ERR_INVALID, "null data pointer");
     if (mn == 0) return MELA_OK;
+    mela_status st0 = resolve(&ctx);
+    if (st0 != MELA_OK) return st0;
     Guard g(ctx);
     uint64_t *Ap, *Bp;
     size_t lda, ldb;
@@ -111,10 +118,12 @@ mela_status rank_entry(mela_ctx *ctx, const void *A, int ac, size_t m, size_t n,
     // the reference rank gateways do not test for sparse storage; reading it
     // as dense data is not reproducible, so it is rejected like matmul/null
     if (ac & MELA_SPARSE) return set_error(MELA_ERR_SPARSE, kSparseMsg);
-    if (!ctx || !rank) return set_error(MELA_ERR_INVALID, "null argument");
+    if (!rank) return set_error(MELA_ERR_INVALID, "null argument");
     *rank = 0.0;
     if (m * n && !A) return set_error(MELA_ERR_INVALID, "null data pointer");
     if (m == 0 || n == 0) return MELA_OK;
+    mela_status st0 = resolve(&ctx);
+    if (st0 != MELA_OK) return st0;
     Guard g(ctx);
     uint64_t *Ap;
     size_t lda;
@@ -168,8 +177,10 @@ mela_status null_entry(mela_ctx *ctx, const void *H, int hc, size_t m, size_t n,
     // null_gf2.c:151-156: sparse, then class
     if (hc & MELA_SPARSE) return set_error(MELA_ERR_SPARSE, kSparseMsg);
     if (!is_supported(hc)) return set_error(MELA_ERR_TYPE, "H must be double or logical.");
-    if (!ctx || !alloc) return set_error(MELA_ERR_INVALID, "null argument");
+    if (!alloc) return set_error(MELA_ERR_INVALID, "null argument");
     if (m * n && !H) return set_error(MELA_ERR_INVALID, "null data pointer");
+    mela_status st0 = resolve(&ctx);
+    if (st0 != MELA_OK) return st0;
     Guard g(ctx);
     std::vector<int> piv;
     ElimResult res;

@@ -1,0 +1,550 @@
+// Batched rank / RREF / canonical null basis for thousands of uneven matrices
+// in one persistent launch (no reference counterpart: the reference handles
+// one matrix per MEX call, e.g. null_gf2.c:142).
+//
+// Every workgroup loops: take the next matrix from a device queue (largest
+// first), copy it into its working slot, eliminate it panel by panel entirely
+// inside the workgroup (panel select -> pivot rows -> table or direct XOR
+// sweep -> earlier pivot rows), then write its rank and its packed null
+// basis.  Per-matrix mode (chosen by the autotuner): GE (XOR each selected
+// pivot row) or M4RI with 4-bit tables (16 table lookups per row word).
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "elim_device.h"
+
+namespace mela {
+
+struct BatchMat {
+    uint64_t a_off;  // words: input A (m x lda) in the input pointer table
+    uint64_t w_off;  // words: working copy (m x wc) in the work arena
+    uint64_t q_off;  // words: pivot rows (min(m,n) x wc) in the work arena
+    uint64_t z_off;  // words: packed null basis (n x ldz) in the Z arena
+    uint32_t m, n;
+    uint32_t lda, ldz;
+    uint32_t mode;  // 0 = GE, 1 = M4RI (4-bit tables)
+    uint32_t pad;
+};
+
+namespace {
+
+constexpr int BT = 256;          // threads per workgroup
+constexpr int BT_WAVES = BT / 64;
+constexpr int TW = 16;           // words per update tile
+constexpr int MAX_M = 4096;
+constexpr int MAX_N = 8192;
+constexpr int MAX_WC = MAX_N / 64;
+
+struct BatchLDS {
+    dev::SelectLDS<BT> sel;
+    uint64_t comb[64];
+    int selrow[64];
+    uint16_t act[MAX_M];
+    uint64_t pmask[MAX_WC];  // pivot column bitmap
+    int pbefore[MAX_WC];     // pivots in columns below 64*w
+    union {
+        uint64_t T[16][16][TW];    // M4RI nibble tables (elimination)
+        uint64_t P[64][TW];        // GE pivot rows (elimination)
+        uint16_t freecol[MAX_N];   // free columns ascending (null extraction)
+    } u;
+    int s_idx;
+    int s_r, s_nact;
+};
+
+__device__ __forceinline__ int pivots_before(const BatchLDS &L, int p) {
+    const int w = p >> 6;
+    return L.pbefore[w] + dev::popc64(L.pmask[w] & ((1ull << (p & 63)) - 1ull));
+}
+
+__global__ void __launch_bounds__(BT)
+k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order, int count,
+             int *__restrict__ counter, const uint64_t *const *__restrict__ inputs,
+             uint64_t *__restrict__ work, uint64_t *__restrict__ zarena, uint32_t *__restrict__ ranks,
+             int want_null) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    BatchLDS &L = *reinterpret_cast<BatchLDS *>(smem);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+    for (;;) {
+        if (tid == 0) L.s_idx = atomicAdd(counter, 1);
+        __syncthreads();
+        const int qi = L.s_idx;
+        if (qi >= count) break;
+        const int mi = order[qi];
+        const BatchMat d = mats[mi];
+        const int m = (int)d.m, n = (int)d.n;
+        const int wc = (n + 63) >> 6;
+        const int rmax = m < n ? m : n;
+        uint64_t *M = work + d.w_off;
+        uint64_t *Q = work + d.q_off;
+        const uint64_t *A = inputs[mi];
+
+        for (int i = tid; i < m * wc; i += BT) {
+            const int r = i / wc, w = i - r * wc;
+            M[i] = A[(size_t)r * d.lda + w];
+        }
+        for (int i = tid; i < m; i += BT) L.act[i] = (uint16_t)i;
+        for (int i = tid; i < wc; i += BT) L.pmask[i] = 0;
+        if (tid == 0) {
+            L.s_r = 0;
+            L.s_nact = m;
+        }
+        __syncthreads();
+
+        for (int g = 0; g < wc; ++g) {
+            const int n_act = L.s_nact, r0 = L.s_r;
+            if (n_act == 0 || r0 >= rmax) break;
+            auto rowfn = [&](int q) { return M + (size_t)q * wc; };
+            const dev::PanelOut po =
+                dev::wg_panel_select<BT>(L.sel, L.act, n_act, g, rowfn, L.comb, L.selrow);
+            const uint64_t pm = po.pm;
+            if (pm == 0) continue;  // no pivot in this panel (uniform)
+            const int rho = po.rho;
+            // pivot rows Q[r0 + j], words g .. wc-1 (words below g are zero)
+            for (int i = tid; i < rho * wc; i += BT) {
+                const int j = i / wc, w = i - j * wc;
+                uint64_t t = pm;
+                for (int q = 0; q < j; ++q) t &= t - 1;
+                const int c = dev::ffs64(t);
+                uint64_t acc = 0;
+                if (w >= g)
+                    for (uint64_t s = L.comb[c]; s; s &= s - 1)
+                        acc ^= M[(size_t)L.selrow[dev::ffs64(s)] * wc + w];
+                Q[(size_t)(r0 + j) * wc + w] = acc;
+            }
+            if (tid == 0) {
+                L.pmask[g] = pm;
+                L.s_r = r0 + rho;
+                L.s_nact = po.n_act_new;
+            }
+            __syncthreads();
+            // sweep: active rows and (for the null basis) earlier pivot rows
+            const int nact = po.n_act_new;
+            const int total = nact + (want_null ? r0 : 0);
+            // 4 rows per wave pass, 16 lanes (words) per row
+            const int sub = lane >> 4, wl = lane & 15;
+            for (int w0 = g + 1; w0 < wc; w0 += TW) {
+                const int w = w0 + wl;
+                const bool wok = w < wc;
+                if (d.mode == 1) {
+                    for (int e = tid; e < 16 * TW; e += BT) {
+                        const int t = e / TW, ww = e - t * TW;
+                        uint64_t q[4];
+#pragma unroll
+                        for (int b = 0; b < 4; ++b) {
+                            const int c = 4 * t + b;
+                            q[b] = 0;
+                            if (w0 + ww < wc && ((pm >> c) & 1ull))
+                                q[b] = Q[(size_t)(r0 + dev::popc64(pm & ((1ull << c) - 1ull))) * wc +
+                                         w0 + ww];
+                        }
+                        uint64_t cur = 0;
+                        L.u.T[t][0][ww] = 0;
+#pragma unroll
+                        for (int v = 1; v < 16; ++v) {
+                            cur ^= q[__builtin_ctz(v)];
+                            L.u.T[t][v ^ (v >> 1)][ww] = cur;
+                        }
+                    }
+                } else {
+                    for (int e = tid; e < 64 * TW; e += BT) {
+                        const int c = e / TW, ww = e - c * TW;
+                        uint64_t v = 0;
+                        if (w0 + ww < wc && ((pm >> c) & 1ull))
+                            v = Q[(size_t)(r0 + dev::popc64(pm & ((1ull << c) - 1ull))) * wc + w0 + ww];
+                        L.u.P[c][ww] = v;
+                    }
+                }
+                __syncthreads();
+                for (int q0 = wave * 4 + sub; q0 < total; q0 += BT_WAVES * 4) {
+                    uint64_t *rp = q0 < nact ? M + (size_t)L.act[q0] * wc : Q + (size_t)(q0 - nact) * wc;
+                    uint64_t x = rp[g] & pm;
+                    if (x == 0 || !wok) continue;
+                    uint64_t a = rp[w];
+                    if (d.mode == 1) {
+#pragma unroll
+                        for (int t = 0; t < 16; ++t) a ^= L.u.T[t][(x >> (4 * t)) & 15u][wl];
+                    } else {
+                        while (x) {
+                            a ^= L.u.P[dev::ffs64(x)][wl];
+                            x &= x - 1;
+                        }
+                    }
+                    rp[w] = a;
+                }
+                __syncthreads();
+            }
+            // word g of the earlier pivot rows (after every tile read it)
+            if (want_null) {
+                for (int q = tid; q < r0; q += BT) {
+                    const uint64_t x = Q[(size_t)q * wc + g];
+                    uint64_t a = x, h = x & pm;
+                    while (h) {
+                        const int c = dev::ffs64(h);
+                        h &= h - 1;
+                        a ^= Q[(size_t)(r0 + dev::popc64(pm & ((1ull << c) - 1ull))) * wc + g];
+                    }
+                    Q[(size_t)q * wc + g] = a;
+                }
+            }
+            __syncthreads();
+        }
+
+        const int r = L.s_r;
+        if (tid == 0) ranks[mi] = (uint32_t)r;
+        if (want_null) {
+            const int f = n - r;
+            // prefix pivot counts per word, free column list
+            if (tid == 0) {
+                int acc = 0;
+                for (int w = 0; w < wc; ++w) {
+                    L.pbefore[w] = acc;
+                    acc += dev::popc64(L.pmask[w]);
+                }
+            }
+            __syncthreads();
+            for (int p = tid; p < n; p += BT)
+                if (!((L.pmask[p >> 6] >> (p & 63)) & 1ull)) L.u.freecol[p - pivots_before(L, p)] = (uint16_t)p;
+            __syncthreads();
+            uint64_t *Z = zarena + d.z_off;
+            const int ldz = (int)d.ldz;
+            const int fw = (f + 63) >> 6;
+            for (int i = tid; i < n * ldz; i += BT) {
+                const int p = i / ldz, jw = i - p * ldz;
+                uint64_t word = 0;
+                if (jw < fw) {
+                    const bool isp = (L.pmask[p >> 6] >> (p & 63)) & 1ull;
+                    const int pb = pivots_before(L, p);
+                    if (isp) {
+                        const uint64_t *qrow = Q + (size_t)pb * wc;
+                        const int jn = f - jw * 64 < 64 ? f - jw * 64 : 64;
+                        for (int b = 0; b < jn; ++b) {
+                            const int fc = L.u.freecol[jw * 64 + b];
+                            word |= ((qrow[fc >> 6] >> (fc & 63)) & 1ull) << b;
+                        }
+                    } else {
+                        const int fi = p - pb;
+                        if ((fi >> 6) == jw) word = 1ull << (fi & 63);
+                    }
+                }
+                Z[i] = word;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_reset_counter(int *c) { *c = 0; }
+
+// ones[i] += popcount of matrix i (one block per (matrix, row chunk))
+__global__ void __launch_bounds__(256)
+k_batch_popcount(const BatchMat *__restrict__ mats, const uint64_t *const *__restrict__ inputs,
+                 unsigned long long *__restrict__ ones) {
+    const BatchMat d = mats[blockIdx.x];
+    const int wc = (int)((d.n + 63) >> 6);
+    const uint64_t *A = inputs[blockIdx.x];
+    unsigned long long acc = 0;
+    for (int i = threadIdx.x; i < (int)d.m * wc; i += 256) {
+        const int r = i / wc, w = i - r * wc;
+        acc += (unsigned long long)__popcll(A[(size_t)r * d.lda + w]);
+    }
+    atomicAdd(&ones[blockIdx.x], acc);
+}
+
+}  // namespace
+
+size_t batch_lds_bytes() { return sizeof(BatchLDS); }
+
+}  // namespace mela
+
+// ----------------------------------------------------------------- host side
+struct mela_batch {
+    mela_ctx *ctx = nullptr;
+    size_t count = 0;
+    int flags = 0;
+    std::vector<mela::BatchMat> mats;
+    std::vector<int> order;
+    std::vector<const uint64_t *> inputs;
+    mela::BatchMat *d_mats = nullptr;
+    int *d_order = nullptr;
+    int *d_counter = nullptr;
+    const uint64_t **d_inputs = nullptr;
+    uint64_t *d_work = nullptr;
+    uint64_t *d_z = nullptr;
+    uint32_t *d_ranks = nullptr;
+    size_t work_words = 0, z_words = 0;
+    int grid = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int tuned = -1;  // index of the autotuned mode assignment
+};
+
+namespace mela {
+mela_status batch_launch_raw(mela_batch *b, hipStream_t s);
+}
+
+using namespace mela;
+
+namespace mela {
+
+mela_status batch_launch_raw(mela_batch *b, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, b->d_counter);
+    hipLaunchKernelGGL(k_batch_rref, dim3(b->grid), dim3(BT), batch_lds_bytes(), s, b->d_mats,
+                       b->d_order, (int)b->mats.size(), b->d_counter, b->d_inputs, b->d_work, b->d_z, b->d_ranks,
+                       (b->flags & MELA_BATCH_NULL) ? 1 : 0);
+    MELA_HIP(hipGetLastError());
+    return MELA_OK;
+}
+
+}  // namespace mela
+
+namespace {
+
+void free_batch_device(mela_batch *b) {
+    void *ptrs[] = {b->d_mats, b->d_order, b->d_counter, b->d_inputs, b->d_work, b->d_z, b->d_ranks};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+}
+
+template <typename T>
+mela_status dalloc(T **p, size_t count) {
+    hipError_t e = hipMalloc((void **)p, (count ? count : 1) * sizeof(T));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return set_error(MELA_ERR_NOMEM, "Out of memory.", "mela:memory");
+    }
+    return MELA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+mela_status mela_batch_create(mela_ctx *ctx, const mela_bitmat *mats, size_t count, int flags,
+                              mela_batch **out) {
+    if (!ctx || !out || (count && !mats)) return set_error(MELA_ERR_INVALID, "null argument");
+    *out = nullptr;
+    (void)hipSetDevice(ctx->device);
+    static bool attr_set = false;
+    if (!attr_set) {
+        MELA_HIP(hipFuncSetAttribute((const void *)k_batch_rref,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)batch_lds_bytes()));
+        attr_set = true;
+    }
+    mela_batch *b = new mela_batch();
+    b->ctx = ctx;
+    b->count = count;
+    b->flags = flags;
+    size_t work = 0, z = 0;
+    for (size_t i = 0; i < count; ++i) {
+        const mela_bitmat &M = mats[i];
+        if ((M.cols && M.ld < words64(M.cols)) || (M.rows && M.cols && !M.data)) {
+            delete b;
+            return set_error(MELA_ERR_INVALID, "bad matrix descriptor in batch");
+        }
+        if (M.rows > (size_t)MAX_M || M.cols > (size_t)MAX_N) {
+            delete b;
+            return set_error(MELA_ERR_INVALID,
+                             "batch matrices are limited to 4096 x 8192; use the single-matrix "
+                             "entry points for larger ones");
+        }
+        BatchMat d{};
+        d.m = (uint32_t)M.rows;
+        d.n = (uint32_t)M.cols;
+        d.lda = (uint32_t)M.ld;
+        const size_t wc = words64(M.cols);
+        const size_t rmax = std::min(M.rows, M.cols);
+        d.w_off = work;
+        work += M.rows * wc;
+        d.q_off = work;
+        work += rmax * wc;
+        d.ldz = (uint32_t)wc;
+        d.z_off = z;
+        z += M.cols * wc;
+        d.mode = (flags & MELA_BATCH_GE) ? 0 : 1;
+        b->mats.push_back(d);
+        b->inputs.push_back(M.data);
+    }
+    // queue order: largest work first (m * wc^2 ~ sweep cost)
+    b->order.resize(b->mats.size());
+    std::iota(b->order.begin(), b->order.end(), 0);
+    std::sort(b->order.begin(), b->order.end(), [&](int x, int y) {
+        const double cx = (double)b->mats[x].m * words64(b->mats[x].n) * words64(b->mats[x].n);
+        const double cy = (double)b->mats[y].m * words64(b->mats[y].n) * words64(b->mats[y].n);
+        return cx > cy;
+    });
+    b->work_words = work;
+    b->z_words = z;
+    const size_t nb = b->mats.size();
+    mela_status st;
+    if ((st = dalloc(&b->d_mats, nb)) || (st = dalloc(&b->d_order, nb)) ||
+        (st = dalloc(&b->d_counter, 1)) || (st = dalloc(&b->d_inputs, nb)) ||
+        (st = dalloc(&b->d_work, work)) || (st = dalloc(&b->d_z, z)) ||
+        (st = dalloc(&b->d_ranks, nb))) {
+        free_batch_device(b);
+        delete b;
+        return st;
+    }
+    hipStream_t s = ctx->stream;
+    if (nb) {
+        MELA_HIP(hipMemcpyAsync(b->d_mats, b->mats.data(), nb * sizeof(BatchMat), hipMemcpyHostToDevice, s));
+        MELA_HIP(hipMemcpyAsync(b->d_order, b->order.data(), nb * sizeof(int), hipMemcpyHostToDevice, s));
+        MELA_HIP(hipMemcpyAsync(b->d_inputs, b->inputs.data(), nb * sizeof(void *), hipMemcpyHostToDevice, s));
+    }
+    int occ = 0;
+    MELA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)k_batch_rref, BT,
+                                                          batch_lds_bytes()));
+    if (occ < 1) occ = 1;
+    b->grid = std::max(1, std::min<int>((int)nb, occ * ctx->cu_count));
+    MELA_HIP(hipStreamSynchronize(s));
+    *out = b;
+    return MELA_OK;
+}
+
+mela_status mela_batch_set_modes(mela_batch *b, const int *modes) {
+    if (!b || !modes) return set_error(MELA_ERR_INVALID, "null argument");
+    for (size_t i = 0; i < b->count; ++i) b->mats[i].mode = modes[i] ? 1u : 0u;
+    MELA_HIP(hipMemcpyAsync(b->d_mats, b->mats.data(), b->mats.size() * sizeof(BatchMat),
+                            hipMemcpyHostToDevice, b->ctx->stream));
+    MELA_HIP(hipStreamSynchronize(b->ctx->stream));
+    return MELA_OK;
+}
+
+mela_status mela_batch_autotune(mela_batch *b, float *best_ms) {
+    if (!b) return set_error(MELA_ERR_INVALID, "null batch");
+    const size_t nb = b->mats.size();
+    if (nb == 0) {
+        if (best_ms) *best_ms = 0.f;
+        return MELA_OK;
+    }
+    mela_ctx *ctx = b->ctx;
+    hipStream_t s = ctx->stream;
+    // per-matrix density (sparse parity-check matrices favour GE: few pivot
+    // bits per row word; dense ones favour the 4-bit tables)
+    std::vector<double> dens(nb);
+    {
+        unsigned long long *d_ones = nullptr;
+        mela_status st = dalloc(&d_ones, nb);
+        if (st != MELA_OK) return st;
+        std::vector<unsigned long long> zero(nb, 0ull);
+        MELA_HIP(hipMemcpyAsync(d_ones, zero.data(), nb * 8, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_batch_popcount, dim3((unsigned)nb), dim3(256), 0, s, b->d_mats,
+                           b->d_inputs, d_ones);
+        MELA_HIP(hipGetLastError());
+        MELA_HIP(hipMemcpyAsync(zero.data(), d_ones, nb * 8, hipMemcpyDeviceToHost, s));
+        MELA_HIP(hipStreamSynchronize(s));
+        (void)hipFree(d_ones);
+        for (size_t i = 0; i < nb; ++i)
+            dens[i] = (double)zero[i] / std::max(1.0, (double)b->mats[i].m * b->mats[i].n);
+    }
+    std::vector<std::vector<int>> cands;
+    auto add = [&](auto pred) {
+        std::vector<int> md(nb);
+        for (size_t i = 0; i < nb; ++i) md[i] = pred(i) ? 1 : 0;
+        cands.push_back(md);
+    };
+    add([&](size_t) { return false; });                               // all GE
+    add([&](size_t) { return true; });                                // all M4RI
+    add([&](size_t i) { return dens[i] > 0.05; });                     // dense -> M4RI
+    add([&](size_t i) { return dens[i] > 0.05 && b->mats[i].n > 512; });
+    add([&](size_t i) { return b->mats[i].n > 1024; });
+    hipEvent_t e0, e1;
+    MELA_HIP(hipEventCreate(&e0));
+    MELA_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    size_t bi = 0;
+    for (size_t c = 0; c < cands.size(); ++c) {
+        mela_status st = mela_batch_set_modes(b, cands[c].data());
+        if (st != MELA_OK) return st;
+        float tot = 0.f;
+        for (int rep = 0; rep < 2; ++rep) {
+            MELA_HIP(hipEventRecord(e0, s));
+            if ((st = batch_launch_raw(b, s)) != MELA_OK) return st;
+            MELA_HIP(hipEventRecord(e1, s));
+            MELA_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            MELA_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (rep) tot = ms;  // second run (warm)
+        }
+        if (tot < best) {
+            best = tot;
+            bi = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    mela_status st = mela_batch_set_modes(b, cands[bi].data());
+    if (st != MELA_OK) return st;
+    b->tuned = (int)bi;
+    if (best_ms) *best_ms = best;
+    return MELA_OK;
+}
+
+mela_status mela_batch_run(mela_batch *b) {
+    if (!b) return set_error(MELA_ERR_INVALID, "null batch");
+    mela_ctx *ctx = b->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    if (!b->mats.empty()) {
+        if (b->flags & MELA_BATCH_GRAPH) {
+            if (!b->exec) {
+                MELA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                mela_status st = batch_launch_raw(b, s);
+                hipGraph_t g = nullptr;
+                hipError_t e = hipStreamEndCapture(s, &g);
+                if (st != MELA_OK) return st;
+                if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+                b->graph = g;
+                MELA_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+            }
+            MELA_HIP(hipGraphLaunch(b->exec, s));
+        } else {
+            mela_status st = batch_launch_raw(b, s);
+            if (st != MELA_OK) return st;
+        }
+    }
+    return MELA_OK;
+}
+
+mela_status mela_batch_results(mela_batch *b, uint32_t *ranks, mela_bitmat *nulls) {
+    if (!b) return set_error(MELA_ERR_INVALID, "null batch");
+    mela_ctx *ctx = b->ctx;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        MELA_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    std::vector<uint32_t> r(b->mats.size());
+    if (!r.empty())
+        MELA_HIP(hipMemcpy(r.data(), b->d_ranks, r.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < b->count; ++i) {
+        if (ranks) ranks[i] = r[i];
+        if (nulls) {
+            const BatchMat &d = b->mats[i];
+            nulls[i].data = b->d_z + d.z_off;
+            nulls[i].rows = d.n;
+            nulls[i].cols = d.n - r[i];
+            nulls[i].ld = d.ldz ? d.ldz : 1;
+        }
+    }
+    return MELA_OK;
+}
+
+void mela_batch_destroy(mela_batch *b) {
+    if (!b) return;
+    if (b->ctx) {
+        (void)hipSetDevice(b->ctx->device);
+        (void)hipStreamSynchronize(b->ctx->stream);
+    }
+    free_batch_device(b);
+    delete b;
+}
+
+}  // extern "C"

@@ -87,6 +87,19 @@ mela_status mela_ctx_create(int device, mela_ctx **out) {
     return MELA_OK;
 }
 
+mela_status mela_ctx_default(mela_ctx **out) {
+    static std::mutex mu;
+    static mela_ctx *dflt = nullptr;
+    if (!out) return set_error(MELA_ERR_INVALID, "null output pointer");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!dflt) {
+        mela_status st = mela_ctx_create(0, &dflt);
+        if (st != MELA_OK) return st;
+    }
+    *out = dflt;
+    return MELA_OK;
+}
+
 void mela_ctx_destroy(mela_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);

@@ -1,0 +1,206 @@
+// Workgroup-level building blocks of GF(2) elimination, shared by the
+// single-matrix kernels (elim.hip) and the batched persistent kernel
+// (batch.hip).
+#pragma once
+
+#include "common.h"
+
+namespace mela {
+namespace dev {
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
+    uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l);
+    uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ int popc64(uint64_t x) { return __popcll(x); }
+__device__ __forceinline__ int ffs64(uint64_t x) { return __ffsll((unsigned long long)x) - 1; }
+
+// Wave-level Gauss-Jordan over the 64-bit words v (one per lane; lanes with
+// valid == false take no part).  Afterwards pivot lanes hold the RREF basis of
+// span{v} (pbit = pivot bit), non-pivot lanes hold 0; comb (if tracked) is the
+// set of original lanes XORed into each lane.  Pivot of column c = the lowest
+// non-pivot lane with bit c set.
+template <bool TRACK>
+__device__ __forceinline__ void wave_gj(uint64_t &v, uint64_t &comb, bool valid, bool &piv,
+                                        int &pbit, int lane) {
+    piv = false;
+    pbit = -1;
+    if (!valid) v = 0;
+    for (int c = 0; c < 64; ++c) {
+        const bool has = ((v >> c) & 1ull) != 0;
+        const uint64_t bal = __ballot(has && !piv);
+        if (bal == 0) continue;
+        const int f = ffs64(bal);
+        const uint64_t pv = readlane64(v, f);
+        uint64_t pc = 0;
+        if (TRACK) pc = readlane64(comb, f);
+        if (lane == f) {
+            piv = true;
+            pbit = c;
+        } else if (has) {
+            v ^= pv;
+            if (TRACK) comb ^= pc;
+        }
+    }
+}
+
+// LDS state of one panel selection.
+template <int NT>
+struct SelectLDS {
+    uint64_t bas[64];  // RREF basis of the panel, indexed by pivot bit
+    uint64_t cy[NT];   // candidate words of the current chunk
+    int cpos[NT];      // their positions in the active list
+    int wcnt[NT / 64];
+    int selpos[64];
+    int selrow[64];
+    int flag[64];
+    uint32_t mv[64];
+    unsigned long long pm;
+    int nsel;
+};
+
+// Result of a panel selection (valid on every thread after the call).
+struct PanelOut {
+    uint64_t pm;        // pivot bits of the panel
+    int rho;            // number of pivots (= popcount(pm))
+    int n_act_new;      // active rows left
+};
+
+// Selects <= 64 active rows whose panel words (word g of each row) span all
+// active rows' panel words, removes them from the active list act[0..n_act)
+// (stable for the rest of the list up to hole filling), and returns for each
+// pivot bit c the RREF basis combination comb[c] over the selected rows
+// selrow[0..rho).  All NT threads of the workgroup must call it.
+//   row(q)   : pointer to the row of active-list entry q's row id
+//   act      : active list (global or LDS), ACT = its element type
+template <int NT, typename ACT, typename RowFn>
+__device__ PanelOut wg_panel_select(SelectLDS<NT> &L, ACT *act, int n_act, int g, RowFn row,
+                                    uint64_t *comb_out, int *selrow_out) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid < 64) L.bas[tid] = 0;
+    if (tid == 0) {
+        L.pm = 0;
+        L.nsel = 0;
+    }
+    __syncthreads();
+    for (int base = 0; base < n_act; base += NT) {
+        const int pos = base + tid;
+        uint64_t x = 0;
+        if (pos < n_act) x = row((int)act[pos])[g];
+        const uint64_t pm = L.pm;
+        uint64_t y = x;
+        uint64_t hit = x & pm;
+        while (hit) {
+            const int c = ffs64(hit);
+            hit &= hit - 1;
+            y ^= L.bas[c];
+        }
+        const uint64_t bal = __ballot(y != 0);
+        if (lane == 0) L.wcnt[wave] = popc64(bal);
+        __syncthreads();
+        int off = 0, ncand = 0;
+        for (int q = 0; q < NT / 64; ++q) {
+            const int cq = L.wcnt[q];
+            if (q < wave) off += cq;
+            ncand += cq;
+        }
+        if (y != 0) {
+            const int my = off + popc64(bal & ((1ull << lane) - 1ull));
+            L.cy[my] = y;
+            L.cpos[my] = pos;
+        }
+        __syncthreads();
+        if (wave == 0) {
+            uint64_t pmw = L.pm;
+            int nsel = L.nsel;
+            for (int gb = 0; gb < ncand && pmw != ~0ull; gb += 64) {
+                const int ci = gb + lane;
+                const bool valid = ci < ncand;
+                uint64_t v = valid ? L.cy[ci] : 0;
+                uint64_t h2 = v & pmw;  // pivots added since the chunk filter
+                while (h2) {
+                    const int c = ffs64(h2);
+                    h2 &= h2 - 1;
+                    v ^= L.bas[c];
+                }
+                uint64_t dummy = 0;
+                bool piv;
+                int pbit;
+                wave_gj<false>(v, dummy, valid && v != 0, piv, pbit, lane);
+                const uint64_t pl = __ballot(piv);
+                uint64_t newmask = 0;
+                if (piv) L.bas[pbit] = v;
+                __threadfence_block();
+                for (uint64_t t = pl; t; t &= t - 1)
+                    newmask |= 1ull << __builtin_amdgcn_readlane(pbit, ffs64(t));
+                // the new vectors are RREF among themselves and zero on the old
+                // pivot bits: one pass clears the new bits from the old basis
+                if ((pmw >> lane) & 1ull) {
+                    uint64_t b = L.bas[lane];
+                    uint64_t hb = b & newmask;
+                    while (hb) {
+                        const int c = ffs64(hb);
+                        hb &= hb - 1;
+                        b ^= L.bas[c];
+                    }
+                    L.bas[lane] = b;
+                }
+                if (piv) L.selpos[nsel + popc64(pl & ((1ull << lane) - 1ull))] = L.cpos[ci];
+                __threadfence_block();
+                nsel += popc64(pl);
+                pmw |= newmask;
+            }
+            if (lane == 0) {
+                L.pm = pmw;
+                L.nsel = nsel;
+            }
+        }
+        __syncthreads();
+        if (L.pm == ~0ull) break;
+    }
+
+    PanelOut out;
+    const int nsel = L.nsel;
+    const int n_new = n_act - nsel;
+    if (wave == 0) {
+        // holes (selected positions below n_new) take the unselected rows of
+        // positions n_new .. n_act-1
+        int my_pos = -1;
+        if (lane < nsel) {
+            my_pos = L.selpos[lane];
+            L.selrow[lane] = (int)act[my_pos];
+        }
+        L.flag[lane] = 0;
+        __threadfence_block();
+        if (lane < nsel && my_pos >= n_new) L.flag[my_pos - n_new] = 1;
+        __threadfence_block();
+        const bool is_hole = lane < nsel && my_pos < n_new;
+        const bool is_mover = lane < nsel && L.flag[lane] == 0;
+        const uint64_t hb = __ballot(is_hole), mb = __ballot(is_mover);
+        const uint64_t below = (1ull << lane) - 1ull;
+        if (is_mover) L.mv[popc64(mb & below)] = (uint32_t)act[n_new + lane];
+        __threadfence_block();
+        if (is_hole) act[my_pos] = (ACT)L.mv[popc64(hb & below)];
+        // RREF of the selected rows' panel words, tracking combinations
+        uint64_t v = 0, comb = 0;
+        if (lane < nsel) {
+            v = row(L.selrow[lane])[g];
+            comb = 1ull << lane;
+        }
+        bool piv;
+        int pbit;
+        wave_gj<true>(v, comb, lane < nsel, piv, pbit, lane);
+        if (lane < nsel) selrow_out[lane] = L.selrow[lane];
+        if (piv) comb_out[pbit] = comb;
+    }
+    __syncthreads();
+    out.pm = L.pm;
+    out.rho = nsel;
+    out.n_act_new = n_new;
+    return out;
+}
+
+}  // namespace dev
+}  // namespace mela

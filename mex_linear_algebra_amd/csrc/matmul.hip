@@ -46,7 +46,7 @@ k_andpop(const uint64_t *__restrict__ A, size_t lda, const uint64_t *__restrict_
     const int lb_col = tid;
     const bool a_ok = i0 + la_row < m;
     const bool b_ok = j0 + lb_col < n;
-    const uint32_t *a_src = A32 + (size_t)(i0 + la_row) * lda32 + la_off;
+    const uint32_t *a_src = A32 + (size_t)(i0 + la_row) * lda32;
     const uint32_t *b_src = B32 + (size_t)(j0 + lb_col) * ldb32;
 
     uint32_t ra[4], rb[AP_KC];
