@@ -129,10 +129,13 @@ mela_status mela_dev_alloc(mela_ctx *ctx, size_t bytes, void **dptr);
 mela_status mela_dev_free(mela_ctx *ctx, void *dptr);
 mela_status mela_copy_h2d(mela_ctx *ctx, void *dst, const void *src, size_t bytes);
 mela_status mela_copy_d2h(mela_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* device-to-device copy by a plain kernel (16-byte aligned; enqueued only) */
+mela_status mela_copy_d2d(mela_ctx *ctx, void *dst, const void *src, size_t bytes);
 mela_status mela_bitmat_alloc(mela_ctx *ctx, size_t rows, size_t cols, mela_bitmat *out);
 void mela_bitmat_free(mela_ctx *ctx, mela_bitmat *m);
 
-/* Bernoulli(1/2) bits: word (i,w) = splitmix64(splitmix64(seed) + i*ceil(cols/64) + w) */
+/* Bernoulli(1/2) bits: word (i,w) = splitmix64(splitmix64(k1 + i*ceil(cols/64) + w) ^ k2),
+ * k1 = splitmix64(seed), k2 = splitmix64(seed ^ 0xD1B54A32D192ED03) */
 mela_status mela_gen_bernoulli(mela_ctx *ctx, uint64_t seed, mela_bitmat *M);
 
 /* pack a column-major logical/double host-or-device array into M (device) */
@@ -188,6 +191,49 @@ mela_status mela_batch_autotune(mela_batch *batch, float *best_ms);
 mela_status mela_batch_run(mela_batch *batch);
 mela_status mela_batch_results(mela_batch *batch, uint32_t *ranks, mela_bitmat *nulls);
 void mela_batch_destroy(mela_batch *batch);
+
+/* ---------------------------------------------------------------- serving
+ *
+ * No reference counterpart.  A server admits mixed rank / null-space / matmul
+ * requests on device-resident packed inputs by an HBM budget, batches the
+ * admitted rank and null requests continuously into the persistent batch
+ * kernel, runs jobs concurrently on `workers` streams, and reports latency
+ * percentiles.  Inputs must stay valid until the request completes; outputs
+ * (null basis or product) stay on the device until mela_server_release. */
+enum { MELA_OP_RANK = 0, MELA_OP_NULL = 1, MELA_OP_MATMUL = 2 };
+typedef struct mela_request {
+    int op;          /* MELA_OP_* */
+    int variant;     /* MELA_ELIM_* for rank/null, MELA_ALGO_* for matmul */
+    mela_bitmat A;   /* device input */
+    mela_bitmat B;   /* second factor (matmul only) */
+} mela_request;
+typedef struct mela_result {
+    mela_status status;
+    size_t rank;       /* rank (rank and null requests) */
+    mela_bitmat out;   /* null basis (cols x nullity) or product; device */
+    double latency_ms; /* submit -> completion */
+} mela_result;
+typedef struct mela_server_stats {
+    uint64_t submitted, completed, rejected, cancelled, failed;
+    double p50_ms, p99_ms, mean_ms;
+    size_t hbm_in_use, hbm_peak;
+    uint64_t jobs, batched_requests;
+} mela_server_stats;
+typedef struct mela_server mela_server;
+mela_status mela_server_create(int device, size_t hbm_budget_bytes, int workers,
+                               int max_batch, mela_server **server);
+/* queues a request; MELA_ERR_ADMISSION if it alone exceeds the budget */
+mela_status mela_server_submit(mela_server *server, const mela_request *req,
+                               uint64_t *ticket);
+/* waits (timeout_s < 0: forever); MELA_ERR_TIMEOUT / MELA_ERR_CANCELLED */
+mela_status mela_server_wait(mela_server *server, uint64_t ticket, double timeout_s,
+                             mela_result *result);
+mela_status mela_server_cancel(mela_server *server, uint64_t ticket);
+/* frees a finished request's device output and forgets the ticket */
+mela_status mela_server_release(mela_server *server, uint64_t ticket);
+mela_status mela_server_stats_get(mela_server *server, mela_server_stats *stats);
+mela_status mela_server_reset_stats(mela_server *server);
+void mela_server_destroy(mela_server *server);
 
 #ifdef __cplusplus
 }

@@ -24,6 +24,8 @@ __all__ = [
     "mela_matmul_gf2", "mela_matmul_m4ri", "mela_rank_gf2", "mela_rank_m4ri",
     "mela_null_gf2", "mela_null_m4ri",
     "ALGO_AUTO", "ALGO_ANDPOP", "ALGO_M4RM", "ALGO_MFMA_I8", "ELIM_GE", "ELIM_M4RI",
+    "Batch", "BATCH_NULL", "BATCH_GE", "BATCH_GRAPH",
+    "Server", "Request", "Result", "OP_RANK", "OP_NULL", "OP_MATMUL",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -52,6 +54,26 @@ class BitMatrix(C.Structure):
 
 
 ALLOC_FN = C.CFUNCTYPE(C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t)
+
+OP_RANK, OP_NULL, OP_MATMUL = 0, 1, 2
+
+
+class Request(C.Structure):
+    """mela_request: op, variant, device inputs A (and B for matmul)."""
+    _fields_ = [("op", C.c_int), ("variant", C.c_int), ("A", BitMatrix), ("B", BitMatrix)]
+
+
+class Result(C.Structure):
+    """mela_result: status, rank, device output, latency."""
+    _fields_ = [("status", C.c_int), ("rank", C.c_size_t), ("out", BitMatrix),
+                ("latency_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
+                ("cancelled", C.c_uint64), ("failed", C.c_uint64), ("p50_ms", C.c_double),
+                ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("hbm_in_use", C.c_size_t),
+                ("hbm_peak", C.c_size_t), ("jobs", C.c_uint64), ("batched_requests", C.c_uint64)]
 
 _lib = None
 _lib_lock = threading.Lock()
@@ -102,6 +124,8 @@ def load():
         L.mela_copy_h2d.restype = st
         L.mela_copy_d2h.argtypes = [ctxp, vp, vp, sz]
         L.mela_copy_d2h.restype = st
+        L.mela_copy_d2d.argtypes = [ctxp, vp, vp, sz]
+        L.mela_copy_d2d.restype = st
         L.mela_bitmat_alloc.argtypes = [ctxp, sz, sz, bm]
         L.mela_bitmat_alloc.restype = st
         L.mela_bitmat_free.argtypes = [ctxp, bm]
@@ -120,6 +144,36 @@ def load():
         L.mela_rref_packed.restype = st
         L.mela_null_packed.argtypes = [ctxp, bm, C.c_int, bm, C.POINTER(sz)]
         L.mela_null_packed.restype = st
+        bp = C.c_void_p
+        L.mela_batch_create.argtypes = [ctxp, bm, sz, C.c_int, C.POINTER(bp)]
+        L.mela_batch_create.restype = st
+        L.mela_batch_set_modes.argtypes = [bp, C.POINTER(C.c_int)]
+        L.mela_batch_set_modes.restype = st
+        L.mela_batch_autotune.argtypes = [bp, C.POINTER(C.c_float)]
+        L.mela_batch_autotune.restype = st
+        L.mela_batch_run.argtypes = [bp]
+        L.mela_batch_run.restype = st
+        L.mela_batch_results.argtypes = [bp, C.POINTER(C.c_uint32), bm]
+        L.mela_batch_results.restype = st
+        L.mela_batch_destroy.argtypes = [bp]
+        L.mela_batch_destroy.restype = None
+        srv = C.c_void_p
+        L.mela_server_create.argtypes = [C.c_int, sz, C.c_int, C.c_int, C.POINTER(srv)]
+        L.mela_server_create.restype = st
+        L.mela_server_submit.argtypes = [srv, C.POINTER(Request), C.POINTER(C.c_uint64)]
+        L.mela_server_submit.restype = st
+        L.mela_server_wait.argtypes = [srv, C.c_uint64, C.c_double, C.POINTER(Result)]
+        L.mela_server_wait.restype = st
+        L.mela_server_cancel.argtypes = [srv, C.c_uint64]
+        L.mela_server_cancel.restype = st
+        L.mela_server_release.argtypes = [srv, C.c_uint64]
+        L.mela_server_release.restype = st
+        L.mela_server_stats_get.argtypes = [srv, C.POINTER(ServerStats)]
+        L.mela_server_stats_get.restype = st
+        L.mela_server_reset_stats.argtypes = [srv]
+        L.mela_server_reset_stats.restype = st
+        L.mela_server_destroy.argtypes = [srv]
+        L.mela_server_destroy.restype = None
         _lib = L
         return L
 
@@ -213,6 +267,99 @@ class Context:
         f = C.c_size_t()
         _check(self.lib.mela_null_packed(self.handle, C.byref(A), variant, C.byref(Z), C.byref(f)))
         return Z
+
+
+BATCH_NULL, BATCH_GE, BATCH_GRAPH = 1, 2, 4
+
+
+class Batch:
+    """Batched rank + canonical null basis of many packed device matrices in
+    one persistent launch (mela_batch_* in include/mela.h)."""
+
+    def __init__(self, ctx: Context, mats, flags: int = BATCH_NULL | BATCH_GRAPH):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.count = len(mats)
+        self._mats = (BitMatrix * max(self.count, 1))(*mats)
+        h = C.c_void_p()
+        _check(self.lib.mela_batch_create(ctx.handle, self._mats, self.count, flags, C.byref(h)))
+        self.handle = h
+
+    def set_modes(self, modes):
+        arr = (C.c_int * max(self.count, 1))(*[int(x) for x in modes])
+        _check(self.lib.mela_batch_set_modes(self.handle, arr))
+
+    def autotune(self) -> float:
+        ms = C.c_float()
+        _check(self.lib.mela_batch_autotune(self.handle, C.byref(ms)))
+        return ms.value
+
+    def run(self):
+        _check(self.lib.mela_batch_run(self.handle))
+
+    def results(self):
+        ranks = (C.c_uint32 * max(self.count, 1))()
+        nulls = (BitMatrix * max(self.count, 1))()
+        _check(self.lib.mela_batch_results(self.handle, ranks, nulls))
+        return [ranks[i] for i in range(self.count)], [nulls[i] for i in range(self.count)]
+
+    def close(self):
+        if self.handle:
+            self.lib.mela_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching request server (mela_server_* in include/mela.h)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 64 << 30, workers: int = 4,
+                 max_batch: int = 64):
+        self.lib = load()
+        h = C.c_void_p()
+        _check(self.lib.mela_server_create(device, hbm_budget, workers, max_batch, C.byref(h)))
+        self.handle = h
+
+    def submit(self, op: int, A: BitMatrix, B: BitMatrix | None = None, variant: int = 0) -> int:
+        req = Request(op, variant, A, B if B is not None else BitMatrix())
+        t = C.c_uint64()
+        _check(self.lib.mela_server_submit(self.handle, C.byref(req), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int, timeout_s: float = -1.0) -> Result:
+        res = Result()
+        _check(self.lib.mela_server_wait(self.handle, ticket, timeout_s, C.byref(res)))
+        return res
+
+    def cancel(self, ticket: int):
+        _check(self.lib.mela_server_cancel(self.handle, ticket))
+
+    def release(self, ticket: int):
+        _check(self.lib.mela_server_release(self.handle, ticket))
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(self.lib.mela_server_stats_get(self.handle, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def reset_stats(self):
+        _check(self.lib.mela_server_reset_stats(self.handle))
+
+    def close(self):
+        if self.handle:
+            self.lib.mela_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _default = None

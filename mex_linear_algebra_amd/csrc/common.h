@@ -68,6 +68,7 @@ mela_status launch_transpose(hipStream_t s, const uint64_t *src, size_t lds,
 mela_status launch_gen_bernoulli(hipStream_t s, uint64_t seed, uint64_t *dst,
                                  size_t rows, size_t cols, size_t ld);
 mela_status launch_fill_zero(hipStream_t s, void *dst, size_t bytes);
+mela_status launch_copy(hipStream_t s, void *dst, const void *src, size_t bytes);
 
 // matmul.hip
 mela_status matmul_device(mela_ctx *ctx, const uint64_t *A, size_t lda,

@@ -155,6 +155,11 @@ mela_status mela_copy_d2h(mela_ctx *ctx, void *dst, const void *src, size_t byte
     return MELA_OK;
 }
 
+mela_status mela_copy_d2d(mela_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx) return set_error(MELA_ERR_INVALID, "null context");
+    return launch_copy(ctx->stream, dst, src, bytes);
+}
+
 mela_status mela_bitmat_alloc(mela_ctx *ctx, size_t rows, size_t cols, mela_bitmat *out) {
     if (!ctx || !out) return set_error(MELA_ERR_INVALID, "null argument");
     const size_t ld = words64(cols) ? words64(cols) : 1;

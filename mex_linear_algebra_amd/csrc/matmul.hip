@@ -245,15 +245,12 @@ mela_status matmul_device(mela_ctx *ctx, const uint64_t *A, size_t lda, const ui
     hipStream_t s = ctx->stream;
     const size_t nw = words64(n);
     if (m == 0 || n == 0) return MELA_OK;
-    if (k == 0) {
-        // empty inner dimension: C = 0
-        for (size_t i = 0; i < 1; ++i) {
-        }
-        return launch_fill_zero(s, C, m * ldc * sizeof(uint64_t));
-    }
+    if (k == 0) return launch_fill_zero(s, C, m * ldc * sizeof(uint64_t));  // C = 0
     if (m > (size_t)INT32_MAX || n > (size_t)INT32_MAX || k > (size_t)INT32_MAX)
         return set_error(MELA_ERR_INVALID, "matrix dimension exceeds 2^31-1");
-    if (algo == MELA_ALGO_AUTO) algo = MELA_ALGO_M4RM;
+    // AUTO: the AND+parity kernel measured fastest on MI355X at every shape
+    // tried so far (32768^3: 44.8 ms vs 50.4 ms for M4RM)
+    if (algo == MELA_ALGO_AUTO) algo = MELA_ALGO_ANDPOP;
 
     if (algo == MELA_ALGO_ANDPOP) {
         // Bt = B^T (n x ceil(k/64) words)

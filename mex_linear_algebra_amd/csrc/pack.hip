@@ -112,7 +112,7 @@ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
     return z ^ (z >> 31);
 }
 
-__global__ void __launch_bounds__(256) k_gen(uint64_t key, uint64_t *__restrict__ dst,
+__global__ void __launch_bounds__(256) k_gen(uint64_t k1, uint64_t k2, uint64_t *__restrict__ dst,
                                              size_t rows, size_t cols, size_t ld) {
     size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (idx >= rows * ld) return;
@@ -120,7 +120,7 @@ __global__ void __launch_bounds__(256) k_gen(uint64_t key, uint64_t *__restrict_
     size_t wc = (cols + 63) / 64;
     uint64_t v = 0;
     if (w < wc) {
-        v = splitmix64(key + (uint64_t)(i * wc + w));
+        v = splitmix64(splitmix64(k1 + (uint64_t)(i * wc + w)) ^ k2);
         if (w == wc - 1 && (cols & 63)) v &= (1ull << (cols & 63)) - 1;
     }
     dst[idx] = v;
@@ -130,6 +130,13 @@ __global__ void __launch_bounds__(256) k_zero(uint4 *__restrict__ p, size_t n16)
     size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     size_t stride = (size_t)gridDim.x * 256;
     for (; i < n16; i += stride) p[i] = make_uint4(0, 0, 0, 0);
+}
+
+__global__ void __launch_bounds__(256) k_copy16(const uint4 *__restrict__ src,
+                                                uint4 *__restrict__ dst, size_t n16) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (; i < n16; i += stride) dst[i] = src[i];
 }
 
 __global__ void k_zero_tail(uint8_t *__restrict__ p, size_t n) {
@@ -183,8 +190,25 @@ mela_status launch_gen_bernoulli(hipStream_t s, uint64_t seed, uint64_t *dst, si
                                  size_t cols, size_t ld) {
     size_t total = rows * ld;
     if (total == 0) return MELA_OK;
+    // two splitmix64 rounds: one round on a counter leaves GF(2)-linear
+    // dependences between bit columns
     hipLaunchKernelGGL(k_gen, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
-                       host_splitmix64(seed), dst, rows, cols, ld);
+                       host_splitmix64(seed), host_splitmix64(seed ^ 0xD1B54A32D192ED03ull), dst,
+                       rows, cols, ld);
+    MELA_HIP(hipGetLastError());
+    return MELA_OK;
+}
+
+// Plain device copy (16-byte vectors); used to calibrate HBM byte counters.
+mela_status launch_copy(hipStream_t s, void *dst, const void *src, size_t bytes) {
+    if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16)
+        return set_error(MELA_ERR_INVALID, "copy needs 16-byte aligned sizes and pointers");
+    const size_t n16 = bytes / 16;
+    if (!n16) return MELA_OK;
+    size_t blocks = (n16 + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, s, (const uint4 *)src,
+                       (uint4 *)dst, n16);
     MELA_HIP(hipGetLastError());
     return MELA_OK;
 }

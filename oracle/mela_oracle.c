@@ -572,20 +572,25 @@ uint64_t orc_splitmix64(uint64_t x)
     return z ^ (z >> 31);
 }
 
-/* word (i,w) = splitmix64(splitmix64(seed) + i*ceil(cols/64) + w), bits past
- * the last column cleared; the device generator computes the same words. */
+/* word (i,w) = splitmix64(splitmix64(k1 + i*ceil(cols/64) + w) ^ k2) with
+ * k1 = splitmix64(seed), k2 = splitmix64(seed ^ 0xD1B54A32D192ED03); bits past
+ * the last column cleared.  Two rounds: a single splitmix64 round on a
+ * counter leaves GF(2)-linear dependences between bit columns (a 12000 x 11968
+ * block had rank 11855), which would make "random" matrices rank-deficient.
+ * The device generator computes the same words. */
 void orc_gen_bernoulli_packed(uint64_t seed, size_t rows, size_t cols,
                               uint64_t *out, size_t ld)
 {
     size_t wc = W64(cols);
-    uint64_t key = orc_splitmix64(seed);
+    uint64_t k1 = orc_splitmix64(seed);
+    uint64_t k2 = orc_splitmix64(seed ^ 0xD1B54A32D192ED03ULL);
     uint64_t last = (cols & 63) ? (((uint64_t)1 << (cols & 63)) - 1) : ~(uint64_t)0;
 #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < rows; ++i) {
         for (size_t w = 0; w < ld; ++w) {
             uint64_t v = 0;
             if (w < wc) {
-                v = orc_splitmix64(key + (uint64_t)(i * wc + w));
+                v = orc_splitmix64(orc_splitmix64(k1 + (uint64_t)(i * wc + w)) ^ k2);
                 if (w == wc - 1)
                     v &= last;
             }

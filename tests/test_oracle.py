@@ -210,9 +210,18 @@ def test_generator_words(orc):
     P = orc.gen_bernoulli_packed(5, 3, 130)
     assert P.shape == (3, 3)
     assert int(P[0, 2]) >> 2 == 0  # bits past column 130 are clear
-    key = orc.load().orc_splitmix64(5)
-    assert int(P[1, 1]) == int(orc.load().orc_splitmix64((key + 1 * 3 + 1) & (2**64 - 1)))
+    sm = orc.load().orc_splitmix64
+    k1, k2 = int(sm(5)), int(sm(5 ^ 0xD1B54A32D192ED03))
+    M64 = 2**64 - 1
+    assert int(P[1, 1]) == int(sm(int(sm((k1 + 1 * 3 + 1) & M64)) ^ k2))
     # splitmix64 known answer for seed 0 (first output of the published generator)
-    assert int(orc.load().orc_splitmix64(0)) == 0xE220A8397B1DCDAF
+    assert int(sm(0)) == 0xE220A8397B1DCDAF
     ones = np.unpackbits(orc.gen_bernoulli_packed(9, 256, 256).view(np.uint8)).mean()
     assert abs(ones - 0.5) < 0.01
+
+
+def test_generator_has_no_gf2_column_dependences(orc):
+    # a random 4000 x 3900 bit matrix has full column rank except with
+    # probability ~2^-100; a single hash round on a counter failed this
+    P = orc.gen_bernoulli_packed(12, 4000, 3900)
+    assert orc.rank_packed(P, 4000, 3900, "m4ri") == 3900
