@@ -86,8 +86,26 @@ def cpu_matmul_baseline(n, budget_s=25.0):
                       f"{len(times)} call(s), {dt:.2f} s each"}
 
 
+def roofline_for(algo, achieved, traffic):
+    """Roofline of the dominant kernel of each product algorithm (W = 2mnk)."""
+    if algo in ("mfma_fp4", "mfma_i8"):
+        # dense MFMA: fp4 10.1 PFLOP/s, int8 5.0 POP/s; one AND+XOR pair of W
+        # is one multiply-add (2 ops), so W/s compares directly with ops/s
+        peak = 10100.0 if algo == "mfma_fp4" else 5000.0
+        return {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak,
+                "unit": "Tbitop/s", "frac": round(achieved / peak, 4), "traffic": traffic,
+                "kernel": "k_mfma_gf2<%s>" % ("fp4" if algo == "mfma_fp4" else "int8"),
+                "note": "achieved = 2mnk / HIP-event time per call (expand + MFMA kernels)"}
+    return {"bound": "valu", "achieved": round(achieved, 2), "peak": VALU_BITOP3_PEAK,
+            "unit": "Tbitop/s", "frac": round(achieved / VALU_BITOP3_PEAK, 4), "traffic": traffic,
+            "kernel": "k_andpop" if algo in ("andpop", "auto") else "k_m4rm",
+            "note": "achieved = 2mnk / HIP-event time per launch on the library stream; peak = "
+                    "bitwise VALU with AND+XOR fused in v_bitop3_b32 (2 x 1258)"}
+
+
 def bench_matmul(args, mela):
-    algo = {"auto": mela.ALGO_AUTO, "m4rm": mela.ALGO_M4RM, "andpop": mela.ALGO_ANDPOP}[args.algo]
+    algo = {"auto": mela.ALGO_AUTO, "m4rm": mela.ALGO_M4RM, "andpop": mela.ALGO_ANDPOP,
+            "mfma_i8": mela.ALGO_MFMA_I8, "mfma_fp4": mela.ALGO_MFMA_FP4}[args.algo]
     n = args.n
     ctx = mela.Context(0)
     A, B, Cm = ctx.bitmat(n, n), ctx.bitmat(n, n), ctx.bitmat(n, n)
@@ -127,12 +145,7 @@ def bench_matmul(args, mela):
         "data": "synthetic Bernoulli(1/2) bits, splitmix64 seeds 1 (A) and 2 (B), generated on device",
         "config": {"workload": f"packed GF(2) matmul {n}x{n} * {n}x{n}, operands resident in HBM",
                    "algo": args.algo, "parallelism": "1 GPU"},
-        "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": VALU_BITOP3_PEAK,
-                     "unit": "Tbitop/s", "frac": round(achieved / VALU_BITOP3_PEAK, 4),
-                     "traffic": args.traffic,
-                     "kernel": "k_andpop" if args.algo == "andpop" else "k_m4rm",
-                     "note": "achieved = 2mnk / HIP-event time per launch on the library stream; "
-                             "peak = bitwise VALU with AND+XOR fused in v_bitop3_b32 (2 x 1258)"},
+        "roofline": roofline_for(args.algo, achieved, args.traffic),
         "kernel_ms": round(dev_ms, 3),
     }
     if not args.no_cpu:
@@ -374,7 +387,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="matmul", choices=["matmul", "elim", "batch", "serve", "copy"])
-    ap.add_argument("--algo", default="andpop", choices=["auto", "m4rm", "andpop"])
+    ap.add_argument("--algo", default="andpop",
+                    choices=["auto", "m4rm", "andpop", "mfma_i8", "mfma_fp4"])
     ap.add_argument("--n", type=int, default=32768)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", type=float, default=None,

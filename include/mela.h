@@ -122,7 +122,8 @@ enum {
     MELA_ALGO_AUTO = 0,
     MELA_ALGO_ANDPOP = 1, /* AND + parity over a transposed B (v_bitop3) */
     MELA_ALGO_M4RM = 2,   /* Method of Four Russians, Gray-code tables in LDS */
-    MELA_ALGO_MFMA_I8 = 3 /* exact int8 MFMA, i32 accumulate, & 1 */
+    MELA_ALGO_MFMA_I8 = 3, /* exact int8 MFMA, i32 accumulate, & 1 */
+    MELA_ALGO_MFMA_FP4 = 4 /* exact fp4 (e2m1) MFMA, f32 accumulate, & 1 */
 };
 
 mela_status mela_dev_alloc(mela_ctx *ctx, size_t bytes, void **dptr);
@@ -179,7 +180,9 @@ typedef struct mela_batch mela_batch;
 enum {
     MELA_BATCH_NULL = 1,  /* also compute the packed null bases */
     MELA_BATCH_GE = 2,    /* start every matrix in GE mode (default M4RI) */
-    MELA_BATCH_GRAPH = 4  /* capture the launch in a hipGraph and replay it */
+    MELA_BATCH_GRAPH = 4, /* capture the launch in a hipGraph and replay it */
+    MELA_BATCH_CTX_SCRATCH = 8 /* borrow the context's cached arenas (only one
+                                  such batch may exist per context at a time) */
 };
 mela_status mela_batch_create(mela_ctx *ctx, const mela_bitmat *mats, size_t count,
                               int flags, mela_batch **batch);

@@ -68,7 +68,8 @@ int env_algo(int dflt) {
     if (!e) return dflt;
     if (!std::strcmp(e, "andpop")) return MELA_ALGO_ANDPOP;
     if (!std::strcmp(e, "m4rm")) return MELA_ALGO_M4RM;
-    if (!std::strcmp(e, "mfma")) return MELA_ALGO_MFMA_I8;
+    if (!std::strcmp(e, "mfma_i8")) return MELA_ALGO_MFMA_I8;
+    if (!std::strcmp(e, "mfma_fp4")) return MELA_ALGO_MFMA_FP4;
     return dflt;
 }
 

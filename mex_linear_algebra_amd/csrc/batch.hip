@@ -280,6 +280,7 @@ struct mela_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int tuned = -1;  // index of the autotuned mode assignment
+    bool ctx_scratch = false;  // arenas borrowed from the context (not freed)
 };
 
 namespace mela {
@@ -304,9 +305,11 @@ mela_status batch_launch_raw(mela_batch *b, hipStream_t s) {
 namespace {
 
 void free_batch_device(mela_batch *b) {
-    void *ptrs[] = {b->d_mats, b->d_order, b->d_counter, b->d_inputs, b->d_work, b->d_z, b->d_ranks};
-    for (void *p : ptrs)
-        if (p) (void)hipFree(p);
+    if (!b->ctx_scratch) {
+        void *ptrs[] = {b->d_mats, b->d_order, b->d_counter, b->d_inputs, b->d_work, b->d_z, b->d_ranks};
+        for (void *p : ptrs)
+            if (p) (void)hipFree(p);
+    }
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
 }
@@ -384,10 +387,33 @@ mela_status mela_batch_create(mela_ctx *ctx, const mela_bitmat *mats, size_t cou
     b->z_words = z;
     const size_t nb = b->mats.size();
     mela_status st;
-    if ((st = dalloc(&b->d_mats, nb)) || (st = dalloc(&b->d_order, nb)) ||
-        (st = dalloc(&b->d_counter, 1)) || (st = dalloc(&b->d_inputs, nb)) ||
-        (st = dalloc(&b->d_work, work)) || (st = dalloc(&b->d_z, z)) ||
-        (st = dalloc(&b->d_ranks, nb))) {
+    if (flags & MELA_BATCH_CTX_SCRATCH) {
+        // one batch at a time on this context: borrow its cached arenas
+        b->ctx_scratch = true;
+        void *desc = nullptr, *wk = nullptr, *za = nullptr, *rk = nullptr;
+        const size_t desc_bytes = nb * (sizeof(BatchMat) + sizeof(int) + sizeof(void *)) + 64;
+        if ((st = scratch_get(ctx, SLOT_BATCH_DESC, desc_bytes, &desc)) ||
+            (st = scratch_get(ctx, SLOT_BATCH_WORK, work * 8, &wk)) ||
+            (st = scratch_get(ctx, SLOT_BATCH_Z, z * 8, &za)) ||
+            (st = scratch_get(ctx, SLOT_BATCH_RANK, nb * 4 + 4, &rk))) {
+            delete b;
+            return st;
+        }
+        char *dp = (char *)desc;
+        b->d_mats = (BatchMat *)dp;
+        dp += nb * sizeof(BatchMat);
+        b->d_inputs = (const uint64_t **)dp;
+        dp += nb * sizeof(void *);
+        b->d_order = (int *)dp;
+        dp += nb * sizeof(int);
+        b->d_counter = (int *)(((uintptr_t)dp + 15) & ~(uintptr_t)15);
+        b->d_work = (uint64_t *)wk;
+        b->d_z = (uint64_t *)za;
+        b->d_ranks = (uint32_t *)rk;
+    } else if ((st = dalloc(&b->d_mats, nb)) || (st = dalloc(&b->d_order, nb)) ||
+               (st = dalloc(&b->d_counter, 1)) || (st = dalloc(&b->d_inputs, nb)) ||
+               (st = dalloc(&b->d_work, work)) || (st = dalloc(&b->d_z, z)) ||
+               (st = dalloc(&b->d_ranks, nb))) {
         free_batch_device(b);
         delete b;
         return st;
@@ -522,8 +548,12 @@ mela_status mela_batch_results(mela_batch *b, uint32_t *ranks, mela_bitmat *null
         MELA_HIP(hipStreamSynchronize(ctx->stream));
     }
     std::vector<uint32_t> r(b->mats.size());
-    if (!r.empty())
-        MELA_HIP(hipMemcpy(r.data(), b->d_ranks, r.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (!r.empty()) {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        MELA_HIP(hipMemcpyAsync(r.data(), b->d_ranks, r.size() * sizeof(uint32_t),
+                                hipMemcpyDeviceToHost, ctx->stream));
+        MELA_HIP(hipStreamSynchronize(ctx->stream));
+    }
     for (size_t i = 0; i < b->count; ++i) {
         if (ranks) ranks[i] = r[i];
         if (nulls) {

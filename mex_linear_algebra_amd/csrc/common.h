@@ -35,6 +35,17 @@ struct Scratch {
     size_t bytes = 0;
 };
 
+// A captured elimination sequence (all panel launches of one shape), replayed
+// with hipGraphLaunch when the same shape and work buffers come back.
+struct ElimGraph {
+    size_t m = 0, n = 0, ld = 0;
+    int variant = 0, rref = 0;
+    const void *bufs[4] = {nullptr, nullptr, nullptr, nullptr};
+    int seen = 0;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+};
+
 }  // namespace mela
 
 struct mela_ctx {
@@ -44,6 +55,10 @@ struct mela_ctx {
     std::vector<mela::Scratch> scratch;  // indexed by slot
     int cu_count = 256;
     hipEvent_t ev = nullptr;
+    std::vector<mela::ElimGraph> elim_graphs;
+    // graph capture of repeated eliminations; off for contexts that share the
+    // process with other threads doing synchronous HIP calls (server workers)
+    bool capture_ok = true;
 };
 
 namespace mela {
@@ -54,7 +69,8 @@ mela_status scratch_get(mela_ctx *ctx, int slot, size_t bytes, void **out);
 enum ScratchSlot {
     SLOT_A = 0, SLOT_B, SLOT_C, SLOT_BT, SLOT_IN_A, SLOT_IN_B, SLOT_OUT,
     SLOT_ELIM_M, SLOT_ELIM_Q, SLOT_ELIM_META, SLOT_ELIM_ACT, SLOT_ELIM_TAB,
-    SLOT_NULL_Z, SLOT_COUNT
+    SLOT_NULL_Z, SLOT_XA, SLOT_XB,
+    SLOT_BATCH_DESC, SLOT_BATCH_WORK, SLOT_BATCH_Z, SLOT_BATCH_RANK, SLOT_COUNT
 };
 
 // ------------------------------------------------- kernel launchers ----
@@ -74,6 +90,11 @@ mela_status launch_copy(hipStream_t s, void *dst, const void *src, size_t bytes)
 mela_status matmul_device(mela_ctx *ctx, const uint64_t *A, size_t lda,
                           const uint64_t *B, size_t ldb, uint64_t *C, size_t ldc,
                           size_t m, size_t k, size_t n, int algo);
+
+// mfma.hip
+mela_status matmul_mfma(mela_ctx *ctx, const uint64_t *A, size_t lda, const uint64_t *B,
+                        size_t ldb, uint64_t *C, size_t ldc, size_t m, size_t k, size_t n,
+                        bool fp4);
 
 // elim.hip
 struct ElimResult {

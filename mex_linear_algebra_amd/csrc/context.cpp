@@ -35,6 +35,12 @@ mela_status scratch_get(mela_ctx *ctx, int slot, size_t bytes, void **out) {
             // the previous block may still be read by queued work
             hipError_t e = hipStreamSynchronize(ctx->stream);
             if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+            // captured eliminations may point into it
+            for (auto &g : ctx->elim_graphs) {
+                if (g.exec) (void)hipGraphExecDestroy(g.exec);
+                if (g.graph) (void)hipGraphDestroy(g.graph);
+            }
+            ctx->elim_graphs.clear();
             (void)hipFree(sc.ptr);
             sc.ptr = nullptr;
             sc.bytes = 0;
@@ -104,6 +110,10 @@ void mela_ctx_destroy(mela_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &g : ctx->elim_graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
     for (auto &s : ctx->scratch)
         if (s.ptr) (void)hipFree(s.ptr);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -21,7 +21,7 @@
 //                   (and, for RREF, from every earlier pivot row) with one
 //                   table XOR per 4 pivot columns (M4RI) or one XOR per pivot
 //                   row (GE), sweeping words g+1 .. wc-1 -- the HBM-bound part.
-#include "common.h"
+#include "elim_device.h"
 
 namespace mela {
 
@@ -30,219 +30,47 @@ struct ElimMeta {
     int r;      // pivot rows found so far
     int r0;     // pivot rows before the current panel
     int rho;    // pivots in the current panel
-    unsigned long long pm;     // pivot bits (columns 64g + c) of the current panel
+    uint64_t pm;        // pivot bits (columns 64g + c) of the current panel
     int sel_row[64];           // selected row ids (selection slot -> row)
-    unsigned long long comb[64];  // by pivot bit c: selection slots XORed into pivot row c
+    uint64_t comb[64];  // by pivot bit c: selection slots XORed into pivot row c
 };
 
 namespace {
 
-__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
-    uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l);
-    uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), l);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-__device__ __forceinline__ int popc64(uint64_t x) { return __popcll(x); }
-
-// Wave-level Gauss-Jordan over the 64-bit words v (one per lane, lanes with
-// valid == false take no part).  Afterwards pivot lanes hold the RREF basis of
-// span{v} (pbit = pivot bit), non-pivot lanes hold 0; comb (if tracked) is the
-// set of original lanes XORed into each lane.  Pivot of column c = the lowest
-// non-pivot lane with bit c set.
-template <bool TRACK>
-__device__ __forceinline__ void wave_gj(uint64_t &v, uint64_t &comb, bool valid, bool &piv,
-                                        int &pbit, int lane) {
-    piv = false;
-    pbit = -1;
-    if (!valid) v = 0;
-    for (int c = 0; c < 64; ++c) {
-        const bool has = ((v >> c) & 1ull) != 0;
-        const uint64_t bal = __ballot(has && !piv);
-        if (bal == 0) continue;
-        const int f = __ffsll((unsigned long long)bal) - 1;
-        const uint64_t pv = readlane64(v, f);
-        uint64_t pc = 0;
-        if (TRACK) pc = readlane64(comb, f);
-        if (lane == f) {
-            piv = true;
-            pbit = c;
-        } else if (has) {
-            v ^= pv;
-            if (TRACK) comb ^= pc;
-        }
-    }
-}
+using dev::ffs64;
+using dev::popc64;
 
 constexpr int PS_THREADS = 1024;
 
-// One workgroup.  Streams the active rows' panel words in chunks of 1024,
-// filters each against the current RREF basis, and inserts the survivors 64 at
-// a time with a wave Gauss-Jordan until 64 pivots are found or rows run out.
+// One workgroup: selects the panel's pivot rows (elim_device.h) and records
+// the panel in meta and pivcols.
 __global__ void __launch_bounds__(PS_THREADS)
 k_panel_select(const uint64_t *__restrict__ M, size_t ld, uint32_t *__restrict__ act,
                ElimMeta *__restrict__ meta, int *__restrict__ pivcols, int g, int rmax) {
-    __shared__ uint64_t s_bas[64];   // RREF basis by pivot bit
-    __shared__ uint64_t s_cy[PS_THREADS];
-    __shared__ int s_cpos[PS_THREADS];
-    __shared__ int s_wcnt[16];
-    __shared__ int s_selpos[64];
-    __shared__ int s_selrow[64];
-    __shared__ int s_flag[64];
-    __shared__ unsigned long long s_pm;
-    __shared__ int s_nsel;
-
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __shared__ dev::SelectLDS<PS_THREADS> L;
     const int n_act = meta->n_act;
     const int r_before = meta->r;
-    if (tid < 64) s_bas[tid] = 0;
+    if (n_act == 0 || r_before >= rmax) {
+        if (threadIdx.x == 0) {
+            meta->r0 = r_before;
+            meta->rho = 0;
+            meta->pm = 0;
+        }
+        return;
+    }
+    auto rowfn = [&](int q) { return M + (size_t)q * ld; };
+    const dev::PanelOut po =
+        dev::wg_panel_select<PS_THREADS>(L, act, n_act, g, rowfn, meta->comb, meta->sel_row);
+    const int tid = threadIdx.x;
+    if (tid < 64 && ((po.pm >> tid) & 1ull))
+        pivcols[r_before + popc64(po.pm & ((1ull << tid) - 1ull))] = g * 64 + tid;
     if (tid == 0) {
-        s_pm = 0;
-        s_nsel = 0;
-    }
-    __syncthreads();
-
-    const int room = rmax - r_before;  // pivots still possible
-    for (int base = 0; base < n_act && room > 0; base += PS_THREADS) {
-        const int pos = base + tid;
-        uint64_t x = 0;
-        if (pos < n_act) x = M[(size_t)act[pos] * ld + g];
-        // filter against the current basis (one pass is exact for RREF)
-        uint64_t pm = s_pm;
-        uint64_t y = x;
-        uint64_t hit = x & pm;
-        while (hit) {
-            const int c = __ffsll((unsigned long long)hit) - 1;
-            hit &= hit - 1;
-            y ^= s_bas[c];
-        }
-        const uint64_t bal = __ballot(y != 0);
-        if (lane == 0) s_wcnt[wave] = popc64(bal);
-        __syncthreads();
-        int off = 0, ncand = 0;
-        for (int q = 0; q < 16; ++q) {
-            const int cq = s_wcnt[q];
-            if (q < wave) off += cq;
-            ncand += cq;
-        }
-        if (y != 0) {
-            const int my = off + popc64(bal & ((1ull << lane) - 1ull));
-            s_cy[my] = y;
-            s_cpos[my] = pos;
-        }
-        __syncthreads();
-        if (wave == 0) {
-            uint64_t pmw = s_pm;
-            int nsel = s_nsel;
-            for (int gb = 0; gb < ncand && pmw != ~0ull; gb += 64) {
-                const int ci = gb + lane;
-                const bool valid = ci < ncand;
-                uint64_t v = valid ? s_cy[ci] : 0;
-                // reduce against pivots added since the chunk filter
-                uint64_t h2 = v & pmw;
-                while (h2) {
-                    const int c = __ffsll((unsigned long long)h2) - 1;
-                    h2 &= h2 - 1;
-                    v ^= s_bas[c];
-                }
-                uint64_t dummy = 0;
-                bool piv;
-                int pbit;
-                wave_gj<false>(v, dummy, valid && v != 0, piv, pbit, lane);
-                // (pivots can never exceed min(m,n): every pivot consumes an
-                // active row and a column below n)
-                const uint64_t pl = __ballot(piv);
-                // clear new pivot bits from the old basis (new vectors are RREF
-                // among themselves and zero on old pivot bits)
-                uint64_t newmask = 0;
-                // gather (pbit, vector) of new pivots through LDS
-                if (piv) s_bas[pbit] = v;
-                __threadfence_block();
-                for (uint64_t t = pl; t; t &= t - 1) {
-                    const int l = __ffsll((unsigned long long)t) - 1;
-                    newmask |= 1ull << __builtin_amdgcn_readlane(pbit, l);
-                }
-                if (lane < 64 && ((pmw >> lane) & 1ull)) {
-                    uint64_t b = s_bas[lane];
-                    uint64_t hb = b & newmask;
-                    while (hb) {
-                        const int c = __ffsll((unsigned long long)hb) - 1;
-                        hb &= hb - 1;
-                        b ^= s_bas[c];
-                    }
-                    s_bas[lane] = b;
-                }
-                // record selected positions in candidate order
-                if (piv) {
-                    const int slot = nsel + popc64(pl & ((1ull << lane) - 1ull));
-                    s_selpos[slot] = s_cpos[ci];
-                }
-                __threadfence_block();
-                nsel += popc64(pl);
-                pmw |= newmask;
-            }
-            if (lane == 0) {
-                s_pm = pmw;
-                s_nsel = nsel;
-            }
-        }
-        __syncthreads();
-        if (s_pm == ~0ull) break;
-    }
-
-    // ---- finalise on wave 0: selected rows, act compaction, RREF + comb ----
-    if (wave != 0) return;
-    const int nsel = s_nsel;
-    const int n_new = n_act - nsel;
-    int my_pos = -1;
-    if (lane < nsel) {
-        my_pos = s_selpos[lane];
-        s_selrow[lane] = (int)act[my_pos];
-    }
-    s_flag[lane] = 0;
-    __threadfence_block();
-    if (lane < nsel && my_pos >= n_new) s_flag[my_pos - n_new] = 1;
-    __threadfence_block();
-    // holes: selected positions below n_new; movers: unselected positions >= n_new
-    const bool is_hole = lane < nsel && my_pos < n_new;
-    const bool is_mover = lane < nsel && s_flag[lane] == 0;
-    const uint64_t hb = __ballot(is_hole), mb = __ballot(is_mover);
-    const uint64_t below = (1ull << lane) - 1ull;
-    int mover_rank = is_mover ? popc64(mb & below) : -1;
-    uint32_t mover_row = is_mover ? act[n_new + lane] : 0;
-    // hole k takes mover k: pass mover rows by rank through LDS
-    __shared__ uint32_t s_mv[64];
-    if (is_mover) s_mv[mover_rank] = mover_row;
-    __threadfence_block();
-    if (is_hole) act[my_pos] = s_mv[popc64(hb & below)];
-
-    // RREF basis of the selected rows' original panel words, with combs
-    uint64_t v = 0, comb = 0;
-    if (lane < nsel) {
-        v = M[(size_t)s_selrow[lane] * ld + g];
-        comb = 1ull << lane;
-    }
-    bool piv;
-    int pbit;
-    wave_gj<true>(v, comb, lane < nsel, piv, pbit, lane);
-    if (lane < nsel) meta->sel_row[lane] = s_selrow[lane];
-    // pivot bits
-    uint64_t pm = 0;
-    for (uint64_t t = __ballot(piv); t; t &= t - 1) {
-        const int l = __ffsll((unsigned long long)t) - 1;
-        pm |= 1ull << __builtin_amdgcn_readlane(pbit, l);
-    }
-    if (piv) meta->comb[pbit] = comb;
-    const int rho = popc64(pm);
-    if (lane == 0) {
         meta->r0 = r_before;
-        meta->rho = rho;
-        meta->pm = pm;
-        meta->r = r_before + rho;
-        meta->n_act = n_new;
+        meta->rho = po.rho;
+        meta->pm = po.pm;
+        meta->r = r_before + po.rho;
+        meta->n_act = po.n_act_new;
     }
-    // pivot columns ascending: bit c is the popcount(pm below c)-th new pivot
-    if ((pm >> lane) & 1ull) pivcols[r_before + popc64(pm & below)] = g * 64 + lane;
 }
 
 // Q[r0 + j][w] for the j-th new pivot (ascending bit c) = XOR of the selected
@@ -277,13 +105,38 @@ __device__ __forceinline__ uint64_t *row_ptr(uint64_t *M, size_t ld, const uint3
     return q < n_act ? M + (size_t)act[q] * ld : Q + (size_t)(q - n_act) * ldq;
 }
 
-constexpr int UP_TILE = 64;  // words per workgroup (one per lane)
+constexpr int UP_TILE = 64;   // words per workgroup (one per lane)
+constexpr int UP_WAVES = 8;   // 512 threads
+constexpr int UP_ROWS = 4;    // rows in flight per wave
+
+// Loads UP_ROWS rows (their panel words x and this lane's word w) before any
+// use, so each wave keeps 4 independent row reads in flight.
+struct RowBatch {
+    uint64_t *rp[UP_ROWS];
+    uint64_t x[UP_ROWS];
+    uint64_t a[UP_ROWS];
+};
+
+__device__ __forceinline__ void load_rows(RowBatch &b, uint64_t *M, size_t ld,
+                                          const uint32_t *act, uint64_t *Q, size_t ldq,
+                                          int n_act, int total, int q0, int g, int w, bool wok,
+                                          uint64_t pm) {
+#pragma unroll
+    for (int j = 0; j < UP_ROWS; ++j) {
+        const int q = q0 + j;
+        b.rp[j] = q < total ? row_ptr(M, ld, act, Q, ldq, n_act, q) : nullptr;
+    }
+#pragma unroll
+    for (int j = 0; j < UP_ROWS; ++j) b.x[j] = b.rp[j] ? (b.rp[j][g] & pm) : 0ull;
+#pragma unroll
+    for (int j = 0; j < UP_ROWS; ++j) b.a[j] = (b.rp[j] && wok) ? b.rp[j][w] : 0ull;
+}
 
 // M4RI form: 16 nibble tables of the panel's pivot rows (slot c = pivot bit c,
 // zero where c is not a pivot) in LDS; each row XORs 16 table rows selected
 // by the 16 nibbles of its panel word.  Words g+1 .. wc-1 only (word g is
 // handled by k_update_panelword so that every tile reads an unmodified x).
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(UP_WAVES * 64)
 k_update_m4ri(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
               uint64_t *__restrict__ Q, size_t ldq, const ElimMeta *__restrict__ meta, int g,
               int wc, int rref) {
@@ -295,10 +148,10 @@ k_update_m4ri(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int w = g + 1 + blockIdx.x * UP_TILE + lane;
     const bool wok = w < wc;
-    if (blockIdx.y * 4 >= (unsigned)total) return;
+    if ((int)blockIdx.y * UP_WAVES * UP_ROWS >= total) return;
 #pragma unroll
-    for (int tt = 0; tt < 4; ++tt) {
-        const int t = wave * 4 + tt;
+    for (int tt = 0; tt < 16 / UP_WAVES; ++tt) {
+        const int t = wave * (16 / UP_WAVES) + tt;
         uint64_t q[4];
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
@@ -307,31 +160,34 @@ k_update_m4ri(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ 
             if (wok && ((pm >> c) & 1ull))
                 q[b] = Q[(size_t)(r0 + popc64(pm & ((1ull << c) - 1ull))) * ldq + w];
         }
-        uint64_t L[16];
-        L[0] = 0;
+        uint64_t cur = 0;
+        T[t][0][lane] = 0;
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
-#pragma unroll
-            for (int u = 0; u < (1 << b); ++u) L[(1 << b) + u] = L[u] ^ q[b];
-#pragma unroll
-        for (int v = 0; v < 16; ++v) T[t][v][lane] = L[v];
+        for (int v = 1; v < 16; ++v) {
+            cur ^= q[__builtin_ctz(v)];
+            T[t][v ^ (v >> 1)][lane] = cur;
+        }
     }
     __syncthreads();
-    const int stride = gridDim.y * 4;
-    for (int q0 = blockIdx.y * 4 + wave; q0 < total; q0 += stride) {
-        uint64_t *rp = row_ptr(M, ld, act, Q, ldq, n_act, q0);
-        const uint64_t x = rp[g] & pm;
-        if (x == 0 || !wok) continue;
-        uint64_t a = rp[w];
+    const int stride = gridDim.y * UP_WAVES * UP_ROWS;
+    for (int q0 = (blockIdx.y * UP_WAVES + wave) * UP_ROWS; q0 < total; q0 += stride) {
+        RowBatch b;
+        load_rows(b, M, ld, act, Q, ldq, n_act, total, q0, g, w, wok, pm);
 #pragma unroll
-        for (int t = 0; t < 16; ++t) a ^= T[t][(x >> (4 * t)) & 15u][lane];
-        rp[w] = a;
+        for (int j = 0; j < UP_ROWS; ++j) {
+            const uint64_t x = b.x[j];
+            if (x == 0 || !wok) continue;
+            uint64_t a = b.a[j];
+#pragma unroll
+            for (int t = 0; t < 16; ++t) a ^= T[t][(x >> (4 * t)) & 15u][lane];
+            b.rp[j][w] = a;
+        }
     }
 }
 
 // GE form: the panel's pivot rows themselves in LDS; each row XORs one pivot
 // row per set pivot bit of its panel word.
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(UP_WAVES * 64)
 k_update_ge(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
             uint64_t *__restrict__ Q, size_t ldq, const ElimMeta *__restrict__ meta, int g, int wc,
             int rref) {
@@ -343,26 +199,29 @@ k_update_ge(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ ac
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int w = g + 1 + blockIdx.x * UP_TILE + lane;
     const bool wok = w < wc;
-    if (blockIdx.y * 4 >= (unsigned)total) return;
-    for (int c = wave; c < 64; c += 4) {
+    if ((int)blockIdx.y * UP_WAVES * UP_ROWS >= total) return;
+    for (int c = wave; c < 64; c += UP_WAVES) {
         uint64_t v = 0;
         if (wok && ((pm >> c) & 1ull))
             v = Q[(size_t)(r0 + popc64(pm & ((1ull << c) - 1ull))) * ldq + w];
         P[c][lane] = v;
     }
     __syncthreads();
-    const int stride = gridDim.y * 4;
-    for (int q0 = blockIdx.y * 4 + wave; q0 < total; q0 += stride) {
-        uint64_t *rp = row_ptr(M, ld, act, Q, ldq, n_act, q0);
-        uint64_t x = rp[g] & pm;
-        if (x == 0 || !wok) continue;
-        uint64_t a = rp[w];
-        while (x) {
-            const int c = __ffsll((unsigned long long)x) - 1;
-            x &= x - 1;
-            a ^= P[c][lane];
+    const int stride = gridDim.y * UP_WAVES * UP_ROWS;
+    for (int q0 = (blockIdx.y * UP_WAVES + wave) * UP_ROWS; q0 < total; q0 += stride) {
+        RowBatch b;
+        load_rows(b, M, ld, act, Q, ldq, n_act, total, q0, g, w, wok, pm);
+#pragma unroll
+        for (int j = 0; j < UP_ROWS; ++j) {
+            uint64_t x = b.x[j];
+            if (x == 0 || !wok) continue;
+            uint64_t a = b.a[j];
+            while (x) {
+                a ^= P[ffs64(x)][lane];
+                x &= x - 1;
+            }
+            b.rp[j][w] = a;
         }
-        rp[w] = a;
     }
 }
 
@@ -490,37 +349,72 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
     const size_t tot = m * wc;
     hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, A, lda,
                        Mw, ld, m, wc);
-    hipLaunchKernelGGL(k_elim_init, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, meta, act,
-                       (int)m);
     MELA_HIP(hipGetLastError());
 
     const int cu = ctx->cu_count > 0 ? ctx->cu_count : 256;
-    for (size_t g = 0; g < wc; ++g) {
-        hipLaunchKernelGGL(k_panel_select, dim3(1), dim3(PS_THREADS), 0, s, Mw, ld, act, meta,
-                           pivcols, (int)g, (int)rmax);
-        dim3 gb((unsigned)((wc + 255) / 256), 64);
-        hipLaunchKernelGGL(k_build_pivots, gb, dim3(256), 0, s, Mw, ld, Q, ld, meta, (int)g,
-                           (int)wc);
-        const size_t rem = wc - g - 1;
-        if (rem > 0) {
-            const unsigned tx = (unsigned)((rem + UP_TILE - 1) / UP_TILE);
-            // rows are strided over gridDim.y; size it to fill the chip ~2x
-            const size_t rows_hint = m + (rref ? rmax : 0);
-            unsigned ty = (unsigned)((2 * cu + tx - 1) / tx);
-            const unsigned ty_max = (unsigned)((rows_hint + 3) / 4);
-            if (ty > ty_max) ty = ty_max;
-            if (ty < 1) ty = 1;
-            if (variant == MELA_ELIM_GE)
-                hipLaunchKernelGGL(k_update_ge, dim3(tx, ty), dim3(256), 0, s, Mw, ld, act, Q, ld,
-                                   meta, (int)g, (int)wc, rref ? 1 : 0);
-            else
-                hipLaunchKernelGGL(k_update_m4ri, dim3(tx, ty), dim3(256), 0, s, Mw, ld, act, Q,
-                                   ld, meta, (int)g, (int)wc, rref ? 1 : 0);
+    auto enqueue_panels = [&]() -> mela_status {
+        hipLaunchKernelGGL(k_elim_init, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, meta,
+                           act, (int)m);
+        for (size_t g = 0; g < wc; ++g) {
+            hipLaunchKernelGGL(k_panel_select, dim3(1), dim3(PS_THREADS), 0, s, Mw, ld, act, meta,
+                               pivcols, (int)g, (int)rmax);
+            dim3 gb((unsigned)((wc + 255) / 256), 64);
+            hipLaunchKernelGGL(k_build_pivots, gb, dim3(256), 0, s, Mw, ld, Q, ld, meta, (int)g,
+                               (int)wc);
+            const size_t rem = wc - g - 1;
+            if (rem > 0) {
+                const unsigned tx = (unsigned)((rem + UP_TILE - 1) / UP_TILE);
+                // rows are strided over gridDim.y; size it to fill the chip ~2x
+                const size_t rows_hint = m + (rref ? rmax : 0);
+                unsigned ty = (unsigned)((2 * cu + tx - 1) / tx);
+                const unsigned ty_max =
+                    (unsigned)((rows_hint + UP_WAVES * UP_ROWS - 1) / (UP_WAVES * UP_ROWS));
+                if (ty > ty_max) ty = ty_max;
+                if (ty < 1) ty = 1;
+                if (variant == MELA_ELIM_GE)
+                    hipLaunchKernelGGL(k_update_ge, dim3(tx, ty), dim3(UP_WAVES * 64), 0, s, Mw, ld,
+                                       act, Q, ld, meta, (int)g, (int)wc, rref ? 1 : 0);
+                else
+                    hipLaunchKernelGGL(k_update_m4ri, dim3(tx, ty), dim3(UP_WAVES * 64), 0, s, Mw,
+                                       ld, act, Q, ld, meta, (int)g, (int)wc, rref ? 1 : 0);
+            }
+            if (rref)
+                hipLaunchKernelGGL(k_update_panelword, dim3((unsigned)((rmax + 255) / 256)),
+                                   dim3(256), 0, s, Q, ld, meta, (int)g);
         }
-        if (rref)
-            hipLaunchKernelGGL(k_update_panelword, dim3((unsigned)((rmax + 255) / 256)), dim3(256),
-                               0, s, Q, ld, meta, (int)g);
         MELA_HIP(hipGetLastError());
+        return MELA_OK;
+    };
+    // replay a captured sequence for a shape seen before (the work buffers are
+    // the context's cached scratch, so their addresses repeat)
+    ElimGraph *eg = nullptr;
+    for (auto &e : ctx->elim_graphs)
+        if (e.m == m && e.n == n && e.ld == ld && e.variant == variant && e.rref == (int)rref &&
+            e.bufs[0] == Mw && e.bufs[1] == Q && e.bufs[2] == meta && e.bufs[3] == act)
+            eg = &e;
+    if (!eg) {
+        ElimGraph e;
+        e.m = m, e.n = n, e.ld = ld, e.variant = variant, e.rref = rref ? 1 : 0;
+        e.bufs[0] = Mw, e.bufs[1] = Q, e.bufs[2] = meta, e.bufs[3] = act;
+        ctx->elim_graphs.push_back(e);
+        eg = &ctx->elim_graphs.back();
+    }
+    eg->seen++;
+    if (!eg->exec && eg->seen >= 2 && wc <= 4096 && ctx->capture_ok) {
+        MELA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        mela_status st = enqueue_panels();
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &graph);
+        if (st != MELA_OK) return st;
+        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+        eg->graph = graph;
+        MELA_HIP(hipGraphInstantiate(&eg->exec, graph, nullptr, nullptr, 0));
+    }
+    if (eg->exec) {
+        MELA_HIP(hipGraphLaunch(eg->exec, s));
+    } else {
+        mela_status st = enqueue_panels();
+        if (st != MELA_OK) return st;
     }
     int r_dev = 0;
     MELA_HIP(hipMemcpyAsync(&r_dev, &meta->r, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -19,31 +19,57 @@ __device__ __forceinline__ int ffs64(uint64_t x) { return __ffsll((unsigned long
 
 // Wave-level Gauss-Jordan over the 64-bit words v (one per lane; lanes with
 // valid == false take no part).  Afterwards pivot lanes hold the RREF basis of
-// span{v} (pbit = pivot bit), non-pivot lanes hold 0; comb (if tracked) is the
-// set of original lanes XORed into each lane.  Pivot of column c = the lowest
-// non-pivot lane with bit c set.
+// span{v} (pbit = pivot bit = lowest set bit of the lane's word), non-pivot
+// lanes hold 0; comb (if tracked) is the set of original lanes XORed into each
+// lane.  Pivot of column c = the lowest non-pivot lane with bit c set.  The
+// pivot set is kept as a wave-uniform mask so each column costs one ballot,
+// one s_ff1, two (four) v_readlane and an exec-masked XOR.
 template <bool TRACK>
 __device__ __forceinline__ void wave_gj(uint64_t &v, uint64_t &comb, bool valid, bool &piv,
                                         int &pbit, int lane) {
-    piv = false;
-    pbit = -1;
-    if (!valid) v = 0;
+    uint32_t lo = valid ? (uint32_t)v : 0u, hi = valid ? (uint32_t)(v >> 32) : 0u;
+    uint32_t clo = (uint32_t)comb, chi = (uint32_t)(comb >> 32);
+    uint64_t pivmask = 0;  // wave-uniform
+#pragma unroll
     for (int c = 0; c < 64; ++c) {
-        const bool has = ((v >> c) & 1ull) != 0;
-        const uint64_t bal = __ballot(has && !piv);
-        if (bal == 0) continue;
-        const int f = ffs64(bal);
-        const uint64_t pv = readlane64(v, f);
-        uint64_t pc = 0;
-        if (TRACK) pc = readlane64(comb, f);
-        if (lane == f) {
-            piv = true;
-            pbit = c;
-        } else if (has) {
-            v ^= pv;
-            if (TRACK) comb ^= pc;
+        const uint32_t word = c < 32 ? lo : hi;
+        const bool has = (word >> (c & 31)) & 1u;
+        const uint64_t hb = __ballot(has);
+        const uint64_t cand = hb & ~pivmask;
+        if (cand == 0) continue;
+        const int f = ffs64(cand);
+        const uint32_t plo = __builtin_amdgcn_readlane(lo, f);
+        const uint32_t phi = __builtin_amdgcn_readlane(hi, f);
+        uint32_t pclo = 0, pchi = 0;
+        if (TRACK) {
+            pclo = __builtin_amdgcn_readlane(clo, f);
+            pchi = __builtin_amdgcn_readlane(chi, f);
+        }
+        pivmask |= 1ull << f;
+        if (has && lane != f) {
+            lo ^= plo;
+            hi ^= phi;
+            if (TRACK) {
+                clo ^= pclo;
+                chi ^= pchi;
+            }
         }
     }
+    v = ((uint64_t)hi << 32) | lo;
+    if (TRACK) comb = ((uint64_t)chi << 32) | clo;
+    piv = (pivmask >> lane) & 1ull;
+    pbit = piv ? ffs64(v) : -1;
+}
+
+// XOR of v over the 64 lanes of the wave (every lane receives the result).
+__device__ __forceinline__ uint64_t wave_xor64(uint64_t v) {
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+        lo ^= __shfl_xor(lo, s, 64);
+        hi ^= __shfl_xor(hi, s, 64);
+    }
+    return ((uint64_t)hi << 32) | lo;
 }
 
 // LDS state of one panel selection.
@@ -59,6 +85,7 @@ struct SelectLDS {
     uint32_t mv[64];
     unsigned long long pm;
     int nsel;
+    int fast;  // the fast path selected 64 rows (combinations already final)
 };
 
 // Result of a panel selection (valid on every thread after the call).
@@ -83,9 +110,62 @@ __device__ PanelOut wg_panel_select(SelectLDS<NT> &L, ACT *act, int n_act, int g
     if (tid == 0) {
         L.pm = 0;
         L.nsel = 0;
+        L.fast = 0;
     }
     __syncthreads();
-    for (int base = 0; base < n_act; base += NT) {
+    // ---- fast path (dense panels): one tracked Gauss-Jordan over the first 64
+    // active rows, then single insertions of the next rows until 64 pivots; a
+    // panel that does not reach 64 pivots this way takes the general path.
+    const int n0 = n_act < NT ? n_act : NT;
+    if (n_act >= 64) {
+        if (tid < n0) L.cy[tid] = row((int)act[tid])[g];
+        __syncthreads();
+        if (wave == 0) {
+            uint64_t v = L.cy[lane];
+            uint64_t comb = 1ull << lane;
+            bool piv;
+            int pbit;
+            wave_gj<true>(v, comb, true, piv, pbit, lane);
+            uint64_t pivl = __ballot(piv);  // pivot lanes = selection slots
+            int slotpos = lane;              // active-list position of slot `lane`
+            const int lim = n0 < 64 + 128 ? n0 : 64 + 128;
+            for (int pos = 64; pos < lim && pivl != ~0ull; ++pos) {
+                const uint64_t x = L.cy[pos];
+                if (x == 0) continue;
+                const bool use = piv && ((x >> pbit) & 1ull);
+                const uint64_t red = wave_xor64(use ? v : 0ull);
+                const uint64_t cred = wave_xor64(use ? comb : 0ull);
+                const uint64_t z = x ^ red;  // x reduced by the RREF basis
+                if (z == 0) continue;
+                const int sl = ffs64(~pivl);  // a free slot
+                const uint64_t cz = cred ^ (1ull << sl);
+                const int p = ffs64(z);
+                if (piv && ((v >> p) & 1ull)) {
+                    v ^= z;
+                    comb ^= cz;
+                }
+                if (lane == sl) {
+                    v = z;
+                    comb = cz;
+                    piv = true;
+                    pbit = p;
+                    slotpos = pos;
+                }
+                pivl |= 1ull << sl;
+            }
+            if (pivl == ~0ull) {  // 64 pivots: slot = lane
+                L.selpos[lane] = slotpos;
+                comb_out[pbit] = comb;
+                if (lane == 0) {
+                    L.pm = ~0ull;
+                    L.nsel = 64;
+                    L.fast = 1;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (int base = 0; base < n_act && !L.fast; base += NT) {
         const int pos = base + tid;
         uint64_t x = 0;
         if (pos < n_act) x = row((int)act[pos])[g];
@@ -183,17 +263,19 @@ __device__ PanelOut wg_panel_select(SelectLDS<NT> &L, ACT *act, int n_act, int g
         if (is_mover) L.mv[popc64(mb & below)] = (uint32_t)act[n_new + lane];
         __threadfence_block();
         if (is_hole) act[my_pos] = (ACT)L.mv[popc64(hb & below)];
-        // RREF of the selected rows' panel words, tracking combinations
-        uint64_t v = 0, comb = 0;
-        if (lane < nsel) {
-            v = row(L.selrow[lane])[g];
-            comb = 1ull << lane;
-        }
-        bool piv;
-        int pbit;
-        wave_gj<true>(v, comb, lane < nsel, piv, pbit, lane);
         if (lane < nsel) selrow_out[lane] = L.selrow[lane];
-        if (piv) comb_out[pbit] = comb;
+        if (!L.fast) {
+            // RREF of the selected rows' panel words, tracking combinations
+            uint64_t v = 0, comb = 0;
+            if (lane < nsel) {
+                v = row(L.selrow[lane])[g];
+                comb = 1ull << lane;
+            }
+            bool piv;
+            int pbit;
+            wave_gj<true>(v, comb, lane < nsel, piv, pbit, lane);
+            if (piv) comb_out[pbit] = comb;
+        }
     }
     __syncthreads();
     out.pm = L.pm;

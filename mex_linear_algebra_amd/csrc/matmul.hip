@@ -35,8 +35,10 @@ k_andpop(const uint64_t *__restrict__ A, size_t lda, const uint64_t *__restrict_
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int i0 = blockIdx.y * AP_ROWS;
-    const int j0 = blockIdx.x * AP_COLS;
+    // blockIdx.x walks row tiles fastest: the workgroups resident together on
+    // an XCD share one B^T column tile (1 MiB at k = 32768) in that XCD's L2
+    const int i0 = blockIdx.x * AP_ROWS;
+    const int j0 = blockIdx.y * AP_COLS;
     const uint32_t *A32 = reinterpret_cast<const uint32_t *>(A);
     const uint32_t *B32 = reinterpret_cast<const uint32_t *>(Bt);
     const size_t lda32 = lda * 2, ldb32 = ldb * 2;
@@ -261,8 +263,8 @@ mela_status matmul_device(mela_ctx *ctx, const uint64_t *A, size_t lda, const ui
         st = launch_transpose(s, B, ldb, k, n, (uint64_t *)bt, ldbt);
         if (st != MELA_OK) return st;
         const int kd = (int)((k + 31) / 32);
-        dim3 grid((unsigned)((n + AP_COLS - 1) / AP_COLS), (unsigned)((m + AP_ROWS - 1) / AP_ROWS));
-        if (grid.y > 65535) return set_error(MELA_ERR_INVALID, "too many rows for one launch");
+        dim3 grid((unsigned)((m + AP_ROWS - 1) / AP_ROWS), (unsigned)((n + AP_COLS - 1) / AP_COLS));
+        if (grid.y > 65535) return set_error(MELA_ERR_INVALID, "too many columns for one launch");
         hipLaunchKernelGGL(k_andpop, grid, dim3(256), 0, s, A, lda, (const uint64_t *)bt, ldbt, C,
                            ldc, (int)m, (int)n, kd);
         MELA_HIP(hipGetLastError());
@@ -289,6 +291,11 @@ mela_status matmul_device(mela_ctx *ctx, const uint64_t *A, size_t lda, const ui
         MELA_HIP(hipGetLastError());
         (void)nw;
         return MELA_OK;
+    }
+    if (algo == MELA_ALGO_MFMA_I8 || algo == MELA_ALGO_MFMA_FP4) {
+        if (k >= (1u << 24))
+            return set_error(MELA_ERR_INVALID, "MFMA path needs k < 2^24 (exact accumulation)");
+        return matmul_mfma(ctx, A, lda, B, ldb, C, ldc, m, k, n, algo == MELA_ALGO_MFMA_FP4);
     }
     return set_error(MELA_ERR_INVALID, "unknown matmul algorithm");
 }

@@ -186,6 +186,7 @@ void worker_main(mela_server *s) {
     mela_ctx *ctx = nullptr;
     mela_status cst = mela_ctx_create(s->device, &ctx);
     std::string cerr = cst == MELA_OK ? "" : mela_last_error();
+    if (ctx) ctx->capture_ok = false;  // other workers issue synchronous HIP calls
     std::unique_lock<std::mutex> lk(s->mu);
     for (;;) {
         s->cv_jobs.wait(lk, [&] { return s->stop || !s->jobs.empty(); });
@@ -214,8 +215,9 @@ void worker_main(mela_server *s) {
             for (Req *q : rq) mats.push_back(q->r.A);
             const bool want_null = job.kind == Job::BATCH_NULL;
             mela_batch *b = nullptr;
-            mela_status st = mela_batch_create(ctx, mats.data(), mats.size(),
-                                               want_null ? MELA_BATCH_NULL : 0, &b);
+            mela_status st = mela_batch_create(
+                ctx, mats.data(), mats.size(),
+                (want_null ? MELA_BATCH_NULL : 0) | MELA_BATCH_CTX_SCRATCH, &b);
             if (st == MELA_OK) st = mela_batch_run(b);
             std::vector<uint32_t> ranks(rq.size());
             std::vector<mela_bitmat> nulls(rq.size());
