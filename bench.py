@@ -310,6 +310,7 @@ def bench_serve(args, mela):
         picks = rng.integers(0, 1 << 30, nreq)
         tickets = []
         done_at = []
+        errors = []
         lock = threading.Lock()
 
         def drain():
@@ -323,7 +324,10 @@ def bench_serve(args, mela):
                     time.sleep(0.0005)
                     continue
                 t = tickets[i]
-                srv.wait(t)
+                try:
+                    srv.wait(t)
+                except mela.MelaError as e:  # counted, reported below
+                    errors.append(str(e))
                 done_at.append(time.perf_counter())
                 srv.release(t)
                 i += 1
@@ -350,7 +354,8 @@ def bench_serve(args, mela):
         row = {"offered_req_per_s": rate, "requests": nreq, "achieved_req_per_s": round(nreq / span, 1),
                "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
                "mean_ms": round(st["mean_ms"], 3), "jobs": st["jobs"],
-               "batched_requests": st["batched_requests"], "hbm_peak_GB": round(st["hbm_peak"] / 1e9, 3)}
+               "batched_requests": st["batched_requests"], "hbm_peak_GB": round(st["hbm_peak"] / 1e9, 3),
+               "failed": len(errors)}
         results.append(row)
         print(json.dumps({"workload": "serve", **row}), flush=True)
     sat = max(r["achieved_req_per_s"] for r in results)
@@ -364,6 +369,49 @@ def bench_serve(args, mela):
     for A, B in mm:
         ctx.free(A)
         ctx.free(B)
+
+
+def bench_c1(args, mela):
+    """2048 x 2048 logical A, B (column-major host arrays, as MATLAB passes
+    them): matmul (both entry points), rank(A) and null(A) end to end through
+    the C ABI -- host copies, packing, kernels and unpacking included -- next to
+    the oracle's restatement of the reference on the same inputs."""
+    import numpy as np
+    from oracle import oracle as orc
+    rng = np.random.default_rng(1)
+    n = 2048
+    A = rng.integers(0, 2, (n, n)).astype(bool)
+    B = rng.integers(0, 2, (n, n)).astype(bool)
+    ctx = mela.Context(0)
+
+    def timeit(fn, reps):
+        fn()
+        t = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            t.append(time.perf_counter() - t0)
+        t.sort()
+        return t[len(t) // 2] * 1e3
+
+    rows = []
+    for name, gfn, cfn, work in (
+            ("matmul_gf2", lambda: mela.mela_matmul_gf2(A, B, ctx=ctx),
+             lambda: orc.matmul(A, B, "gf2"), 2.0 * n ** 3),
+            ("matmul_m4ri", lambda: mela.mela_matmul_m4ri(A, B, ctx=ctx),
+             lambda: orc.matmul(A, B, "m4ri"), 2.0 * n ** 3),
+            ("rank_gf2", lambda: mela.mela_rank_gf2(A, ctx=ctx), lambda: orc.rank(A, "gf2"), None),
+            ("rank_m4ri", lambda: mela.mela_rank_m4ri(A, ctx=ctx), lambda: orc.rank(A, "m4ri"), None),
+            ("null_gf2", lambda: mela.mela_null_gf2(A, ctx=ctx), lambda: orc.null(A, "gf2"), None),
+            ("null_m4ri", lambda: mela.mela_null_m4ri(A, ctx=ctx), lambda: orc.null(A, "m4ri"), None)):
+        g = timeit(gfn, 5)
+        c = timeit(cfn, 3)
+        row = {"workload": f"C1 {name} 2048x2048 logical, end to end (host in/out)",
+               "gpu_ms": round(g, 3), "cpu_oracle_ms": round(c, 3), "speedup": round(c / g, 1),
+               "cpu_threads": orc.num_threads()}
+        if work:
+            row["gpu_Tbitops_incl_pcie"] = round(work / (g * 1e-3) / 1e12, 2)
+        print(json.dumps(row), flush=True)
 
 
 def bench_copy(args, mela):
@@ -386,7 +434,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="matmul", choices=["matmul", "elim", "batch", "serve", "copy"])
+    ap.add_argument("--workload", default="matmul", choices=["matmul", "elim", "batch", "serve", "c1", "copy"])
     ap.add_argument("--algo", default="andpop",
                     choices=["auto", "m4rm", "andpop", "mfma_i8", "mfma_fp4"])
     ap.add_argument("--n", type=int, default=32768)
@@ -408,7 +456,7 @@ def main():
         sys.exit(2)
     import mex_linear_algebra_amd as mela
     {"matmul": bench_matmul, "elim": bench_elim, "batch": bench_batch, "serve": bench_serve,
-     "copy": bench_copy}[args.workload](args, mela)
+     "c1": bench_c1, "copy": bench_copy}[args.workload](args, mela)
 
 
 if __name__ == "__main__":

@@ -24,8 +24,6 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
-constexpr int MF_ROWS = 128;   // workgroup tile rows (A)
-constexpr int MF_COLS = 128;   // workgroup tile columns (B)
 constexpr int MF_UNITS = 4;    // 16-byte units of k per row per LDS stage (64 bytes)
 
 // bits -> expanded bytes.  FP4: each source bit becomes one nibble (0x0 or
@@ -68,74 +66,96 @@ k_expand(const uint64_t *__restrict__ src, size_t lds, size_t rows, size_t kword
     }
 }
 
-// C tile 128 x 128 per workgroup; 4 waves in a 2 x 2 grid, 64 x 64 each
-// (2 x 2 MFMA 32x32 tiles).  k advances 64 bytes of expanded data per LDS
-// stage (two MFMA k-blocks of 32 bytes), double buffered.  LDS is stored
-// fragment-major ([unit][row]) so each lane's 16-byte fragment read is
-// contiguous across the wave.
-template <bool FP4>
+// C tile (2*WT) x (2*WT) per workgroup; 4 waves in a 2 x 2 grid, each WT x WT
+// (WT/32 x WT/32 MFMA 32x32 tiles, accumulators in AGPRs).  k advances 64
+// bytes of expanded data per LDS stage (two MFMA k-blocks of 32 bytes),
+// double buffered.  LDS is stored fragment-major ([unit][row]) so each
+// lane's 16-byte fragment read is contiguous across the wave.  WT = 128 gives
+// 256 x 256 tiles: 32 KiB of operands per stage for 128 MFMAs.
+template <bool FP4, int WT>
 __global__ void __launch_bounds__(256)
 k_mfma_gf2(const uint8_t *__restrict__ A, size_t lda, const uint8_t *__restrict__ B,
            size_t ldb, uint64_t *__restrict__ C, size_t ldc, int m, int n, int kbytes) {
-    __shared__ uint4 sA[2][MF_UNITS][MF_ROWS];
-    __shared__ uint4 sB[2][MF_UNITS][MF_COLS];
+    constexpr int TR = WT / 32;          // MFMA tiles per wave, each direction
+    constexpr int ROWS = 2 * WT;         // workgroup tile rows (= columns)
+    constexpr int LPT = ROWS * MF_UNITS / 256;  // 16-byte chunks per thread per operand
+    __shared__ uint4 sA[2][MF_UNITS][ROWS];
+    __shared__ uint4 sB[2][MF_UNITS][ROWS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
     // row tiles walk fastest so resident workgroups share B tiles in L2
-    const int i0 = blockIdx.x * MF_ROWS, j0 = blockIdx.y * MF_COLS;
+    const int i0 = blockIdx.x * ROWS, j0 = blockIdx.y * ROWS;
 
-    // loader: thread t -> row t/2, units (t%2)*2 and (t%2)*2+1
-    const int lrow = tid >> 1, lu = (tid & 1) * 2;
+    // loader: thread t takes row t % ROWS of units t / ROWS + q * (256 / ROWS);
+    // consecutive threads read consecutive rows (each 16 bytes of one unit).
+    // Named registers (no array) keep the staged chunks out of scratch.
+    const int lrow = tid % ROWS, lu0 = tid / ROWS;
+    constexpr int USTEP = 256 / ROWS;
     const bool a_ok = i0 + lrow < m, b_ok = j0 + lrow < n;
     const uint8_t *a_src = A + (size_t)(a_ok ? i0 + lrow : 0) * lda;
     const uint8_t *b_src = B + (size_t)(b_ok ? j0 + lrow : 0) * ldb;
-    uint4 ra0, ra1, rb0, rb1;
+    uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
     const uint4 zero4 = make_uint4(0, 0, 0, 0);
-#define MF_GLOAD(kb0)                                                                        \
-    do {                                                                                     \
-        const int o0 = (kb0) + lu * 16, o1 = o0 + 16;                                         \
-        ra0 = (a_ok && o0 < kbytes) ? *reinterpret_cast<const uint4 *>(a_src + o0) : zero4;  \
-        ra1 = (a_ok && o1 < kbytes) ? *reinterpret_cast<const uint4 *>(a_src + o1) : zero4;  \
-        rb0 = (b_ok && o0 < kbytes) ? *reinterpret_cast<const uint4 *>(b_src + o0) : zero4;  \
-        rb1 = (b_ok && o1 < kbytes) ? *reinterpret_cast<const uint4 *>(b_src + o1) : zero4;  \
+#define MF_LD1(q, RA, RB)                                                                       \
+    do {                                                                                        \
+        const int off = kb0 + (lu0 + (q) * USTEP) * 16;                                         \
+        const bool in = off < kbytes;                                                           \
+        RA = (a_ok && in) ? *reinterpret_cast<const uint4 *>(a_src + off) : zero4;              \
+        RB = (b_ok && in) ? *reinterpret_cast<const uint4 *>(b_src + off) : zero4;              \
     } while (0)
-#define MF_SSTORE(buf)              \
-    do {                            \
-        sA[buf][lu][lrow] = ra0;     \
-        sA[buf][lu + 1][lrow] = ra1; \
-        sB[buf][lu][lrow] = rb0;     \
-        sB[buf][lu + 1][lrow] = rb1; \
-    } while (0)
+    auto gload = [&](int kb0) __attribute__((always_inline)) {
+        MF_LD1(0, ra0, rb0);
+        if constexpr (LPT > 1) MF_LD1(1, ra1, rb1);
+        if constexpr (LPT > 2) MF_LD1(2, ra2, rb2);
+        if constexpr (LPT > 3) MF_LD1(3, ra3, rb3);
+    };
+#undef MF_LD1
+    auto sstore = [&](int buf) __attribute__((always_inline)) {
+        sA[buf][lu0][lrow] = ra0;
+        sB[buf][lu0][lrow] = rb0;
+        if constexpr (LPT > 1) {
+            sA[buf][lu0 + USTEP][lrow] = ra1;
+            sB[buf][lu0 + USTEP][lrow] = rb1;
+        }
+        if constexpr (LPT > 2) {
+            sA[buf][lu0 + 2 * USTEP][lrow] = ra2;
+            sB[buf][lu0 + 2 * USTEP][lrow] = rb2;
+        }
+        if constexpr (LPT > 3) {
+            sA[buf][lu0 + 3 * USTEP][lrow] = ra3;
+            sB[buf][lu0 + 3 * USTEP][lrow] = rb3;
+        }
+    };
 
     using Acc = typename std::conditional<FP4, v16f, v16i>::type;
-    Acc acc[2][2];
+    Acc acc[TR][TR];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < TR; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < TR; ++b)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
 
     const int nstage = (kbytes + MF_UNITS * 16 - 1) / (MF_UNITS * 16);
-    MF_GLOAD(0);
-    MF_SSTORE(0);
+    gload(0);
+    sstore(0);
     __syncthreads();
     const int half = lane >> 5, l32 = lane & 31;
     for (int st = 0; st < nstage; ++st) {
         const int buf = st & 1;
-        if (st + 1 < nstage) MF_GLOAD((st + 1) * MF_UNITS * 16);
+        if (st + 1 < nstage) gload((st + 1) * MF_UNITS * 16);
 #pragma unroll
         for (int kb = 0; kb < 2; ++kb) {
-            uint4 fa[2], fb[2];
+            uint4 fa[TR], fb[TR];
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                fa[t] = sA[buf][kb * 2 + half][wr * 64 + t * 32 + l32];
-                fb[t] = sB[buf][kb * 2 + half][wc * 64 + t * 32 + l32];
+            for (int t = 0; t < TR; ++t) {
+                fa[t] = sA[buf][kb * 2 + half][wr * WT + t * 32 + l32];
+                fb[t] = sB[buf][kb * 2 + half][wc * WT + t * 32 + l32];
             }
 #pragma unroll
-            for (int tr = 0; tr < 2; ++tr)
+            for (int tr = 0; tr < TR; ++tr)
 #pragma unroll
-                for (int tc = 0; tc < 2; ++tc) {
+                for (int tc = 0; tc < TR; ++tc) {
                     if constexpr (FP4) {
                         v8i a8 = {(int)fa[tr].x, (int)fa[tr].y, (int)fa[tr].z, (int)fa[tr].w, 0, 0, 0, 0};
                         v8i b8 = {(int)fb[tc].x, (int)fb[tc].y, (int)fb[tc].z, (int)fb[tc].w, 0, 0, 0, 0};
@@ -148,36 +168,32 @@ k_mfma_gf2(const uint8_t *__restrict__ A, size_t lda, const uint8_t *__restrict_
                     }
                 }
         }
-        if (st + 1 < nstage) MF_SSTORE(buf ^ 1);
+        if (st + 1 < nstage) sstore(buf ^ 1);
         __syncthreads();
     }
 
-#undef MF_GLOAD
-#undef MF_SSTORE
-
     // 32x32 accumulator layout: register r of lane l holds row
-    // 8*(r/4) + 4*(l/32) + r%4 and column l%32 of its tile.  Bits of the two
-    // column tiles of a wave form one 64-bit word per row.
+    // 8*(r/4) + 4*(l/32) + r%4 and column l%32 of its tile.  Column tiles
+    // (2c, 2c+1) of a wave form one 64-bit word per row.
     const int nw = (n + 63) >> 6;
 #pragma unroll
-    for (int tr = 0; tr < 2; ++tr)
+    for (int tr = 0; tr < TR; ++tr)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            uint64_t bal[2];
+            const int rbase = i0 + wr * WT + tr * 32 + 8 * (r >> 2) + (r & 3);
 #pragma unroll
-            for (int tc = 0; tc < 2; ++tc) {
-                const int v = (int)acc[tr][tc][r];
-                bal[tc] = __ballot(v & 1);
-            }
-            const int rbase = i0 + wr * 64 + tr * 32 + 8 * (r >> 2) + (r & 3);
-            const int w = (j0 + wc * 64) >> 6;
-            // lane 0 writes the row of lanes 0-31, lane 32 the row of lanes 32-63
-            if ((lane & 31) == 0 && w < nw) {
-                const int sh = lane >> 5;  // 0 or 1
-                const int row = rbase + 4 * sh;
-                const uint64_t lo = sh ? (bal[0] >> 32) : (bal[0] & 0xffffffffull);
-                const uint64_t hi = sh ? (bal[1] >> 32) : (bal[1] & 0xffffffffull);
-                if (row < m) C[(size_t)row * ldc + w] = lo | (hi << 32);
+            for (int tc = 0; tc < TR; tc += 2) {
+                const uint64_t b0 = __ballot((int)acc[tr][tc][r] & 1);
+                const uint64_t b1 = __ballot((int)acc[tr][tc + 1][r] & 1);
+                const int w = (j0 + wc * WT + tc * 32) >> 6;
+                // lane 0 writes the row of lanes 0-31, lane 32 that of lanes 32-63
+                if ((lane & 31) == 0 && w < nw) {
+                    const int sh = lane >> 5;
+                    const int row = rbase + 4 * sh;
+                    const uint64_t lo = sh ? (b0 >> 32) : (b0 & 0xffffffffull);
+                    const uint64_t hi = sh ? (b1 >> 32) : (b1 & 0xffffffffull);
+                    if (row < m) C[(size_t)row * ldc + w] = lo | (hi << 32);
+                }
             }
         }
 }
@@ -209,14 +225,17 @@ mela_status matmul_mfma(mela_ctx *ctx, const uint64_t *A, size_t lda, const uint
         hipLaunchKernelGGL(k_expand<false>, dim3((unsigned)((tb + 255) / 256)), dim3(256), 0, s,
                            (const uint64_t *)pbt, kw, n, kw, (uint8_t *)pb, row_bytes);
     }
-    dim3 grid((unsigned)((m + MF_ROWS - 1) / MF_ROWS), (unsigned)((n + MF_COLS - 1) / MF_COLS));
+    // 128 x 128 tiles (256 x 256 tiles measured slower: one wave per SIMD
+    // leaves the global-load latency of each stage exposed)
+    constexpr unsigned T = 128;
+    dim3 grid((unsigned)((m + T - 1) / T), (unsigned)((n + T - 1) / T));
     if (grid.y > 65535) return set_error(MELA_ERR_INVALID, "too many columns for one launch");
+    const uint8_t *xa = (const uint8_t *)pa, *xb = (const uint8_t *)pb;
+    const int kb = (int)row_bytes;
     if (fp4)
-        hipLaunchKernelGGL(k_mfma_gf2<true>, grid, dim3(256), 0, s, (const uint8_t *)pa, row_bytes,
-                           (const uint8_t *)pb, row_bytes, C, ldc, (int)m, (int)n, (int)row_bytes);
+        hipLaunchKernelGGL((k_mfma_gf2<true, 64>), grid, dim3(256), 0, s, xa, row_bytes, xb, row_bytes, C, ldc, (int)m, (int)n, kb);
     else
-        hipLaunchKernelGGL(k_mfma_gf2<false>, grid, dim3(256), 0, s, (const uint8_t *)pa, row_bytes,
-                           (const uint8_t *)pb, row_bytes, C, ldc, (int)m, (int)n, (int)row_bytes);
+        hipLaunchKernelGGL((k_mfma_gf2<false, 64>), grid, dim3(256), 0, s, xa, row_bytes, xb, row_bytes, C, ldc, (int)m, (int)n, kb);
     MELA_HIP(hipGetLastError());
     return MELA_OK;
 }
