@@ -62,6 +62,13 @@ void mela_ctx_destroy(mela_ctx *ctx);
 /* HIP stream (hipStream_t) every kernel of this context runs on */
 void *mela_ctx_stream(mela_ctx *ctx);
 mela_status mela_ctx_sync(mela_ctx *ctx);
+/* Kernel timing (no reference counterpart; for benchmarks and profiling):
+   between begin and end, every mela_matmul_packed call on this context
+   records HIP events around its dominant kernel (k_andpop, k_m4rm or the
+   MFMA product kernel); end synchronises the stream and returns the summed
+   kernel time and the number of timed launches. */
+mela_status mela_ctx_profile_begin(mela_ctx *ctx);
+mela_status mela_ctx_profile_end(mela_ctx *ctx, double *total_ms, int *launches);
 /* message and MATLAB identifier of the last failure on this thread */
 const char *mela_last_error(void);
 const char *mela_last_error_id(void);

@@ -103,6 +103,10 @@ def load():
         L.mela_ctx_stream.restype = vp
         L.mela_ctx_sync.argtypes = [ctxp]
         L.mela_ctx_sync.restype = st
+        L.mela_ctx_profile_begin.argtypes = [ctxp]
+        L.mela_ctx_profile_begin.restype = st
+        L.mela_ctx_profile_end.argtypes = [ctxp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.mela_ctx_profile_end.restype = st
         L.mela_last_error.restype = C.c_char_p
         L.mela_last_error_id.restype = C.c_char_p
         L.mela_version.restype = C.c_char_p
@@ -215,6 +219,16 @@ class Context:
 
     def sync(self):
         _check(self.lib.mela_ctx_sync(self.handle))
+
+    def profile_begin(self):
+        """Start timing the dominant kernel of every matmul_packed call."""
+        _check(self.lib.mela_ctx_profile_begin(self.handle))
+
+    def profile_end(self):
+        """Stop timing; returns (summed kernel ms, number of timed launches)."""
+        ms, cnt = C.c_double(), C.c_int()
+        _check(self.lib.mela_ctx_profile_end(self.handle, C.byref(ms), C.byref(cnt)))
+        return float(ms.value), int(cnt.value)
 
     # ---- device-resident packed helpers ----
     def bitmat(self, rows: int, cols: int) -> BitMatrix:

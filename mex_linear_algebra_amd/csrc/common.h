@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "mela.h"
@@ -59,9 +60,32 @@ struct mela_ctx {
     // graph capture of repeated eliminations; off for contexts that share the
     // process with other threads doing synchronous HIP calls (server workers)
     bool capture_ok = true;
+    // event pairs around the dominant kernel of each product while profiling
+    bool prof_on = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof;
 };
 
 namespace mela {
+
+// Records an event on the context stream before (start) or after the
+// dominant kernel of a product when profiling is on; never used inside a
+// captured graph.
+inline void prof_mark(mela_ctx *ctx, bool start) {
+    if (!ctx->prof_on) return;
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
+    }
+    (void)hipEventRecord(e, ctx->stream);
+    if (start) {
+        ctx->prof.push_back({e, nullptr});
+    } else if (!ctx->prof.empty() && ctx->prof.back().second == nullptr) {
+        ctx->prof.back().second = e;
+    } else {
+        (void)hipEventDestroy(e);
+    }
+}
 
 // Returns a device buffer of at least `bytes` for `slot` (kept across calls).
 mela_status scratch_get(mela_ctx *ctx, int slot, size_t bytes, void **out);

@@ -106,10 +106,19 @@ mela_status mela_ctx_default(mela_ctx **out) {
     return MELA_OK;
 }
 
+static void prof_clear(mela_ctx *ctx) {
+    for (auto &pr : ctx->prof) {
+        if (pr.first) (void)hipEventDestroy(pr.first);
+        if (pr.second) (void)hipEventDestroy(pr.second);
+    }
+    ctx->prof.clear();
+}
+
 void mela_ctx_destroy(mela_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    prof_clear(ctx);
     for (auto &g : ctx->elim_graphs) {
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
         if (g.graph) (void)hipGraphDestroy(g.graph);
@@ -125,6 +134,39 @@ void *mela_ctx_stream(mela_ctx *ctx) { return ctx ? (void *)ctx->stream : nullpt
 mela_status mela_ctx_sync(mela_ctx *ctx) {
     if (!ctx) return set_error(MELA_ERR_INVALID, "null context");
     MELA_HIP(hipStreamSynchronize(ctx->stream));
+    return MELA_OK;
+}
+
+mela_status mela_ctx_profile_begin(mela_ctx *ctx) {
+    if (!ctx) return set_error(MELA_ERR_INVALID, "null context");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    MELA_HIP(hipSetDevice(ctx->device));
+    prof_clear(ctx);
+    ctx->prof_on = true;
+    return MELA_OK;
+}
+
+mela_status mela_ctx_profile_end(mela_ctx *ctx, double *total_ms, int *launches) {
+    if (!ctx || !total_ms || !launches) return set_error(MELA_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    MELA_HIP(hipSetDevice(ctx->device));
+    ctx->prof_on = false;
+    MELA_HIP(hipStreamSynchronize(ctx->stream));
+    double sum = 0.0;
+    int cnt = 0;
+    for (auto &pr : ctx->prof) {
+        if (!pr.first || !pr.second) continue;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+            sum += ms;
+            ++cnt;
+        } else {
+            (void)hipGetLastError();
+        }
+    }
+    prof_clear(ctx);
+    *total_ms = sum;
+    *launches = cnt;
     return MELA_OK;
 }
 

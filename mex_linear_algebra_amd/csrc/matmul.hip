@@ -250,9 +250,13 @@ mela_status matmul_device(mela_ctx *ctx, const uint64_t *A, size_t lda, const ui
     if (k == 0) return launch_fill_zero(s, C, m * ldc * sizeof(uint64_t));  // C = 0
     if (m > (size_t)INT32_MAX || n > (size_t)INT32_MAX || k > (size_t)INT32_MAX)
         return set_error(MELA_ERR_INVALID, "matrix dimension exceeds 2^31-1");
-    // AUTO: the AND+parity kernel measured fastest on MI355X at every shape
-    // tried so far (32768^3: 44.8 ms vs 50.4 ms for M4RM)
-    if (algo == MELA_ALGO_AUTO) algo = MELA_ALGO_ANDPOP;
+    // AUTO: the fp4 matrix-core path once the product is large enough to
+    // amortise expanding the operands (m*n*k >= 2^33 bit-products, exact
+    // while k < 2^24); the AND+parity kernel below that.
+    if (algo == MELA_ALGO_AUTO)
+        algo = ((double)m * (double)n * (double)k >= 8589934592.0 && k < (1u << 24))
+                   ? MELA_ALGO_MFMA_FP4
+                   : MELA_ALGO_ANDPOP;
 
     if (algo == MELA_ALGO_ANDPOP) {
         // Bt = B^T (n x ceil(k/64) words)
@@ -265,8 +269,10 @@ mela_status matmul_device(mela_ctx *ctx, const uint64_t *A, size_t lda, const ui
         const int kd = (int)((k + 31) / 32);
         dim3 grid((unsigned)((m + AP_ROWS - 1) / AP_ROWS), (unsigned)((n + AP_COLS - 1) / AP_COLS));
         if (grid.y > 65535) return set_error(MELA_ERR_INVALID, "too many columns for one launch");
+        prof_mark(ctx, true);
         hipLaunchKernelGGL(k_andpop, grid, dim3(256), 0, s, A, lda, (const uint64_t *)bt, ldbt, C,
                            ldc, (int)m, (int)n, kd);
+        prof_mark(ctx, false);
         MELA_HIP(hipGetLastError());
         return MELA_OK;
     }
@@ -286,8 +292,10 @@ mela_status matmul_device(mela_ctx *ctx, const uint64_t *A, size_t lda, const ui
             mela_status st = launch_fill_zero(s, C, m * ldc * sizeof(uint64_t));
             if (st != MELA_OK) return st;
         }
+        prof_mark(ctx, true);
         hipLaunchKernelGGL(k_m4rm, dim3(gx, gy, splits), dim3(1024), 0, s, A, lda, B, ldb, C, ldc,
                            (int)m, (int)n, (int)k, per, atomic);
+        prof_mark(ctx, false);
         MELA_HIP(hipGetLastError());
         (void)nw;
         return MELA_OK;

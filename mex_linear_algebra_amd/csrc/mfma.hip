@@ -12,6 +12,10 @@
 // values of k), so one kernel template serves both formats.  A and B
 // fragments use the same lane/byte -> k assignment, which makes the k order
 // inside a fragment irrelevant to the result.
+//
+// Expanded operands are padded with zero rows to a multiple of 256 and with
+// zero bytes to a row pitch that is a whole number of k stages, so the inner
+// loops carry no bounds checks.
 #include <type_traits>
 
 #include "common.h"
@@ -24,109 +28,165 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
-constexpr int MF_UNITS = 4;    // 16-byte units of k per row per LDS stage (64 bytes)
+constexpr int XROWS = 256;         // row padding of the expanded operands
+constexpr int RING = 4;            // k stages in the LDS ring of k_mfma_ring
+constexpr int RING_STAGE = 64;     // bytes of k per row per ring stage
+constexpr int SMALL_STAGE = 128;   // bytes of k per row per stage of k_mfma_small
+constexpr int KPAD = 256;          // row pitch multiple: lcm(RING * RING_STAGE, SMALL_STAGE)
+static_assert(KPAD % (RING * RING_STAGE) == 0 && KPAD % SMALL_STAGE == 0, "pitch multiple");
 
-// bits -> expanded bytes.  FP4: each source bit becomes one nibble (0x0 or
-// 0x2), two per byte (element 2i low nibble); INT8: one byte (0 or 1).
+// bits -> expanded bytes, one thread per 32-byte output chunk.  FP4: each
+// source bit becomes one nibble (0x0 or 0x2), two per byte (element 2i in the
+// low nibble), one source word per chunk; INT8: one byte (0 or 1), half a
+// source word per chunk.  Rows >= rows and words >= kw are written as zeros.
 template <bool FP4>
 __global__ void __launch_bounds__(256)
-k_expand(const uint64_t *__restrict__ src, size_t lds, size_t rows, size_t kwords,
-         uint8_t *__restrict__ dst, size_t ldd_bytes) {
+k_expand(const uint64_t *__restrict__ src, size_t lds, size_t rows, size_t kw, size_t rows_pad,
+         size_t pitch, uint8_t *__restrict__ dst) {
+    const size_t chunks = pitch / 32;
     const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
-    const size_t i = idx / kwords, w = idx % kwords;
-    if (i >= rows) return;
-    const uint64_t x = src[i * lds + w];
-    if (FP4) {
-        // 64 bits -> 32 bytes
-        uint4 out[2];
-        uint32_t *o = reinterpret_cast<uint32_t *>(out);
+    const size_t i = idx / chunks, c = idx % chunks;
+    if (i >= rows_pad) return;
+    const size_t w = FP4 ? c : c >> 1;
+    uint32_t o[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < rows && w < kw) {
+        const uint64_t x = src[i * lds + w];
+        if (FP4) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {  // 8 bits -> one dword (4 bytes)
-            const uint32_t b = (uint32_t)(x >> (8 * q)) & 0xffu;
-            uint32_t d = 0;
+            for (int q = 0; q < 8; ++q) {  // 8 bits -> one dword
+                const uint32_t b = (uint32_t)(x >> (8 * q)) & 0xffu;
+                uint32_t d = 0;
 #pragma unroll
-            for (int t = 0; t < 8; ++t) d |= ((b >> t) & 1u) << (4 * t + 1);
-            o[q] = d;
+                for (int t = 0; t < 8; ++t) d |= ((b >> t) & 1u) << (4 * t + 1);
+                o[q] = d;
+            }
+        } else {
+            const uint32_t h = (uint32_t)(x >> (32 * (c & 1)));
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {  // 4 bits -> one dword
+                const uint32_t b = (h >> (4 * q)) & 0xfu;
+                o[q] = (b & 1u) | ((b & 2u) << 7) | ((b & 4u) << 14) | ((b & 8u) << 21);
+            }
         }
-        uint4 *p = reinterpret_cast<uint4 *>(dst + i * ldd_bytes + w * 32);
-        p[0] = out[0];
-        p[1] = out[1];
+    }
+    uint4 *p = reinterpret_cast<uint4 *>(dst + i * pitch + c * 32);
+    p[0] = make_uint4(o[0], o[1], o[2], o[3]);
+    p[1] = make_uint4(o[4], o[5], o[6], o[7]);
+}
+
+// Workgroup -> C tile.  Workgroups are dealt to the 8 XCDs round-robin, so
+// the first step gives each XCD a contiguous range of tile indices; the
+// second walks those in groups of 8 tile rows, column by column, so the
+// workgroups resident on one XCD share A and B^T strips in its L2.
+__device__ __forceinline__ void tile_of(int L, int tx, int ty, int &bx, int &by) {
+    const int G = tx * ty;
+    if ((G & 7) == 0) L = (L & 7) * (G >> 3) + (L >> 3);
+    constexpr int GR = 8;
+    const int grp = L / (GR * ty);
+    const int fx = grp * GR;
+    const int gs = min(tx - fx, GR);
+    const int r = L - grp * GR * ty;
+    bx = fx + r % gs;
+    by = r / gs;
+}
+
+template <bool FP4>
+__device__ __forceinline__ void mfma_step(const uint4 &fa, const uint4 &fb,
+                                          typename std::conditional<FP4, v16f, v16i>::type &acc) {
+    if constexpr (FP4) {
+        v8i a8 = {(int)fa.x, (int)fa.y, (int)fa.z, (int)fa.w, 0, 0, 0, 0};
+        v8i b8 = {(int)fb.x, (int)fb.y, (int)fb.z, (int)fb.w, 0, 0, 0, 0};
+        acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a8, b8, acc, 4, 4, 0, 127, 0, 127);
     } else {
-        // 64 bits -> 64 bytes
-        uint4 out[4];
-        uint32_t *o = reinterpret_cast<uint32_t *>(out);
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const uint32_t b = (uint32_t)(x >> (4 * q)) & 0xfu;
-            o[q] = (b & 1u) | ((b & 2u) << 7) | ((b & 4u) << 14) | ((b & 8u) << 21);
-        }
-        uint4 *p = reinterpret_cast<uint4 *>(dst + i * ldd_bytes + w * 64);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) p[q] = out[q];
+        v4i a4 = {(int)fa.x, (int)fa.y, (int)fa.z, (int)fa.w};
+        v4i b4 = {(int)fb.x, (int)fb.y, (int)fb.z, (int)fb.w};
+        acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a4, b4, acc, 0, 0, 0);
     }
 }
 
-// C tile (2*WT) x (2*WT) per workgroup; 4 waves in a 2 x 2 grid, each WT x WT
-// (WT/32 x WT/32 MFMA 32x32 tiles, accumulators in AGPRs).  k advances 64
-// bytes of expanded data per LDS stage (two MFMA k-blocks of 32 bytes),
-// double buffered.  LDS is stored fragment-major ([unit][row]) so each
-// lane's 16-byte fragment read is contiguous across the wave.  WT = 128 gives
-// 256 x 256 tiles: 32 KiB of operands per stage for 128 MFMAs.
-template <bool FP4, int WT>
-__global__ void __launch_bounds__(256)
-k_mfma_gf2(const uint8_t *__restrict__ A, size_t lda, const uint8_t *__restrict__ B,
-           size_t ldb, uint64_t *__restrict__ C, size_t ldc, int m, int n, int kbytes) {
-    constexpr int TR = WT / 32;          // MFMA tiles per wave, each direction
-    constexpr int ROWS = 2 * WT;         // workgroup tile rows (= columns)
-    constexpr int LPT = ROWS * MF_UNITS / 256;  // 16-byte chunks per thread per operand
-    __shared__ uint4 sA[2][MF_UNITS][ROWS];
-    __shared__ uint4 sB[2][MF_UNITS][ROWS];
+// C tile epilogue.  32x32 accumulator layout: register r of lane l holds row
+// 8*(r/4) + 4*(l/32) + r%4 and column l%32 of its tile.  Column tiles
+// (2c, 2c+1) of a wave form one 64-bit word per row; bits past column n of
+// the last word are cleared.
+template <int TR, typename Acc>
+__device__ __forceinline__ void store_tile(Acc (&acc)[TR][TR], int row0, int col0,
+                                           uint64_t *__restrict__ C, size_t ldc, int m, int n) {
+    const int lane = threadIdx.x & 63;
+    const int nw = (n + 63) >> 6;
+    const uint64_t lastmask = (n & 63) ? ((1ull << (n & 63)) - 1ull) : ~0ull;
+#pragma unroll
+    for (int tr = 0; tr < TR; ++tr)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int rbase = row0 + tr * 32 + 8 * (r >> 2) + (r & 3);
+#pragma unroll
+            for (int tc = 0; tc < TR; tc += 2) {
+                const uint64_t b0 = __ballot((int)acc[tr][tc][r] & 1);
+                const uint64_t b1 = __ballot((int)acc[tr][tc + 1][r] & 1);
+                const int w = (col0 + tc * 32) >> 6;
+                // lane 0 writes the row of lanes 0-31, lane 32 that of lanes 32-63
+                if ((lane & 31) == 0 && w < nw) {
+                    const int sh = lane >> 5;
+                    const int row = rbase + 4 * sh;
+                    const uint64_t lo = sh ? (b0 >> 32) : (b0 & 0xffffffffull);
+                    const uint64_t hi = sh ? (b1 >> 32) : (b1 & 0xffffffffull);
+                    uint64_t word = lo | (hi << 32);
+                    if (w == nw - 1) word &= lastmask;
+                    if (row < m) C[(size_t)row * ldc + w] = word;
+                }
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// k_mfma_small: 128 x 128 tiles for grids too small to fill the chip with
+// 256 x 256 tiles.  4 waves in a 2 x 2 grid, each 64 x 64 (2 x 2 MFMA tiles);
+// two workgroups per CU.  Stage = 128 bytes of k, staged through registers:
+// thread t loads 16-byte unit t%8 of rows t/8 + 32q, so eight lanes read one
+// 128-byte row segment.  LDS is fragment-major with the row index
+// XOR-swizzled by f(u) = u ^ ((u&1)<<3) (< 16, so every 32-row block is
+// permuted within itself and fragment reads stay contiguous, while the eight
+// units of a row land in distinct banks).
+template <bool FP4>
+__global__ void __launch_bounds__(256, 2)
+k_mfma_small(const uint8_t *__restrict__ A, const uint8_t *__restrict__ B, size_t pitch,
+             uint64_t *__restrict__ C, size_t ldc, int m, int n, int tx, int ty) {
+    constexpr int TR = 2, WT = 64, ROWS = 128, U = 8;
+    __shared__ uint4 sA[2][U * ROWS];
+    __shared__ uint4 sB[2][U * ROWS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
-    // row tiles walk fastest so resident workgroups share B tiles in L2
-    const int i0 = blockIdx.x * ROWS, j0 = blockIdx.y * ROWS;
-
-    // loader: thread t takes row t % ROWS of units t / ROWS + q * (256 / ROWS);
-    // consecutive threads read consecutive rows (each 16 bytes of one unit).
-    // Named registers (no array) keep the staged chunks out of scratch.
-    const int lrow = tid % ROWS, lu0 = tid / ROWS;
-    constexpr int USTEP = 256 / ROWS;
-    const bool a_ok = i0 + lrow < m, b_ok = j0 + lrow < n;
-    const uint8_t *a_src = A + (size_t)(a_ok ? i0 + lrow : 0) * lda;
-    const uint8_t *b_src = B + (size_t)(b_ok ? j0 + lrow : 0) * ldb;
-    uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
-    const uint4 zero4 = make_uint4(0, 0, 0, 0);
-#define MF_LD1(q, RA, RB)                                                                       \
-    do {                                                                                        \
-        const int off = kb0 + (lu0 + (q) * USTEP) * 16;                                         \
-        const bool in = off < kbytes;                                                           \
-        RA = (a_ok && in) ? *reinterpret_cast<const uint4 *>(a_src + off) : zero4;              \
-        RB = (b_ok && in) ? *reinterpret_cast<const uint4 *>(b_src + off) : zero4;              \
-    } while (0)
-    auto gload = [&](int kb0) __attribute__((always_inline)) {
-        MF_LD1(0, ra0, rb0);
-        if constexpr (LPT > 1) MF_LD1(1, ra1, rb1);
-        if constexpr (LPT > 2) MF_LD1(2, ra2, rb2);
-        if constexpr (LPT > 3) MF_LD1(3, ra3, rb3);
+    int bx, by;
+    tile_of(blockIdx.x, tx, ty, bx, by);
+    const int i0 = bx * ROWS, j0 = by * ROWS;
+    const int lu = tid & 7, lr = tid >> 3;
+    const int fsw = lu ^ ((lu & 1) << 3);
+    const uint8_t *pa = A + (size_t)(i0 + lr) * pitch + lu * 16;
+    const uint8_t *pb = B + (size_t)(j0 + lr) * pitch + lu * 16;
+    const size_t rs = 32 * pitch;
+    uint4 xa0, xa1, xa2, xa3, xb0, xb1, xb2, xb3;
+    auto load = [&](int st) __attribute__((always_inline)) {
+        const size_t ko = (size_t)st * SMALL_STAGE;
+        xa0 = *reinterpret_cast<const uint4 *>(pa + ko);
+        xb0 = *reinterpret_cast<const uint4 *>(pb + ko);
+        xa1 = *reinterpret_cast<const uint4 *>(pa + rs + ko);
+        xb1 = *reinterpret_cast<const uint4 *>(pb + rs + ko);
+        xa2 = *reinterpret_cast<const uint4 *>(pa + 2 * rs + ko);
+        xb2 = *reinterpret_cast<const uint4 *>(pb + 2 * rs + ko);
+        xa3 = *reinterpret_cast<const uint4 *>(pa + 3 * rs + ko);
+        xb3 = *reinterpret_cast<const uint4 *>(pb + 3 * rs + ko);
     };
-#undef MF_LD1
-    auto sstore = [&](int buf) __attribute__((always_inline)) {
-        sA[buf][lu0][lrow] = ra0;
-        sB[buf][lu0][lrow] = rb0;
-        if constexpr (LPT > 1) {
-            sA[buf][lu0 + USTEP][lrow] = ra1;
-            sB[buf][lu0 + USTEP][lrow] = rb1;
-        }
-        if constexpr (LPT > 2) {
-            sA[buf][lu0 + 2 * USTEP][lrow] = ra2;
-            sB[buf][lu0 + 2 * USTEP][lrow] = rb2;
-        }
-        if constexpr (LPT > 3) {
-            sA[buf][lu0 + 3 * USTEP][lrow] = ra3;
-            sB[buf][lu0 + 3 * USTEP][lrow] = rb3;
-        }
+    auto store = [&](int buf) __attribute__((always_inline)) {
+        const int s0 = lu * ROWS + (lr ^ fsw);
+        sA[buf][s0] = xa0;
+        sB[buf][s0] = xb0;
+        sA[buf][s0 + 32] = xa1;
+        sB[buf][s0 + 32] = xb1;
+        sA[buf][s0 + 64] = xa2;
+        sB[buf][s0 + 64] = xb2;
+        sA[buf][s0 + 96] = xa3;
+        sB[buf][s0 + 96] = xb3;
     };
-
     using Acc = typename std::conditional<FP4, v16f, v16i>::type;
     Acc acc[TR][TR];
 #pragma unroll
@@ -135,107 +195,265 @@ k_mfma_gf2(const uint8_t *__restrict__ A, size_t lda, const uint8_t *__restrict_
         for (int b = 0; b < TR; ++b)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
-
-    const int nstage = (kbytes + MF_UNITS * 16 - 1) / (MF_UNITS * 16);
-    gload(0);
-    sstore(0);
-    __syncthreads();
     const int half = lane >> 5, l32 = lane & 31;
+    auto rd = [&](int buf, int kb, uint4 *fa, uint4 *fb) __attribute__((always_inline)) {
+        const int u = 2 * kb + half;
+        const int fu = u ^ (half << 3);
+        const int ra = u * ROWS + wr * WT + (l32 ^ fu);
+        const int rb = u * ROWS + wc * WT + (l32 ^ fu);
+#pragma unroll
+        for (int t = 0; t < TR; ++t) {
+            fa[t] = sA[buf][ra + t * 32];
+            fb[t] = sB[buf][rb + t * 32];
+        }
+    };
+    auto mm = [&](const uint4 *fa, const uint4 *fb) __attribute__((always_inline)) {
+#pragma unroll
+        for (int tr = 0; tr < TR; ++tr)
+#pragma unroll
+            for (int tc = 0; tc < TR; ++tc) mfma_step<FP4>(fa[tr], fb[tc], acc[tr][tc]);
+    };
+    // fragments of k-block kb+1 are read before the MFMAs of k-block kb
+    auto compute = [&](int buf) __attribute__((always_inline)) {
+        uint4 pa0[TR], pb0[TR], pa1[TR], pb1[TR];
+        rd(buf, 0, pa0, pb0);
+        __builtin_amdgcn_sched_barrier(0);
+        rd(buf, 1, pa1, pb1);
+        __builtin_amdgcn_sched_barrier(0);
+        mm(pa0, pb0);
+        __builtin_amdgcn_sched_barrier(0);
+        rd(buf, 2, pa0, pb0);
+        __builtin_amdgcn_sched_barrier(0);
+        mm(pa1, pb1);
+        __builtin_amdgcn_sched_barrier(0);
+        rd(buf, 3, pa1, pb1);
+        __builtin_amdgcn_sched_barrier(0);
+        mm(pa0, pb0);
+        __builtin_amdgcn_sched_barrier(0);
+        mm(pa1, pb1);
+    };
+    const int nstage = (int)(pitch / SMALL_STAGE);
+    load(0);
+    store(0);
+    __syncthreads();
     for (int st = 0; st < nstage; ++st) {
         const int buf = st & 1;
-        if (st + 1 < nstage) gload((st + 1) * MF_UNITS * 16);
-#pragma unroll
-        for (int kb = 0; kb < 2; ++kb) {
-            uint4 fa[TR], fb[TR];
-#pragma unroll
-            for (int t = 0; t < TR; ++t) {
-                fa[t] = sA[buf][kb * 2 + half][wr * WT + t * 32 + l32];
-                fb[t] = sB[buf][kb * 2 + half][wc * WT + t * 32 + l32];
-            }
-#pragma unroll
-            for (int tr = 0; tr < TR; ++tr)
-#pragma unroll
-                for (int tc = 0; tc < TR; ++tc) {
-                    if constexpr (FP4) {
-                        v8i a8 = {(int)fa[tr].x, (int)fa[tr].y, (int)fa[tr].z, (int)fa[tr].w, 0, 0, 0, 0};
-                        v8i b8 = {(int)fb[tc].x, (int)fb[tc].y, (int)fb[tc].z, (int)fb[tc].w, 0, 0, 0, 0};
-                        acc[tr][tc] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
-                            a8, b8, acc[tr][tc], 4, 4, 0, 127, 0, 127);
-                    } else {
-                        v4i a4 = {(int)fa[tr].x, (int)fa[tr].y, (int)fa[tr].z, (int)fa[tr].w};
-                        v4i b4 = {(int)fb[tc].x, (int)fb[tc].y, (int)fb[tc].z, (int)fb[tc].w};
-                        acc[tr][tc] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a4, b4, acc[tr][tc], 0, 0, 0);
-                    }
-                }
-        }
-        if (st + 1 < nstage) sstore(buf ^ 1);
+        // the last prefetch re-reads a valid stage into the idle buffer, so
+        // the loop body stays one basic block and the barriers keep its order
+        load(st + 1 < nstage ? st + 1 : st);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(buf);
+        __builtin_amdgcn_sched_barrier(0);
+        store(buf ^ 1);
         __syncthreads();
     }
+    store_tile<TR>(acc, i0 + wr * WT, j0 + wc * WT, C, ldc, m, n);
+}
 
-    // 32x32 accumulator layout: register r of lane l holds row
-    // 8*(r/4) + 4*(l/32) + r%4 and column l%32 of its tile.  Column tiles
-    // (2c, 2c+1) of a wave form one 64-bit word per row.
-    const int nw = (n + 63) >> 6;
+// ---------------------------------------------------------------------------
+// k_mfma_ring: 256 x 256 tiles, 4 waves in a 2 x 2 grid, each 128 x 128
+// (4 x 4 MFMA tiles, 256 accumulator registers), one workgroup per CU.
+//
+// Operands move from HBM straight into a RING-stage LDS ring with
+// global_load_lds_dwordx4 (no staging registers), so RING-2 stages are in
+// flight while the MFMAs work on the current one.  A stage holds RING_STAGE
+// bytes of k for the 256 A rows and the 256 B^T rows; each wave issues
+// exactly 2U DMA instructions per stage, so "stage s landed" is a plain
+// vmcnt threshold.  The ring buffers are separate LDS arrays and the loop is
+// unrolled by RING, so every LDS access names its buffer statically and the
+// compiler does not make fragment reads wait for the ring's other stages.
+//
+// Stage layout: row-major, U = RING_STAGE/16 units per row; unit p of row r
+// holds global unit p ^ g(r) with g(r) = ((r>>1) ^ (r>>3)) & 3, which keeps
+// the 16 lanes of a fragment read on distinct banks (for 32 and 64 banks)
+// while four lanes still read one 64-byte row segment.
+//
+// Step s: wait until stage s+1 landed, barrier, refill the buffer of stage
+// s-1 with stage s+RING-1, then read k-block 1 of stage s and k-block 0 of
+// stage s+1 from LDS into registers while the MFMAs consume the fragments
+// read before.  Reads and DMA issues are interleaved with the first MFMAs of
+// each block (any LDS read waits for all outstanding LDS traffic once a DMA
+// is pending, so they are issued early).
+__device__ __forceinline__ int ring_swz(int r) { return ((r >> 1) ^ (r >> 3)) & 3; }
+
+template <bool FP4>
+__global__ void __launch_bounds__(256, 1)
+k_mfma_ring(const uint8_t *__restrict__ A, const uint8_t *__restrict__ B, size_t pitch,
+            uint64_t *__restrict__ C, size_t ldc, int m, int n, int tx, int ty) {
+    // The host pass only needs the launch stub; the body uses LDS-DMA
+    // builtins that exist for the device target alone.
+#if defined(__HIP_DEVICE_COMPILE__)
+    constexpr int TR = 4, WT = 128, ROWS = 256, U = RING_STAGE / 16;
+    constexpr int SS = ROWS * U;  // uint4 slots per operand per stage
+    static_assert(RING == 4 && U == 4, "ring layout");
+    __shared__ uint4 r0[2 * SS], r1[2 * SS], r2[2 * SS], r3[2 * SS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    int bx, by;
+    tile_of(blockIdx.x, tx, ty, bx, by);
+    const int i0 = bx * ROWS, j0 = by * ROWS;
+    // DMA sources: instruction i (0..U-1) of this wave fills slots
+    // (i*4 + wave)*64 + lane of a stage (row S/U, position S%U)
+    const uint8_t *da[U], *db[U];
 #pragma unroll
-    for (int tr = 0; tr < TR; ++tr)
+    for (int i = 0; i < U; ++i) {
+        const int S = (i * 4 + wave) * 64 + lane;
+        const int r = S / U, p = S % U;
+        const int u = p ^ ring_swz(r);
+        da[i] = A + (size_t)(i0 + r) * pitch + u * 16;
+        db[i] = B + (size_t)(j0 + r) * pitch + u * 16;
+    }
+    const int nstage = (int)(pitch / RING_STAGE);  // a multiple of RING
+#define MELA_LDS(x) ((__attribute__((address_space(3))) void *)(x))
+    auto dma = [&](uint4 *buf, int st) __attribute__((always_inline)) {
+        const size_t ko = (size_t)(st < nstage ? st : nstage - 1) * RING_STAGE;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int rbase = i0 + wr * WT + tr * 32 + 8 * (r >> 2) + (r & 3);
-#pragma unroll
-            for (int tc = 0; tc < TR; tc += 2) {
-                const uint64_t b0 = __ballot((int)acc[tr][tc][r] & 1);
-                const uint64_t b1 = __ballot((int)acc[tr][tc + 1][r] & 1);
-                const int w = (j0 + wc * WT + tc * 32) >> 6;
-                // lane 0 writes the row of lanes 0-31, lane 32 that of lanes 32-63
-                if ((lane & 31) == 0 && w < nw) {
-                    const int sh = lane >> 5;
-                    const int row = rbase + 4 * sh;
-                    const uint64_t lo = sh ? (b0 >> 32) : (b0 & 0xffffffffull);
-                    const uint64_t hi = sh ? (b1 >> 32) : (b1 & 0xffffffffull);
-                    if (row < m) C[(size_t)row * ldc + w] = lo | (hi << 32);
-                }
-            }
+        for (int i = 0; i < U; ++i) {
+            __builtin_amdgcn_global_load_lds(da[i] + ko, MELA_LDS(buf + (i * 4 + wave) * 64), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(db[i] + ko, MELA_LDS(buf + SS + (i * 4 + wave) * 64), 16, 0, 0);
         }
+    };
+#undef MELA_LDS
+    using Acc = typename std::conditional<FP4, v16f, v16i>::type;
+    Acc acc[TR][TR];
+#pragma unroll
+    for (int a = 0; a < TR; ++a)
+#pragma unroll
+        for (int b = 0; b < TR; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
+    const int half = lane >> 5, l32 = lane & 31;
+    auto rdA = [&](const uint4 *buf, int kb, uint4 *fa) __attribute__((always_inline)) {
+        const int u = 2 * kb + half;
+#pragma unroll
+        for (int t = 0; t < TR; ++t) {
+            const int ra = wr * WT + t * 32 + l32;
+            fa[t] = buf[ra * U + (u ^ ring_swz(ra))];
+        }
+    };
+    auto rdB = [&](const uint4 *buf, int kb, uint4 *fb) __attribute__((always_inline)) {
+        const int u = 2 * kb + half;
+#pragma unroll
+        for (int t = 0; t < TR; ++t) {
+            const int rb = wc * WT + t * 32 + l32;
+            fb[t] = buf[SS + rb * U + (u ^ ring_swz(rb))];
+        }
+    };
+    auto mmh = [&](const uint4 *fa, const uint4 *fb, int t0) __attribute__((always_inline)) {
+#pragma unroll
+        for (int tr = t0; tr < t0 + 2; ++tr)
+#pragma unroll
+            for (int tc = 0; tc < TR; ++tc) mfma_step<FP4>(fa[tr], fb[tc], acc[tr][tc]);
+    };
+    // s_waitcnt immediates (gfx9 encoding): vmcnt(V) and lgkmcnt(L), expcnt free
+    constexpr int VMC = 2 * U * (RING - 3);  // in flight once stage s+1 landed
+    constexpr int WAIT = (VMC & 0xF) | ((VMC >> 4) << 14) | (7 << 4) | (0 << 8);
+    constexpr int V0 = 2 * U * (RING - 2);   // in flight once stage 0 landed
+    constexpr int WAIT0 = (V0 & 0xF) | ((V0 >> 4) << 14) | (7 << 4) | (0xF << 8);
+    uint4 fa0[TR], fb0[TR], fa1[TR], fb1[TR];
+    // step st: fa0/fb0 hold k-block 0 of stage st
+    auto step = [&](const uint4 *cur, const uint4 *nx1, uint4 *refill, int st) __attribute__((always_inline)) {
+        __builtin_amdgcn_s_waitcnt(WAIT);
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_sched_barrier(0);
+        dma(refill, st + RING - 1);
+        rdA(cur, 1, fa1);
+        rdB(cur, 1, fb1);
+        mmh(fa0, fb0, 0);
+        mmh(fa0, fb0, 2);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // LDS read
+            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read (DMA)
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        rdA(nx1, 0, fa0);
+        rdB(nx1, 0, fb0);
+        mmh(fa1, fb1, 0);
+        mmh(fa1, fb1, 2);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 1);
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 1);
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, 8, 1);
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    dma(r0, 0);
+    dma(r1, 1);
+    dma(r2, 2);
+    __builtin_amdgcn_s_waitcnt(WAIT0);
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_sched_barrier(0);
+    rdA(r0, 0, fa0);
+    rdB(r0, 0, fb0);
+    __builtin_amdgcn_sched_barrier(0);
+    // stage j lives in buffer j % RING; DMAs past the last stage re-read it
+    // into buffers that are never computed on
+    for (int st = 0; st < nstage; st += RING) {
+        step(r0, r1, r3, st);
+        step(r1, r2, r0, st + 1);
+        step(r2, r3, r1, st + 2);
+        step(r3, r0, r2, st + 3);
+    }
+    // the trailing DMAs must land before the workgroup's LDS is released
+    __builtin_amdgcn_s_waitcnt(0);
+    store_tile<TR>(acc, i0 + wr * WT, j0 + wc * WT, C, ldc, m, n);
+#endif
 }
 
 }  // namespace
 
-// C = A * B via the matrix cores; fp4 (fmt 4) or int8 expansion.
+// C = A * B via the matrix cores; fp4 or int8 expansion.
 mela_status matmul_mfma(mela_ctx *ctx, const uint64_t *A, size_t lda, const uint64_t *B,
                         size_t ldb, uint64_t *C, size_t ldc, size_t m, size_t k, size_t n,
                         bool fp4) {
     hipStream_t s = ctx->stream;
     const size_t kw = words64(k);
-    const size_t row_bytes = kw * (fp4 ? 32 : 64);  // expanded bytes per row
+    const size_t xbytes = kw * (fp4 ? 32 : 64);  // expanded bytes per row
+    const size_t pitch = (xbytes + KPAD - 1) / KPAD * KPAD;
+    const size_t mp = (m + XROWS - 1) / XROWS * XROWS, np = (n + XROWS - 1) / XROWS * XROWS;
     void *pa = nullptr, *pb = nullptr, *pbt = nullptr;
     mela_status st;
     if ((st = scratch_get(ctx, SLOT_BT, n * kw * 8, &pbt)) != MELA_OK) return st;
-    if ((st = scratch_get(ctx, SLOT_XA, m * row_bytes, &pa)) != MELA_OK) return st;
-    if ((st = scratch_get(ctx, SLOT_XB, n * row_bytes, &pb)) != MELA_OK) return st;
+    if ((st = scratch_get(ctx, SLOT_XA, mp * pitch, &pa)) != MELA_OK) return st;
+    if ((st = scratch_get(ctx, SLOT_XB, np * pitch, &pb)) != MELA_OK) return st;
     if ((st = launch_transpose(s, B, ldb, k, n, (uint64_t *)pbt, kw)) != MELA_OK) return st;
-    const size_t ta = m * kw, tb = n * kw;
+    const size_t ca = mp * (pitch / 32), cb = np * (pitch / 32);
     if (fp4) {
-        hipLaunchKernelGGL(k_expand<true>, dim3((unsigned)((ta + 255) / 256)), dim3(256), 0, s, A,
-                           lda, m, kw, (uint8_t *)pa, row_bytes);
-        hipLaunchKernelGGL(k_expand<true>, dim3((unsigned)((tb + 255) / 256)), dim3(256), 0, s,
-                           (const uint64_t *)pbt, kw, n, kw, (uint8_t *)pb, row_bytes);
+        hipLaunchKernelGGL(k_expand<true>, dim3((unsigned)((ca + 255) / 256)), dim3(256), 0, s, A,
+                           lda, m, kw, mp, pitch, (uint8_t *)pa);
+        hipLaunchKernelGGL(k_expand<true>, dim3((unsigned)((cb + 255) / 256)), dim3(256), 0, s,
+                           (const uint64_t *)pbt, kw, n, kw, np, pitch, (uint8_t *)pb);
     } else {
-        hipLaunchKernelGGL(k_expand<false>, dim3((unsigned)((ta + 255) / 256)), dim3(256), 0, s, A,
-                           lda, m, kw, (uint8_t *)pa, row_bytes);
-        hipLaunchKernelGGL(k_expand<false>, dim3((unsigned)((tb + 255) / 256)), dim3(256), 0, s,
-                           (const uint64_t *)pbt, kw, n, kw, (uint8_t *)pb, row_bytes);
+        hipLaunchKernelGGL(k_expand<false>, dim3((unsigned)((ca + 255) / 256)), dim3(256), 0, s, A,
+                           lda, m, kw, mp, pitch, (uint8_t *)pa);
+        hipLaunchKernelGGL(k_expand<false>, dim3((unsigned)((cb + 255) / 256)), dim3(256), 0, s,
+                           (const uint64_t *)pbt, kw, n, kw, np, pitch, (uint8_t *)pb);
     }
-    // 128 x 128 tiles (256 x 256 tiles measured slower: one wave per SIMD
-    // leaves the global-load latency of each stage exposed)
-    constexpr unsigned T = 128;
-    dim3 grid((unsigned)((m + T - 1) / T), (unsigned)((n + T - 1) / T));
-    if (grid.y > 65535) return set_error(MELA_ERR_INVALID, "too many columns for one launch");
+    MELA_HIP(hipGetLastError());
     const uint8_t *xa = (const uint8_t *)pa, *xb = (const uint8_t *)pb;
-    const int kb = (int)row_bytes;
-    if (fp4)
-        hipLaunchKernelGGL((k_mfma_gf2<true, 64>), grid, dim3(256), 0, s, xa, row_bytes, xb, row_bytes, C, ldc, (int)m, (int)n, kb);
-    else
-        hipLaunchKernelGGL((k_mfma_gf2<false, 64>), grid, dim3(256), 0, s, xa, row_bytes, xb, row_bytes, C, ldc, (int)m, (int)n, kb);
+    // 256 x 256 tiles once they fill every CU, else 128 x 128 (two per CU)
+    const size_t tx2 = mp / 256, ty2 = np / 256;
+    prof_mark(ctx, true);
+    if (tx2 * ty2 >= (size_t)ctx->cu_count) {
+        if (tx2 * ty2 > (size_t)INT32_MAX) return set_error(MELA_ERR_INVALID, "too many tiles for one launch");
+        const dim3 grid((unsigned)(tx2 * ty2));
+        if (fp4)
+            hipLaunchKernelGGL(k_mfma_ring<true>, grid, dim3(256), 0, s, xa, xb, pitch, C, ldc, (int)m, (int)n, (int)tx2, (int)ty2);
+        else
+            hipLaunchKernelGGL(k_mfma_ring<false>, grid, dim3(256), 0, s, xa, xb, pitch, C, ldc, (int)m, (int)n, (int)tx2, (int)ty2);
+    } else {
+        const size_t tx1 = (m + 127) / 128, ty1 = (n + 127) / 128;
+        const dim3 grid((unsigned)(tx1 * ty1));
+        if (fp4)
+            hipLaunchKernelGGL(k_mfma_small<true>, grid, dim3(256), 0, s, xa, xb, pitch, C, ldc, (int)m, (int)n, (int)tx1, (int)ty1);
+        else
+            hipLaunchKernelGGL(k_mfma_small<false>, grid, dim3(256), 0, s, xa, xb, pitch, C, ldc, (int)m, (int)n, (int)tx1, (int)ty1);
+    }
+    prof_mark(ctx, false);
     MELA_HIP(hipGetLastError());
     return MELA_OK;
 }

@@ -106,6 +106,25 @@ def test_matmul_packed_algos(mela, ctx, orc, algo, shape):
     assert np.array_equal(got[:, : want.shape[1]], want)
 
 
+@pytest.mark.parametrize("algo", [0, 3, 4])
+@pytest.mark.parametrize("shape", [(4100, 1000, 4000), (4096, 4160, 4293), (4352, 64, 4097)])
+def test_matmul_mfma_large_tiles_exact(mela, ctx, orc, algo, shape):
+    # grids of >= 256 tiles of 256 x 256 take the LDS-ring kernel; ragged m,
+    # k and n exercise the zero padding and the last-word mask
+    m, k, n = shape
+    print(f"[matmul mfma large tiles] algo={algo} shape={shape}", flush=True)
+    Ap = orc.gen_bernoulli_packed(21, m, k)
+    Bp = orc.gen_bernoulli_packed(22, k, n)
+    dA, dB = ctx.upload(Ap, k), ctx.upload(Bp, n)
+    dC = ctx.bitmat(m, n)
+    ctx.matmul_packed(dA, dB, dC, algo)
+    got = ctx.download(dC)
+    want = orc.matmul_packed(Ap, m, k, Bp, n, "m4ri")
+    for M in (dA, dB, dC):
+        ctx.free(M)
+    assert np.array_equal(got[:, : want.shape[1]], want)
+
+
 def test_device_generator_matches_oracle(ctx, orc):
     M = ctx.bitmat(300, 1000)
     ctx.gen_bernoulli(77, M)
