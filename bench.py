@@ -30,6 +30,11 @@ sys.path.insert(0, ROOT)
 VALU_BITOP3_PEAK = 2516.0
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of the dominant product kernel, measured with two
+# rocprofv3 --pmc passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a
+# 1 GiB copy kernel) on this workload; profiles/matmul_mfma_fp4_32768.md.
+MEASURED_TRAFFIC = {("mfma_fp4", 32768): 2.902e10}
+
 
 class HipEvents:
     """hipEvent timing on a given stream (the library's own stream)."""
@@ -165,7 +170,9 @@ def bench_matmul(args, mela):
         "data": "synthetic Bernoulli(1/2) bits, splitmix64 seeds 1 (A) and 2 (B), generated on device",
         "config": {"workload": f"packed GF(2) matmul {n}x{n} * {n}x{n}, operands resident in HBM",
                    "algo": args.algo, "algo_resolved": resolved, "parallelism": "1 GPU"},
-        "roofline": roofline_for(resolved, achieved, args.traffic,
+        "roofline": roofline_for(resolved, achieved,
+                                 args.traffic if args.traffic is not None
+                                 else MEASURED_TRAFFIC.get((resolved, n)),
                                  big_tiles=((n + 255) // 256) ** 2 >= 256),
         "call_ms": round(dev_ms, 3),
         "kernel_ms": round(kern_ms, 3),
