@@ -9,18 +9,20 @@
 // the results must (and do) match.
 //
 // The device processes one 64-column panel (one packed word column g) at a
-// time, with three launches:
+// time:
 //   k_panel_select  one workgroup finds <= 64 active rows whose panel words
 //                   span all active rows' panel words (wave Gauss-Jordan with
 //                   ballot/readlane), removes them from the active list and
 //                   computes the RREF basis of the panel plus, for each basis
-//                   vector, which selected rows XOR to it;
-//   k_build_pivots  forms the new pivot rows Q (full width) from the selected
-//                   rows;
+//                   vector c, the set comb[c] of selected rows that XOR to it;
+//   k_build_pivots  forms the new pivot rows Q (words g .. wc-1): the selected
+//                   rows of a 64-word tile are staged in LDS and each pivot row
+//                   XORs the ones named by comb[c];
 //   k_update        clears the panel's pivot columns from every active row
 //                   (and, for RREF, from every earlier pivot row) with one
 //                   table XOR per 4 pivot columns (M4RI) or one XOR per pivot
-//                   row (GE), sweeping words g+1 .. wc-1 -- the HBM-bound part.
+//                   row (GE), sweeping words g+1 .. wc-1 -- the HBM-bound part;
+//   k_update_panelword (RREF) word g of the earlier pivot rows.
 #include "elim_device.h"
 
 namespace mela {
@@ -73,29 +75,39 @@ k_panel_select(const uint64_t *__restrict__ M, size_t ld, uint32_t *__restrict__
     }
 }
 
-// Q[r0 + j][w] for the j-th new pivot (ascending bit c) = XOR of the selected
-// rows in comb[c]; words below g are zero (all active rows are zero there).
-__global__ void __launch_bounds__(256)
+constexpr int BP_PIV = 8;  // pivot rows per k_build_pivots workgroup
+
+// Q[r0 + j][w], w in one 64-word tile from g, for BP_PIV pivots j = the XOR of
+// the selected rows named by comb[c] (c = j-th pivot bit).  The tile's 64
+// selected-row words per lane are loaded together and staged in LDS, so the
+// XORs read LDS instead of chaining dependent global loads.
+__global__ void __launch_bounds__(512)
 k_build_pivots(const uint64_t *__restrict__ M, size_t ld, uint64_t *__restrict__ Q, size_t ldq,
                const ElimMeta *__restrict__ meta, int g, int wc) {
+    __shared__ uint64_t S[64][64];
     const uint64_t pm = meta->pm;
-    const int j = blockIdx.y;  // pivot bit c = the j-th set bit of pm
-    if (j >= popc64(pm)) return;
+    const int rho = meta->rho;
+    const int j0 = blockIdx.y * BP_PIV;
+    if (pm == 0 || j0 >= rho) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int w = g + blockIdx.x * 64 + lane;
+    const bool wok = w < wc;
+    uint64_t v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int slot = wave + 8 * k;
+        v[k] = (wok && slot < rho) ? M[(size_t)meta->sel_row[slot] * ld + w] : 0ull;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) S[wave + 8 * k][lane] = v[k];
+    __syncthreads();
+    const int j = j0 + wave;  // one pivot per wave
+    if (j >= rho || !wok) return;
     uint64_t t = pm;
     for (int q = 0; q < j; ++q) t &= t - 1;
-    const int c = __ffsll((unsigned long long)t) - 1;
-    const uint64_t comb = meta->comb[c];
-    const int r0 = meta->r0;
-    const int w = blockIdx.x * 256 + threadIdx.x;
-    if (w >= wc) return;
     uint64_t acc = 0;
-    if (w >= g) {
-        for (uint64_t s = comb; s; s &= s - 1) {
-            const int slot = __ffsll((unsigned long long)s) - 1;
-            acc ^= M[(size_t)meta->sel_row[slot] * ld + w];
-        }
-    }
-    Q[(size_t)(r0 + j) * ldq + w] = acc;
+    for (uint64_t y = meta->comb[ffs64(t)]; y; y &= y - 1) acc ^= S[ffs64(y)][lane];
+    Q[(size_t)(meta->r0 + j) * ldq + w] = acc;
 }
 
 // Row pointer of unified row index q: active rows first, then (RREF) the
@@ -107,29 +119,42 @@ __device__ __forceinline__ uint64_t *row_ptr(uint64_t *M, size_t ld, const uint3
 
 constexpr int UP_TILE = 64;   // words per workgroup (one per lane)
 constexpr int UP_WAVES = 8;   // 512 threads
-constexpr int UP_ROWS = 4;    // rows in flight per wave
+constexpr int UP_ROWS_M4RI = 16;  // rows in flight per wave (M4RI sweep)
+constexpr int UP_ROWS_GE = 4;     // rows in flight per wave (GE sweep: fewer registers)
 
-// Loads UP_ROWS rows (their panel words x and this lane's word w) before any
-// use, so each wave keeps 4 independent row reads in flight.
+// R rows of one wave: row pointers, panel words x and this lane's word w.
+// The pointers are resolved first (one active-list load each); the word loads
+// are issued together afterwards, so all of them are in flight at once.
+template <int R>
 struct RowBatch {
-    uint64_t *rp[UP_ROWS];
-    uint64_t x[UP_ROWS];
-    uint64_t a[UP_ROWS];
+    uint64_t *rp[R];
+    uint64_t x[R];
+    uint64_t a[R];
 };
 
-__device__ __forceinline__ void load_rows(RowBatch &b, uint64_t *M, size_t ld,
-                                          const uint32_t *act, uint64_t *Q, size_t ldq,
-                                          int n_act, int total, int q0, int g, int w, bool wok,
-                                          uint64_t pm) {
+template <int R>
+__device__ __forceinline__ void batch_ptrs(RowBatch<R> &b, uint64_t *M, size_t ld,
+                                           const uint32_t *act, uint64_t *Q, size_t ldq,
+                                           int n_act, int total, int q0) {
 #pragma unroll
-    for (int j = 0; j < UP_ROWS; ++j) {
+    for (int j = 0; j < R; ++j) {
         const int q = q0 + j;
         b.rp[j] = q < total ? row_ptr(M, ld, act, Q, ldq, n_act, q) : nullptr;
     }
+}
+
+template <int R>
+__device__ __forceinline__ void batch_words(RowBatch<R> &b, int g, int w, bool wok, uint64_t pm) {
 #pragma unroll
-    for (int j = 0; j < UP_ROWS; ++j) b.x[j] = b.rp[j] ? (b.rp[j][g] & pm) : 0ull;
+    for (int j = 0; j < R; ++j) b.x[j] = b.rp[j] ? (b.rp[j][g] & pm) : 0ull;
 #pragma unroll
-    for (int j = 0; j < UP_ROWS; ++j) b.a[j] = (b.rp[j] && wok) ? b.rp[j][w] : 0ull;
+    for (int j = 0; j < R; ++j) b.a[j] = (b.rp[j] && wok) ? b.rp[j][w] : 0ull;
+}
+
+// LDS-only workgroup barrier: row loads already in flight stay in flight.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_s_waitcnt((0xF) | (3 << 14) | (7 << 4) | (0 << 8));  // lgkmcnt(0)
+    __builtin_amdgcn_s_barrier();
 }
 
 // M4RI form: 16 nibble tables of the panel's pivot rows (slot c = pivot bit c,
@@ -148,39 +173,49 @@ k_update_m4ri(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int w = g + 1 + blockIdx.x * UP_TILE + lane;
     const bool wok = w < wc;
-    if ((int)blockIdx.y * UP_WAVES * UP_ROWS >= total) return;
+    const int stride = gridDim.y * UP_WAVES * UP_ROWS_M4RI;
+    int q0 = (blockIdx.y * UP_WAVES + wave) * UP_ROWS_M4RI;
+    if ((int)blockIdx.y * UP_WAVES * UP_ROWS_M4RI >= total) return;
+    // table inputs first, then the first row batch, whose word loads stay in
+    // flight while the tables are built
+    uint64_t pq[16 / UP_WAVES][4];
+#pragma unroll
+    for (int tt = 0; tt < 16 / UP_WAVES; ++tt)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int c = 4 * (wave * (16 / UP_WAVES) + tt) + b;
+            pq[tt][b] = (wok && ((pm >> c) & 1ull))
+                            ? Q[(size_t)(r0 + popc64(pm & ((1ull << c) - 1ull))) * ldq + w]
+                            : 0ull;
+        }
+    RowBatch<UP_ROWS_M4RI> rb;
+    batch_ptrs(rb, M, ld, act, Q, ldq, n_act, total, q0);
+    batch_words(rb, g, w, wok, pm);
 #pragma unroll
     for (int tt = 0; tt < 16 / UP_WAVES; ++tt) {
         const int t = wave * (16 / UP_WAVES) + tt;
-        uint64_t q[4];
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int c = 4 * t + b;
-            q[b] = 0;
-            if (wok && ((pm >> c) & 1ull))
-                q[b] = Q[(size_t)(r0 + popc64(pm & ((1ull << c) - 1ull))) * ldq + w];
-        }
         uint64_t cur = 0;
         T[t][0][lane] = 0;
 #pragma unroll
         for (int v = 1; v < 16; ++v) {
-            cur ^= q[__builtin_ctz(v)];
+            cur ^= pq[tt][__builtin_ctz(v)];
             T[t][v ^ (v >> 1)][lane] = cur;
         }
     }
-    __syncthreads();
-    const int stride = gridDim.y * UP_WAVES * UP_ROWS;
-    for (int q0 = (blockIdx.y * UP_WAVES + wave) * UP_ROWS; q0 < total; q0 += stride) {
-        RowBatch b;
-        load_rows(b, M, ld, act, Q, ldq, n_act, total, q0, g, w, wok, pm);
+    lds_barrier();
+    for (; q0 < total; q0 += stride) {
 #pragma unroll
-        for (int j = 0; j < UP_ROWS; ++j) {
-            const uint64_t x = b.x[j];
+        for (int j = 0; j < UP_ROWS_M4RI; ++j) {
+            const uint64_t x = rb.x[j];
             if (x == 0 || !wok) continue;
-            uint64_t a = b.a[j];
+            uint64_t a = rb.a[j];
 #pragma unroll
             for (int t = 0; t < 16; ++t) a ^= T[t][(x >> (4 * t)) & 15u][lane];
-            b.rp[j][w] = a;
+            rb.rp[j][w] = a;
+        }
+        if (q0 + stride < total) {
+            batch_ptrs(rb, M, ld, act, Q, ldq, n_act, total, q0 + stride);
+            batch_words(rb, g, w, wok, pm);
         }
     }
 }
@@ -199,48 +234,62 @@ k_update_ge(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ ac
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int w = g + 1 + blockIdx.x * UP_TILE + lane;
     const bool wok = w < wc;
-    if ((int)blockIdx.y * UP_WAVES * UP_ROWS >= total) return;
-    for (int c = wave; c < 64; c += UP_WAVES) {
-        uint64_t v = 0;
-        if (wok && ((pm >> c) & 1ull))
-            v = Q[(size_t)(r0 + popc64(pm & ((1ull << c) - 1ull))) * ldq + w];
-        P[c][lane] = v;
-    }
-    __syncthreads();
-    const int stride = gridDim.y * UP_WAVES * UP_ROWS;
-    for (int q0 = (blockIdx.y * UP_WAVES + wave) * UP_ROWS; q0 < total; q0 += stride) {
-        RowBatch b;
-        load_rows(b, M, ld, act, Q, ldq, n_act, total, q0, g, w, wok, pm);
+    const int stride = gridDim.y * UP_WAVES * UP_ROWS_GE;
+    int q0 = (blockIdx.y * UP_WAVES + wave) * UP_ROWS_GE;
+    if ((int)blockIdx.y * UP_WAVES * UP_ROWS_GE >= total) return;
+    uint64_t pv[64 / UP_WAVES];
 #pragma unroll
-        for (int j = 0; j < UP_ROWS; ++j) {
-            uint64_t x = b.x[j];
+    for (int k = 0; k < 64 / UP_WAVES; ++k) {
+        const int c = wave + k * UP_WAVES;
+        pv[k] = (wok && ((pm >> c) & 1ull))
+                    ? Q[(size_t)(r0 + popc64(pm & ((1ull << c) - 1ull))) * ldq + w]
+                    : 0ull;
+    }
+    RowBatch<UP_ROWS_GE> rb;
+    batch_ptrs(rb, M, ld, act, Q, ldq, n_act, total, q0);
+    batch_words(rb, g, w, wok, pm);
+#pragma unroll
+    for (int k = 0; k < 64 / UP_WAVES; ++k) P[wave + k * UP_WAVES][lane] = pv[k];
+    lds_barrier();
+    for (; q0 < total; q0 += stride) {
+#pragma unroll
+        for (int j = 0; j < UP_ROWS_GE; ++j) {
+            uint64_t x = rb.x[j];
             if (x == 0 || !wok) continue;
-            uint64_t a = b.a[j];
+            uint64_t a = rb.a[j];
             while (x) {
                 a ^= P[ffs64(x)][lane];
                 x &= x - 1;
             }
-            b.rp[j][w] = a;
+            rb.rp[j][w] = a;
+        }
+        if (q0 + stride < total) {
+            batch_ptrs(rb, M, ld, act, Q, ldq, n_act, total, q0 + stride);
+            batch_words(rb, g, w, wok, pm);
         }
     }
 }
 
-// RREF only: word g of the earlier pivot rows (active rows' word g is dead).
+// RREF only: word g of the earlier pivot rows (active rows' word g is dead),
+// with the new pivot rows' word g staged in LDS by pivot bit.
 __global__ void __launch_bounds__(256)
 k_update_panelword(uint64_t *__restrict__ Q, size_t ldq, const ElimMeta *__restrict__ meta,
                    int g) {
+    __shared__ uint64_t PW[64];
     const uint64_t pm = meta->pm;
     const int r0 = meta->r0;
-    const int q = blockIdx.x * 256 + threadIdx.x;
-    if (pm == 0 || q >= r0) return;
-    uint64_t x = Q[(size_t)q * ldq + g];
-    uint64_t h = x & pm;
+    if (pm == 0) return;
+    const int tid = threadIdx.x;
+    if (tid < 64)
+        PW[tid] = ((pm >> tid) & 1ull)
+                      ? Q[(size_t)(r0 + popc64(pm & ((1ull << tid) - 1ull))) * ldq + g]
+                      : 0ull;
+    __syncthreads();
+    const int q = blockIdx.x * 256 + tid;
+    if (q >= r0) return;
+    const uint64_t x = Q[(size_t)q * ldq + g];
     uint64_t a = x;
-    while (h) {
-        const int c = __ffsll((unsigned long long)h) - 1;
-        h &= h - 1;
-        a ^= Q[(size_t)(r0 + popc64(pm & ((1ull << c) - 1ull))) * ldq + g];
-    }
+    for (uint64_t h = x & pm; h; h &= h - 1) a ^= PW[ffs64(h)];
     Q[(size_t)q * ldq + g] = a;
 }
 
@@ -355,20 +404,25 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
     auto enqueue_panels = [&]() -> mela_status {
         hipLaunchKernelGGL(k_elim_init, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, meta,
                            act, (int)m);
+        // pivot rows are zero left of their panel (only their words >= g are
+        // written); the RREF output and the null extraction read them whole
+        if (rref) {
+            mela_status zs = launch_fill_zero(s, Q, rmax * ld * sizeof(uint64_t));
+            if (zs != MELA_OK) return zs;
+        }
         for (size_t g = 0; g < wc; ++g) {
             hipLaunchKernelGGL(k_panel_select, dim3(1), dim3(PS_THREADS), 0, s, Mw, ld, act, meta,
                                pivcols, (int)g, (int)rmax);
-            dim3 gb((unsigned)((wc + 255) / 256), 64);
-            hipLaunchKernelGGL(k_build_pivots, gb, dim3(256), 0, s, Mw, ld, Q, ld, meta, (int)g,
-                               (int)wc);
+            hipLaunchKernelGGL(k_build_pivots, dim3((unsigned)((wc - g + 63) / 64), 64 / BP_PIV),
+                               dim3(512), 0, s, Mw, ld, Q, ld, meta, (int)g, (int)wc);
             const size_t rem = wc - g - 1;
             if (rem > 0) {
                 const unsigned tx = (unsigned)((rem + UP_TILE - 1) / UP_TILE);
                 // rows are strided over gridDim.y; size it to fill the chip ~2x
                 const size_t rows_hint = m + (rref ? rmax : 0);
                 unsigned ty = (unsigned)((2 * cu + tx - 1) / tx);
-                const unsigned ty_max =
-                    (unsigned)((rows_hint + UP_WAVES * UP_ROWS - 1) / (UP_WAVES * UP_ROWS));
+                const unsigned per = UP_WAVES * (variant == MELA_ELIM_GE ? UP_ROWS_GE : UP_ROWS_M4RI);
+                const unsigned ty_max = (unsigned)((rows_hint + per - 1) / per);
                 if (ty > ty_max) ty = ty_max;
                 if (ty < 1) ty = 1;
                 if (variant == MELA_ELIM_GE)
