@@ -115,6 +115,16 @@ mela_status mela_null_m4ri(mela_ctx *ctx, const void *H, int h_class,
                            size_t m, size_t n, mela_alloc_logical_fn alloc,
                            void *user, size_t *nullity);
 
+/* Reduced row echelon form with pivot columns, the semantics of the MATLAB
+ * comparator the reference ships, gf2null/gf2rref.m:49-73 (no MEX
+ * counterpart): R (m x n logical, caller-allocated m*n bytes, column-major)
+ * receives the RREF with its rank nonzero rows on top; jb (NULL or min(m,n)
+ * doubles) receives the 1-based pivot columns in ascending order; *rank =
+ * length(jb).  Messages: sparse (as matmul), "Input must be a logical or
+ * double matrix.". */
+mela_status mela_rref_gf2(mela_ctx *ctx, const void *A, int a_class, size_t m, size_t n,
+                          uint8_t *R, double *jb, size_t *rank);
+
 /* ------------------------------------------- device-resident packed forms
  *
  * A packed matrix lives in device memory, row-major: row i is ld 64-bit
@@ -173,6 +183,18 @@ mela_status mela_rref_packed(mela_ctx *ctx, const mela_bitmat *A, int variant,
  * (free with mela_bitmat_free). */
 mela_status mela_null_packed(mela_ctx *ctx, const mela_bitmat *A, int variant,
                              mela_bitmat *Z, size_t *nullity);
+
+/* Sparse input (no reference counterpart: the reference rejects sparse
+ * arrays, matmul_gf2.c:78-79, null_gf2.c:151-153).  Packs an m x n matrix in
+ * MATLAB's compressed sparse column form into A (allocated by the caller,
+ * rows = m, cols = n; its contents are replaced): column j's stored entries
+ * are rows ir[jc[j] .. jc[j+1]-1] (0-based, uint64 like mwIndex).  values:
+ * NULL (every stored entry is 1) or nnz logical bytes / doubles
+ * (values_class MELA_LOGICAL / MELA_DOUBLE; the dense parity rule).  jc must
+ * start at 0 and not decrease; row indices >= m are an error. */
+mela_status mela_pack_csc(mela_ctx *ctx, const uint64_t *jc, const uint64_t *ir,
+                          const void *values, int values_class, size_t m, size_t n,
+                          mela_bitmat *A);
 
 /* ------------------------------------------------ batched rank / null space
  *

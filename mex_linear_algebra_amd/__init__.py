@@ -22,7 +22,7 @@ import numpy as np
 __all__ = [
     "MelaError", "Context", "BitMatrix", "lib_path", "load", "default_context",
     "mela_matmul_gf2", "mela_matmul_m4ri", "mela_rank_gf2", "mela_rank_m4ri",
-    "mela_null_gf2", "mela_null_m4ri",
+    "mela_null_gf2", "mela_null_m4ri", "mela_rref_gf2",
     "ALGO_AUTO", "ALGO_ANDPOP", "ALGO_M4RM", "ALGO_MFMA_I8", "ALGO_MFMA_FP4", "ELIM_GE", "ELIM_M4RI",
     "Batch", "BATCH_NULL", "BATCH_GE", "BATCH_GRAPH",
     "Server", "Request", "Result", "OP_RANK", "OP_NULL", "OP_MATMUL",
@@ -119,7 +119,13 @@ def load():
         for f in (L.mela_null_gf2, L.mela_null_m4ri):
             f.argtypes = [ctxp, vp, C.c_int, sz, sz, ALLOC_FN, vp, C.POINTER(sz)]
             f.restype = st
+        L.mela_rref_gf2.argtypes = [ctxp, vp, C.c_int, sz, sz, vp, C.POINTER(C.c_double),
+                                    C.POINTER(sz)]
+        L.mela_rref_gf2.restype = st
         bm = C.POINTER(BitMatrix)
+        L.mela_pack_csc.argtypes = [ctxp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), vp,
+                                    C.c_int, sz, sz, bm]
+        L.mela_pack_csc.restype = st
         L.mela_dev_alloc.argtypes = [ctxp, sz, C.POINTER(vp)]
         L.mela_dev_alloc.restype = st
         L.mela_dev_free.argtypes = [ctxp, vp]
@@ -275,6 +281,43 @@ class Context:
         _check(self.lib.mela_rref_packed(self.handle, C.byref(A), variant, C.byref(R), piv,
                                          C.byref(r)))
         return R, [piv[i] for i in range(r.value)]
+
+    def pack_csc(self, indptr, indices, data, m: int, n: int) -> BitMatrix:
+        """Packed m x n device matrix from compressed sparse column arrays
+        (MATLAB's jc / ir / pr, or scipy's indptr / indices / data).  data:
+        None (every stored entry is 1), bool (nonzero) or float64 (the dense
+        parity rule)."""
+        jc = np.ascontiguousarray(indptr, dtype=np.uint64)
+        ir = np.ascontiguousarray(indices, dtype=np.uint64)
+        if jc.size != n + 1:
+            raise MelaError("indptr must have n + 1 entries")
+        vals, vcls = None, MELA_LOGICAL
+        if data is not None:
+            d = np.asarray(data)
+            if d.dtype == np.bool_:
+                vals, vcls = np.ascontiguousarray(d).view(np.uint8), MELA_LOGICAL
+            elif d.dtype == np.float64:
+                vals, vcls = np.ascontiguousarray(d), MELA_DOUBLE
+            else:
+                raise MelaError("Inputs must be logical or double matrices.")
+        A = self.bitmat(m, n)
+        try:
+            _check(self.lib.mela_pack_csc(
+                self.handle, jc.ctypes.data_as(C.POINTER(C.c_uint64)),
+                ir.ctypes.data_as(C.POINTER(C.c_uint64)) if ir.size else None,
+                None if vals is None or vals.size == 0 else vals.ctypes.data, vcls, m, n,
+                C.byref(A)))
+        except MelaError:
+            self.free(A)
+            raise
+        return A
+
+    def upload_sparse(self, S) -> BitMatrix:
+        """Packed device matrix from a scipy.sparse matrix (any format)."""
+        S = S.tocsc()
+        S.sort_indices()
+        data = S.data if S.dtype in (np.float64, np.bool_) else S.data.astype(np.float64)
+        return self.pack_csc(S.indptr, S.indices, data, S.shape[0], S.shape[1])
 
     def null_packed(self, A: BitMatrix, variant: int = ELIM_M4RI) -> BitMatrix:
         Z = BitMatrix()
@@ -472,6 +515,20 @@ def _null(fn_name, H, ctx):
     _check(getattr(ctx.lib, fn_name)(ctx.handle, _ptr(h), hc, m, n, cb, None, C.byref(f)))
     Z = holder.get("Z", np.zeros((n, f.value), dtype=np.uint8, order="F"))
     return Z.view(np.bool_)
+
+
+def mela_rref_gf2(A, ctx: Context | None = None):
+    """[R, jb] = gf2rref(A) (gf2null/gf2rref.m:49-73): R is the m x n logical
+    reduced row echelon form, jb the 1-based pivot columns (a double row
+    vector, empty when the rank is 0)."""
+    ctx = ctx or default_context()
+    a, ac, m, n = _classify(A)
+    R = np.zeros((m, n), dtype=np.uint8, order="F")
+    jb = np.zeros(max(min(m, n), 1), dtype=np.float64)
+    r = C.c_size_t()
+    _check(ctx.lib.mela_rref_gf2(ctx.handle, _ptr(a), ac, m, n, _ptr(R),
+                                 jb.ctypes.data_as(C.POINTER(C.c_double)), C.byref(r)))
+    return R.view(np.bool_), jb[: r.value].reshape((1, -1))
 
 
 def mela_null_gf2(H, *more, ctx: Context | None = None):

@@ -264,6 +264,94 @@ mela_status mela_null_m4ri(mela_ctx *ctx, const void *H, int h_class, size_t m, 
     return null_entry(ctx, H, h_class, m, n, alloc, user, nullity, MELA_ELIM_M4RI);
 }
 
+mela_status mela_rref_gf2(mela_ctx *ctx, const void *A, int a_class, size_t m, size_t n,
+                          uint8_t *R, double *jb, size_t *rank) {
+    // gf2rref.m:49-73 semantics; messages follow the rank entry points
+    if (a_class & MELA_SPARSE) return set_error(MELA_ERR_SPARSE, kSparseMsg);
+    if (!is_supported(a_class))
+        return set_error(MELA_ERR_TYPE, "Input must be a logical or double matrix.");
+    if (!rank) return set_error(MELA_ERR_INVALID, "null argument");
+    *rank = 0;
+    size_t mn;
+    if (mul_overflows(m, n, &mn)) return set_error(MELA_ERR_INVALID, "matrix too large");
+    if (mn && (!A || !R)) return set_error(MELA_ERR_INVALID, "null data pointer");
+    if (mn == 0) return MELA_OK;
+    mela_status st = resolve(&ctx);
+    if (st != MELA_OK) return st;
+    Guard g(ctx);
+    uint64_t *Ap;
+    size_t lda;
+    if ((st = upload_pack(ctx, A, a_class, m, n, SLOT_IN_A, SLOT_A, &Ap, &lda)) != MELA_OK) return st;
+    ElimResult res;
+    uint64_t *Q = nullptr;
+    size_t ldq = 1;
+    if ((st = eliminate(ctx, Ap, lda, m, n, MELA_ELIM_M4RI, true, true, &res, &Q, &ldq)) != MELA_OK)
+        return st;
+    // R: the rank pivot rows on top, zero rows below, unpacked column-major
+    const size_t wc = words64(n);
+    void *Rp = nullptr, *out = nullptr;
+    if ((st = scratch_get(ctx, SLOT_C, m * wc * 8, &Rp)) != MELA_OK) return st;
+    if ((st = scratch_get(ctx, SLOT_OUT, mn, &out)) != MELA_OK) return st;
+    if ((st = launch_fill_zero(ctx->stream, Rp, m * wc * 8)) != MELA_OK) return st;
+    if (res.rank)
+        MELA_HIP(hipMemcpy2DAsync(Rp, wc * 8, Q, ldq * 8, wc * 8, res.rank, hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+    if ((st = launch_unpack(ctx->stream, (uint64_t *)Rp, wc, m, n, (uint8_t *)out)) != MELA_OK)
+        return st;
+    MELA_HIP(hipMemcpyAsync(R, out, mn, hipMemcpyDeviceToHost, ctx->stream));
+    MELA_HIP(hipStreamSynchronize(ctx->stream));
+    if (jb)
+        for (size_t i = 0; i < res.rank; ++i) jb[i] = (double)res.pivots[i] + 1.0;
+    *rank = res.rank;
+    return MELA_OK;
+}
+
+mela_status mela_pack_csc(mela_ctx *ctx, const uint64_t *jc, const uint64_t *ir,
+                          const void *values, int values_class, size_t m, size_t n,
+                          mela_bitmat *A) {
+    if (!ctx) return set_error(MELA_ERR_INVALID, "null context");
+    mela_status st = check_bitmat(A, "A");
+    if (st != MELA_OK) return st;
+    if (A->rows != m || A->cols != n) return set_error(MELA_ERR_DIM, "A must be m x n");
+    if (!jc) return set_error(MELA_ERR_INVALID, "null column pointer array");
+    if (values && !is_supported(values_class))
+        return set_error(MELA_ERR_TYPE, "Inputs must be logical or double matrices.");
+    // the column pointers must describe nnz entries in order (jc[0] = 0,
+    // non-decreasing); every entry then belongs to exactly one column
+    if (jc[0] != 0) return set_error(MELA_ERR_INVALID, "jc[0] must be 0");
+    for (size_t j = 0; j < n; ++j)
+        if (jc[j + 1] < jc[j]) return set_error(MELA_ERR_INVALID, "jc must be non-decreasing");
+    const size_t nnz = jc[n];
+    if (nnz && !ir) return set_error(MELA_ERR_INVALID, "null row index array");
+    Guard g(ctx);
+    if (m && A->ld && (st = launch_fill_zero(ctx->stream, A->data, m * A->ld * 8)) != MELA_OK)
+        return st;
+    if (nnz == 0) {
+        MELA_HIP(hipStreamSynchronize(ctx->stream));
+        return MELA_OK;
+    }
+    const int vcls = values ? (values_class & 0xff) : -1;
+    const size_t vbytes = values ? nnz * elem_size(values_class) : 0;
+    const size_t off_ir = (n + 1) * 8, off_v = off_ir + nnz * 8;
+    const size_t off_err = (off_v + vbytes + 7) / 8 * 8;
+    void *buf = nullptr;
+    if ((st = scratch_get(ctx, SLOT_IN_A, off_err + 8, &buf)) != MELA_OK) return st;
+    char *b = (char *)buf;
+    MELA_HIP(hipMemcpyAsync(b, jc, (n + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
+    MELA_HIP(hipMemcpyAsync(b + off_ir, ir, nnz * 8, hipMemcpyHostToDevice, ctx->stream));
+    if (vbytes) MELA_HIP(hipMemcpyAsync(b + off_v, values, vbytes, hipMemcpyHostToDevice, ctx->stream));
+    if ((st = launch_fill_zero(ctx->stream, b + off_err, 8)) != MELA_OK) return st;
+    if ((st = launch_pack_csc(ctx->stream, (const uint64_t *)b, (const uint64_t *)(b + off_ir),
+                              vbytes ? b + off_v : nullptr, vcls, nnz, m, n, A->data, A->ld,
+                              (unsigned *)(b + off_err))) != MELA_OK)
+        return st;
+    unsigned err = 0;
+    MELA_HIP(hipMemcpyAsync(&err, b + off_err, sizeof(err), hipMemcpyDeviceToHost, ctx->stream));
+    MELA_HIP(hipStreamSynchronize(ctx->stream));
+    if (err) return set_error(MELA_ERR_INVALID, "row index out of range in ir");
+    return MELA_OK;
+}
+
 mela_status mela_pack_dev(mela_ctx *ctx, const void *src_dev, int src_class, mela_bitmat *M) {
     if (!ctx) return set_error(MELA_ERR_INVALID, "null context");
     mela_status st = check_bitmat(M, "M");

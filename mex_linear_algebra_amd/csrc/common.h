@@ -108,6 +108,9 @@ mela_status launch_transpose(hipStream_t s, const uint64_t *src, size_t lds,
 mela_status launch_gen_bernoulli(hipStream_t s, uint64_t seed, uint64_t *dst,
                                  size_t rows, size_t cols, size_t ld);
 mela_status launch_fill_zero(hipStream_t s, void *dst, size_t bytes);
+mela_status launch_pack_csc(hipStream_t s, const uint64_t *jc, const uint64_t *ir,
+                            const void *vals, int vcls, size_t nnz, size_t m, size_t n,
+                            uint64_t *dst, size_t ld, unsigned *err);
 mela_status launch_copy(hipStream_t s, void *dst, const void *src, size_t bytes);
 
 // matmul.hip

@@ -49,6 +49,37 @@ __global__ void __launch_bounds__(256) k_pack(const T *__restrict__ src, size_t 
     }
 }
 
+// Compressed sparse column input (MATLAB's jc / ir / pr arrays): one thread
+// per stored entry finds its column by binary search over jc (jc[0] = 0,
+// non-decreasing, jc[n] = nnz, checked by the host) and sets its bit with an
+// atomic OR, since entries of different columns share packed words.  Value
+// classes: none (every stored entry is 1), logical (nonzero), double (the
+// dense rule, ((int)x) & 1).  Row indices >= m are flagged in *err and
+// skipped.
+__global__ void __launch_bounds__(256)
+k_pack_csc(const uint64_t *__restrict__ jc, const uint64_t *__restrict__ ir,
+           const void *__restrict__ vals, int vcls, size_t nnz, size_t m, size_t n,
+           uint64_t *__restrict__ dst, size_t ld, unsigned *__restrict__ err) {
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < nnz;
+         k += (size_t)gridDim.x * 256) {
+        size_t lo = 0, hi = n;  // jc[lo] <= k < jc[hi]
+        while (hi - lo > 1) {
+            const size_t mid = (lo + hi) >> 1;
+            if (jc[mid] <= k) lo = mid; else hi = mid;
+        }
+        const uint64_t row = ir[k];
+        if (row >= m) {
+            atomicOr(err, 1u);
+            continue;
+        }
+        uint32_t bit = 1u;
+        if (vcls == MELA_LOGICAL) bit = ((const uint8_t *)vals)[k] != 0;
+        else if (vcls == MELA_DOUBLE) bit = parity_of(((const double *)vals)[k]);
+        if (bit)
+            atomicOr((unsigned long long *)&dst[row * ld + (lo >> 6)], 1ull << (lo & 63));
+    }
+}
+
 // One thread per (row, word): 64 byte stores, each coalesced across the
 // consecutive rows of a wave (column-major logical output).
 __global__ void __launch_bounds__(256) k_unpack(const uint64_t *__restrict__ src, size_t ld,
@@ -163,6 +194,18 @@ mela_status launch_pack(hipStream_t s, const void *src, int cls, size_t rows, si
     else
         hipLaunchKernelGGL(k_pack<uint8_t>, grid, dim3(256), 0, s, (const uint8_t *)src, rows,
                            cols, dst, ld);
+    MELA_HIP(hipGetLastError());
+    return MELA_OK;
+}
+
+mela_status launch_pack_csc(hipStream_t s, const uint64_t *jc, const uint64_t *ir,
+                            const void *vals, int vcls, size_t nnz, size_t m, size_t n,
+                            uint64_t *dst, size_t ld, unsigned *err) {
+    if (nnz == 0) return MELA_OK;
+    size_t blocks = (nnz + 255) / 256;
+    if (blocks > 65536) blocks = 65536;  // grid-stride beyond that
+    hipLaunchKernelGGL(k_pack_csc, dim3((unsigned)blocks), dim3(256), 0, s, jc, ir, vals, vcls, nnz,
+                       m, n, dst, ld, err);
     MELA_HIP(hipGetLastError());
     return MELA_OK;
 }

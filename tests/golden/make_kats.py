@@ -26,6 +26,10 @@ KATS = [
     {"op": "null", "src": "gf2rref.m:13-32 (R, jb=[2 3]) -> canonical null", "kind": "logical",
      "A": [[0, 0, 1, 1], [0, 1, 0, 0], [0, 1, 1, 1], [0, 1, 0, 0]],
      "Z": [[1, 0], [0, 0], [0, 1], [0, 1]], "pivots": [1, 2]},
+    # reduced row echelon form with pivot columns (gf2rref's worked example)
+    {"op": "rref", "src": "gf2rref.m:13-33", "kind": "logical",
+     "A": [[0, 0, 1, 1], [0, 1, 0, 0], [0, 1, 1, 1], [0, 1, 0, 0]],
+     "R": [[0, 1, 0, 0], [0, 0, 1, 1], [0, 0, 0, 0], [0, 0, 0, 0]], "jb": [2, 3]},
     # matmul
     {"op": "matmul", "src": "run_ci_tests.m:93-97", "kind": "double",
      "A": [[2, 3, 4], [5, 6, 7], [8, 9, 10]], "B": [[1, 2], [3, 4], [5, 6]],

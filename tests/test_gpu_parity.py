@@ -54,6 +54,10 @@ def test_kats(mela):
                 Z = fn(A)
                 assert Z.dtype == bool
                 assert np.array_equal(Z, np.array(case["Z"], dtype=bool)), case["src"]
+        elif case["op"] == "rref":
+            R, jb = mela.mela_rref_gf2(A)
+            assert np.array_equal(R, np.array(case["R"], dtype=bool)), case["src"]
+            assert jb.ravel().tolist() == case["jb"], case["src"]
         else:
             B = _kat_matrix(case, "B") if "B" in case else rng.integers(0, 2, case["Bshape"]).astype(bool)
             for fn in (mela.mela_matmul_gf2, mela.mela_matmul_m4ri):

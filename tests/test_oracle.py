@@ -45,6 +45,16 @@ def test_kat_null(orc, method):
         assert np.array_equal(Z, np.array(case["Z"], dtype=bool)), case["src"]
 
 
+def test_kat_rref(orc):
+    # the standard RREF (null_gf2.c:88-131) leaves gf2rref's R and jb
+    for case in _kats("rref"):
+        A = _kat_matrix(case)
+        m, n = A.shape
+        M, piv = orc.rref_packed(orc.pack(A), m, n, "gf2")
+        assert np.array_equal(orc.unpack(M, m, n), np.array(case["R"], dtype=bool)), case["src"]
+        assert [int(c) + 1 for c in piv] == case["jb"], case["src"]
+
+
 @pytest.mark.parametrize("method", ["gf2", "m4ri"])
 def test_kat_matmul(orc, method):
     rng = np.random.default_rng(42)
