@@ -409,8 +409,9 @@ def bench_c1(args, mela):
     from oracle import oracle as orc
     rng = np.random.default_rng(1)
     n = 2048
-    A = rng.integers(0, 2, (n, n)).astype(bool)
-    B = rng.integers(0, 2, (n, n)).astype(bool)
+    # column-major like a MATLAB array, so no layout conversion is timed
+    A = np.asfortranarray(rng.integers(0, 2, (n, n)).astype(bool))
+    B = np.asfortranarray(rng.integers(0, 2, (n, n)).astype(bool))
     ctx = mela.Context(0)
 
     def timeit(fn, reps):
