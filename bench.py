@@ -61,34 +61,58 @@ class HipEvents:
         return float(ms.value)
 
 
+def reference_timing(fn):
+    """The reference benchmark's timing policy (benchmark_mex.m:671-707): one
+    warm-up call; double the iteration count until a round takes >= 10 ms;
+    then rounds of ceil(0.10 s / per-call) iterations, 5 rounds (2 when a call
+    takes >= 0.5 s); the median per-call time in seconds, and the rounds."""
+    fn()
+    iters = 1
+    while True:
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            fn()
+        dt = time.perf_counter() - t0
+        if dt >= 0.010:
+            break
+        iters *= 2
+    per = dt / iters
+    iters = max(1, int(-(-0.10 // per)))
+    rounds = 2 if per >= 0.5 else 5
+    times = []
+    for _ in range(rounds):
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            fn()
+        times.append((time.perf_counter() - t0) / iters)
+    times.sort()
+    return times[len(times) // 2], rounds, iters
+
+
 def cpu_matmul_baseline(n, budget_s=25.0):
     """The oracle's restatement of the reference M4RI product (matmul_m4ri.c:
     178-248, kernel-only on packed operands, OpenMP over 256-row blocks) timed
-    on the host cores: the full n^3 workload when it fits the budget, else an
-    m x n x m sample (m halved until it does)."""
+    on the host cores with the reference's timing policy: the full n^3
+    workload when it fits the budget, else an m x n x m sample (m halved until
+    it does)."""
     from oracle import oracle as orc
     k = n
     m = n
     while True:
-        est_s = 2.0 * m * m * k / 20e12  # rough first guess at ~20 Tbit-op/s
-        if est_s * 3 <= budget_s or m <= 1024:
+        # warm-up + doubling + 2 rounds of one call each ~ 4 calls at this size
+        est_s = 2.0 * m * m * k / 40e12
+        if est_s * 4 <= budget_s or m <= 1024:
             break
         m //= 2
     Ap = orc.gen_bernoulli_packed(1, m, k)
     Bp = orc.gen_bernoulli_packed(2, k, m)
-    times = []
-    t_start = time.perf_counter()
-    while len(times) < 3 and (time.perf_counter() - t_start) < budget_s:
-        t0 = time.perf_counter()
-        orc.matmul_packed(Ap, m, k, Bp, m, "m4ri")
-        times.append(time.perf_counter() - t0)
-    times.sort()
-    dt = times[len(times) // 2]
+    dt, rounds, iters = reference_timing(lambda: orc.matmul_packed(Ap, m, k, Bp, m, "m4ri"))
     return {"value": round(2.0 * m * m * k / dt / 1e12, 3), "unit": "Tbitop/s",
             "cores": orc.num_threads(), "kind": "port",
             "sample": f"oracle restatement of the reference M4RI product (k=8 tables, 256-row "
-                      f"blocks, OpenMP) on packed {m}x{k} * {k}x{m} Bernoulli(1/2), median of "
-                      f"{len(times)} call(s), {dt:.2f} s each"}
+                      f"blocks, OpenMP) on packed {m}x{k} * {k}x{m} Bernoulli(1/2); timing policy "
+                      f"of benchmark_mex.m:671-707 (warm-up, {rounds} rounds x {iters} call(s), "
+                      f"median): {dt:.2f} s per call"}
 
 
 def resolve_auto(m, k, n):
@@ -411,7 +435,8 @@ def bench_c1(args, mela):
     """2048 x 2048 logical A, B (column-major host arrays, as MATLAB passes
     them): matmul (both entry points), rank(A) and null(A) end to end through
     the C ABI -- host copies, packing, kernels and unpacking included -- next to
-    the oracle's restatement of the reference on the same inputs."""
+    the oracle's restatement of the reference on the same inputs, both timed
+    with the reference benchmark's policy."""
     import numpy as np
     from oracle import oracle as orc
     rng = np.random.default_rng(1)
@@ -420,16 +445,6 @@ def bench_c1(args, mela):
     A = np.asfortranarray(rng.integers(0, 2, (n, n)).astype(bool))
     B = np.asfortranarray(rng.integers(0, 2, (n, n)).astype(bool))
     ctx = mela.Context(0)
-
-    def timeit(fn, reps):
-        fn()
-        t = []
-        for _ in range(reps):
-            t0 = time.perf_counter()
-            fn()
-            t.append(time.perf_counter() - t0)
-        t.sort()
-        return t[len(t) // 2] * 1e3
 
     rows = []
     for name, gfn, cfn, work in (
@@ -441,8 +456,8 @@ def bench_c1(args, mela):
             ("rank_m4ri", lambda: mela.mela_rank_m4ri(A, ctx=ctx), lambda: orc.rank(A, "m4ri"), None),
             ("null_gf2", lambda: mela.mela_null_gf2(A, ctx=ctx), lambda: orc.null(A, "gf2"), None),
             ("null_m4ri", lambda: mela.mela_null_m4ri(A, ctx=ctx), lambda: orc.null(A, "m4ri"), None)):
-        g = timeit(gfn, 5)
-        c = timeit(cfn, 3)
+        g = reference_timing(gfn)[0] * 1e3
+        c = reference_timing(cfn)[0] * 1e3
         row = {"workload": f"C1 {name} 2048x2048 logical, end to end (host in/out)",
                "gpu_ms": round(g, 3), "cpu_oracle_ms": round(c, 3), "speedup": round(c / g, 1),
                "cpu_threads": orc.num_threads()}
