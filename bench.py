@@ -89,28 +89,26 @@ def reference_timing(fn):
     return times[len(times) // 2], rounds, iters
 
 
-def cpu_matmul_baseline(n, budget_s=25.0):
+def cpu_matmul_baseline(m, k, n, budget_s=25.0):
     """The oracle's restatement of the reference M4RI product (matmul_m4ri.c:
     178-248, kernel-only on packed operands, OpenMP over 256-row blocks) timed
     on the host cores with the reference's timing policy: the full n^3
     workload when it fits the budget, else an m x n x m sample (m halved until
     it does)."""
     from oracle import oracle as orc
-    k = n
-    m = n
     while True:
         # warm-up + doubling + 2 rounds of one call each ~ 4 calls at this size
-        est_s = 2.0 * m * m * k / 40e12
+        est_s = 2.0 * m * n * k / 40e12
         if est_s * 4 <= budget_s or m <= 1024:
             break
         m //= 2
     Ap = orc.gen_bernoulli_packed(1, m, k)
-    Bp = orc.gen_bernoulli_packed(2, k, m)
-    dt, rounds, iters = reference_timing(lambda: orc.matmul_packed(Ap, m, k, Bp, m, "m4ri"))
-    return {"value": round(2.0 * m * m * k / dt / 1e12, 3), "unit": "Tbitop/s",
+    Bp = orc.gen_bernoulli_packed(2, k, n)
+    dt, rounds, iters = reference_timing(lambda: orc.matmul_packed(Ap, m, k, Bp, n, "m4ri"))
+    return {"value": round(2.0 * m * n * k / dt / 1e12, 3), "unit": "Tbitop/s",
             "cores": orc.num_threads(), "kind": "port",
             "sample": f"oracle restatement of the reference M4RI product (k=8 tables, 256-row "
-                      f"blocks, OpenMP) on packed {m}x{k} * {k}x{m} Bernoulli(1/2); timing policy "
+                      f"blocks, OpenMP) on packed {m}x{k} * {k}x{n} Bernoulli(1/2); timing policy "
                       f"of benchmark_mex.m:671-707 (warm-up, {rounds} rounds x {iters} call(s), "
                       f"median): {dt:.2f} s per call"}
 
@@ -144,9 +142,12 @@ def roofline_for(algo, achieved, traffic, big_tiles=True):
 def bench_matmul(args, mela):
     algo = {"auto": mela.ALGO_AUTO, "m4rm": mela.ALGO_M4RM, "andpop": mela.ALGO_ANDPOP,
             "mfma_i8": mela.ALGO_MFMA_I8, "mfma_fp4": mela.ALGO_MFMA_FP4}[args.algo]
-    n = args.n
+    if args.shape:
+        m, k, n = (int(v) for v in args.shape.lower().split("x"))
+    else:
+        m = k = n = args.n
     ctx = mela.Context(0)
-    A, B, Cm = ctx.bitmat(n, n), ctx.bitmat(n, n), ctx.bitmat(n, n)
+    A, B, Cm = ctx.bitmat(m, k), ctx.bitmat(k, n), ctx.bitmat(m, n)
     ctx.gen_bernoulli(1, A)
     ctx.gen_bernoulli(2, B)
     ctx.sync()
@@ -171,12 +172,14 @@ def bench_matmul(args, mela):
         ctx.matmul_packed(A, B, Cm, algo)
     kern_total, launches = ctx.profile_end()
     kern_ms = kern_total / max(launches, 1)
-    work = 2.0 * n * n * n
+    work = 2.0 * m * k * n
     tbit = work / (wall_ms * 1e-3) / 1e12
     achieved = work / (kern_ms * 1e-3) / 1e12
-    resolved = resolve_auto(n, n, n) if args.algo == "auto" else args.algo
+    resolved = resolve_auto(m, k, n) if args.algo == "auto" else args.algo
+    square = m == k == n
     out = {
-        "metric": "GF(2) matmul Tbitop/s at 32768^3",
+        "metric": ("GF(2) matmul Tbitop/s at 32768^3" if square and n == 32768
+                   else f"GF(2) matmul Tbitop/s at {m}x{k}x{n}"),
         "value": round(tbit, 2),
         "unit": "Tbitop/s",
         "n_gpus": 1,
@@ -192,17 +195,17 @@ def bench_matmul(args, mela):
                   if resolved == "mfma_i8" else
                   "u1 (GF(2) bits packed in u64 words; AND/XOR/popcount)"),
         "data": "synthetic Bernoulli(1/2) bits, splitmix64 seeds 1 (A) and 2 (B), generated on device",
-        "config": {"workload": f"packed GF(2) matmul {n}x{n} * {n}x{n}, operands resident in HBM",
+        "config": {"workload": f"packed GF(2) matmul {m}x{k} * {k}x{n}, operands resident in HBM",
                    "algo": args.algo, "algo_resolved": resolved, "parallelism": "1 GPU"},
         "roofline": roofline_for(resolved, achieved,
                                  args.traffic if args.traffic is not None
-                                 else MEASURED_TRAFFIC.get((resolved, n)),
-                                 big_tiles=((n + 255) // 256) ** 2 >= 256),
+                                 else MEASURED_TRAFFIC.get((resolved, n)) if square else None,
+                                 big_tiles=((m + 255) // 256) * ((n + 255) // 256) >= 256),
         "call_ms": round(dev_ms, 3),
         "kernel_ms": round(kern_ms, 3),
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_matmul_baseline(n)
+        out["cpu_baseline"] = cpu_matmul_baseline(m, k, n)
     print(json.dumps(out), flush=True)
     for M in (A, B, Cm):
         ctx.free(M)
@@ -490,6 +493,8 @@ def main():
     ap.add_argument("--algo", default="auto",
                     choices=["auto", "m4rm", "andpop", "mfma_i8", "mfma_fp4"])
     ap.add_argument("--n", type=int, default=32768)
+    ap.add_argument("--shape", default=None,
+                    help="MxKxN for a rectangular product (e.g. 65536x32768x4096)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", type=float, default=None,
                     help="HBM bytes per launch measured by rocprofv3 --pmc (scaled)")
