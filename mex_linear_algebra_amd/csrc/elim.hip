@@ -447,6 +447,14 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
             e.bufs[0] == Mw && e.bufs[1] == Q && e.bufs[2] == meta && e.bufs[3] == act)
             eg = &e;
     if (!eg) {
+        // bounded cache: the oldest shape (and its executable graph) makes room
+        constexpr size_t kMaxGraphs = 32;
+        if (ctx->elim_graphs.size() >= kMaxGraphs) {
+            ElimGraph &old = ctx->elim_graphs.front();
+            if (old.exec) (void)hipGraphExecDestroy(old.exec);
+            if (old.graph) (void)hipGraphDestroy(old.graph);
+            ctx->elim_graphs.erase(ctx->elim_graphs.begin());
+        }
         ElimGraph e;
         e.m = m, e.n = n, e.ld = ld, e.variant = variant, e.rref = rref ? 1 : 0;
         e.bufs[0] = Mw, e.bufs[1] = Q, e.bufs[2] = meta, e.bufs[3] = act;
