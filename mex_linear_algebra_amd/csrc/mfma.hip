@@ -19,6 +19,7 @@
 #include <type_traits>
 
 #include "common.h"
+#include "transpose_device.h"
 
 namespace mela {
 namespace {
@@ -72,6 +73,52 @@ k_expand(const uint64_t *__restrict__ src, size_t lds, size_t rows, size_t kw, s
     uint4 *p = reinterpret_cast<uint4 *>(dst + i * pitch + c * 32);
     p[0] = make_uint4(o[0], o[1], o[2], o[3]);
     p[1] = make_uint4(o[4], o[5], o[6], o[7]);
+}
+
+// B^T expanded straight from B: one wave per 64 x 64 bit block of B (64
+// k-rows, 64 columns).  After the in-wave transpose lane c holds column
+// 64*jb + c for those 64 k values and writes their expansion to row 64*jb + c
+// of the expanded B^T.  Columns >= n and k rows >= k come out as zeros, which
+// is the padding the product kernels expect.
+template <bool FP4>
+__global__ void __launch_bounds__(256)
+k_transpose_expand(const uint64_t *__restrict__ B, size_t ldb, size_t k, size_t n, size_t kwp,
+                   uint8_t *__restrict__ dst, size_t pitch) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const size_t kb = (size_t)blockIdx.x * 4 + wave;  // 64-row block of B (k)
+    const size_t jb = blockIdx.y;                      // 64-column block of B
+    if (kb >= kwp) return;
+    const size_t i = kb * 64 + lane;
+    const size_t nw = (n + 63) >> 6;
+    uint64_t v = 0;
+    if (i < k && jb < nw) {
+        v = B[i * ldb + jb];
+        if (jb == nw - 1 && (n & 63)) v &= (1ull << (n & 63)) - 1ull;
+    }
+    v = dev::transpose64(v, lane);
+    uint4 *out = reinterpret_cast<uint4 *>(dst + (jb * 64 + lane) * pitch + kb * (FP4 ? 32 : 64));
+    if constexpr (FP4) {
+        uint32_t o[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const uint32_t b = (uint32_t)(v >> (8 * q)) & 0xffu;
+            uint32_t d = 0;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) d |= ((b >> t) & 1u) << (4 * t + 1);
+            o[q] = d;
+        }
+        out[0] = make_uint4(o[0], o[1], o[2], o[3]);
+        out[1] = make_uint4(o[4], o[5], o[6], o[7]);
+    } else {
+        uint32_t o[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const uint32_t b = (uint32_t)(v >> (4 * q)) & 0xfu;
+            o[q] = (b & 1u) | ((b & 2u) << 7) | ((b & 4u) << 14) | ((b & 8u) << 21);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) out[q] = make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+    }
 }
 
 // Workgroup -> C tile.  Workgroups are dealt to the 8 XCDs round-robin, so
@@ -415,23 +462,25 @@ mela_status matmul_mfma(mela_ctx *ctx, const uint64_t *A, size_t lda, const uint
     const size_t xbytes = kw * (fp4 ? 32 : 64);  // expanded bytes per row
     const size_t pitch = (xbytes + KPAD - 1) / KPAD * KPAD;
     const size_t mp = (m + XROWS - 1) / XROWS * XROWS, np = (n + XROWS - 1) / XROWS * XROWS;
-    void *pa = nullptr, *pb = nullptr, *pbt = nullptr;
+    void *pa = nullptr, *pb = nullptr;
     mela_status st;
-    if ((st = scratch_get(ctx, SLOT_BT, n * kw * 8, &pbt)) != MELA_OK) return st;
     if ((st = scratch_get(ctx, SLOT_XA, mp * pitch, &pa)) != MELA_OK) return st;
     if ((st = scratch_get(ctx, SLOT_XB, np * pitch, &pb)) != MELA_OK) return st;
-    if ((st = launch_transpose(s, B, ldb, k, n, (uint64_t *)pbt, kw)) != MELA_OK) return st;
-    const size_t ca = mp * (pitch / 32), cb = np * (pitch / 32);
+    const size_t ca = mp * (pitch / 32);
+    const size_t kwp = pitch / (fp4 ? 32 : 64);  // 64-bit k words the padded pitch holds
+    if (np / 64 > 65535 || (kwp + 3) / 4 > 0x7fffffffu)
+        return set_error(MELA_ERR_INVALID, "too many columns for the matrix-core product");
+    const dim3 gt((unsigned)((kwp + 3) / 4), (unsigned)(np / 64));
     if (fp4) {
         hipLaunchKernelGGL(k_expand<true>, dim3((unsigned)((ca + 255) / 256)), dim3(256), 0, s, A,
                            lda, m, kw, mp, pitch, (uint8_t *)pa);
-        hipLaunchKernelGGL(k_expand<true>, dim3((unsigned)((cb + 255) / 256)), dim3(256), 0, s,
-                           (const uint64_t *)pbt, kw, n, kw, np, pitch, (uint8_t *)pb);
+        hipLaunchKernelGGL(k_transpose_expand<true>, gt, dim3(256), 0, s, B, ldb, k, n, kwp,
+                           (uint8_t *)pb, pitch);
     } else {
         hipLaunchKernelGGL(k_expand<false>, dim3((unsigned)((ca + 255) / 256)), dim3(256), 0, s, A,
                            lda, m, kw, mp, pitch, (uint8_t *)pa);
-        hipLaunchKernelGGL(k_expand<false>, dim3((unsigned)((cb + 255) / 256)), dim3(256), 0, s,
-                           (const uint64_t *)pbt, kw, n, kw, np, pitch, (uint8_t *)pb);
+        hipLaunchKernelGGL(k_transpose_expand<false>, gt, dim3(256), 0, s, B, ldb, k, n, kwp,
+                           (uint8_t *)pb, pitch);
     }
     MELA_HIP(hipGetLastError());
     const uint8_t *xa = (const uint8_t *)pa, *xb = (const uint8_t *)pb;

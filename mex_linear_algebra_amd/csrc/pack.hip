@@ -3,6 +3,7 @@
 // Packed layout (shared with the reference, matmul_gf2.c:33-36): row-major,
 // `ld` 64-bit words per row, bit j of a row at word j>>6, bit j&63.
 #include "common.h"
+#include "transpose_device.h"
 
 namespace mela {
 namespace {
@@ -95,33 +96,6 @@ __global__ void __launch_bounds__(256) k_unpack(const uint64_t *__restrict__ src
     for (size_t b = 0; b < jn; ++b) out[b * rows] = (uint8_t)((word >> b) & 1u);
 }
 
-__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int s) {
-    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
-    lo = __shfl_xor(lo, s, 64);
-    hi = __shfl_xor(hi, s, 64);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// In-wave 64x64 bit transpose: lane l holds row l; afterwards lane c holds
-// column c (bit l = old row l, bit c).  Six butterfly levels swap the
-// off-diagonal s x s sub-blocks.
-__device__ __forceinline__ uint64_t transpose64(uint64_t x, int lane) {
-    const uint64_t masks[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull,
-                               0x00FF00FF00FF00FFull, 0x0F0F0F0F0F0F0F0Full,
-                               0x3333333333333333ull, 0x5555555555555555ull};
-#pragma unroll
-    for (int lv = 0; lv < 6; ++lv) {
-        int s = 32 >> lv;
-        uint64_t m = masks[lv];
-        uint64_t y = shfl_xor64(x, s);
-        if ((lane & s) == 0)
-            x = (x & m) | ((y & m) << s);
-        else
-            x = (x & ~m) | ((y & ~m) >> s);
-    }
-    return x;
-}
-
 // dst (cols x rows, packed along rows) = transpose of src (rows x cols).
 // One wave per 64x64 bit block; blockIdx.x = row block, blockIdx.y = word.
 __global__ void __launch_bounds__(64) k_transpose(const uint64_t *__restrict__ src, size_t lds,
@@ -131,7 +105,7 @@ __global__ void __launch_bounds__(64) k_transpose(const uint64_t *__restrict__ s
     size_t bi = blockIdx.x, bw = blockIdx.y;
     size_t r = bi * 64 + lane;
     uint64_t x = r < rows ? src[r * lds + bw] : 0;
-    x = transpose64(x, lane);
+    x = dev::transpose64(x, lane);
     size_t c = bw * 64 + lane;
     if (c < cols) dst[c * ldd + bi] = x;
 }
