@@ -7,7 +7,13 @@
 // inside the workgroup (panel select -> pivot rows -> table or direct XOR
 // sweep -> earlier pivot rows), then write its rank and its packed null
 // basis.  Per-matrix mode (chosen by the autotuner): GE (XOR each selected
-// pivot row) or M4RI with 4-bit tables (16 table lookups per row word).
+// pivot row) or M4RI with 4-bit tables (16 table lookups per row word); the
+// update tile width (16 or 32 words) is chosen by the autotuner per batch.
+//
+// Per panel and tile of TW words: the selected rows' words are staged in LDS,
+// the tile's pivot rows are formed from them in LDS (and stored to Q), the
+// M4RI tables are built from those, and the sweep keeps 4 row passes' loads
+// in flight per wave.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -33,11 +39,12 @@ namespace {
 
 constexpr int BT = 256;          // threads per workgroup
 constexpr int BT_WAVES = BT / 64;
-constexpr int TW = 16;           // words per update tile
 constexpr int MAX_M = 4096;
 constexpr int MAX_N = 8192;
 constexpr int MAX_WC = MAX_N / 64;
+constexpr int SW_PASSES = 16;    // row passes per wave with loads in flight together
 
+template <int TW>
 struct BatchLDS {
     dev::SelectLDS<BT> sel;
     uint64_t comb[64];
@@ -45,27 +52,43 @@ struct BatchLDS {
     uint16_t act[MAX_M];
     uint64_t pmask[MAX_WC];  // pivot column bitmap
     int pbefore[MAX_WC];     // pivots in columns below 64*w
+    uint64_t pw[64];         // word g of the panel's pivot rows, by pivot bit
+    // S (selected rows of the tile) and P (its pivot rows, by pivot bit) sit
+    // at the two ends of the region the M4RI tables T are built in: P is read
+    // into registers before T overwrites it; the null extraction's column
+    // word tables reuse the region after the elimination.
     union {
-        uint64_t T[16][16][TW];    // M4RI nibble tables (elimination)
-        uint64_t P[64][TW];        // GE pivot rows (elimination)
-        uint16_t freecol[MAX_N];   // free columns ascending (null extraction)
+        struct {
+            uint64_t S[64][TW];
+            uint64_t gap[128 * TW];
+            uint64_t P[64][TW];
+        } sp;
+        uint64_t T[16][16][TW];
+        struct {
+            uint64_t fm[MAX_WC];     // free-column mask of each column word
+            uint64_t mv[MAX_WC][6];  // its compress masks
+            int fb[MAX_WC];          // free columns before the word
+            int ws[MAX_WC + 1];      // column word holding free column 64 j
+        } nl;
     } u;
     int s_idx;
     int s_r, s_nact;
 };
 
-__device__ __forceinline__ int pivots_before(const BatchLDS &L, int p) {
+template <int TW>
+__device__ __forceinline__ int pivots_before(const BatchLDS<TW> &L, int p) {
     const int w = p >> 6;
     return L.pbefore[w] + dev::popc64(L.pmask[w] & ((1ull << (p & 63)) - 1ull));
 }
 
+template <int TW>
 __global__ void __launch_bounds__(BT)
 k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order, int count,
              int *__restrict__ counter, const uint64_t *const *__restrict__ inputs,
              uint64_t *__restrict__ work, uint64_t *__restrict__ zarena, uint32_t *__restrict__ ranks,
              int want_null) {
     extern __shared__ __align__(16) unsigned char smem[];
-    BatchLDS &L = *reinterpret_cast<BatchLDS *>(smem);
+    BatchLDS<TW> &L = *reinterpret_cast<BatchLDS<TW> *>(smem);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
     for (;;) {
@@ -82,9 +105,21 @@ k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order, i
         uint64_t *Q = work + d.q_off;
         const uint64_t *A = inputs[mi];
 
-        for (int i = tid; i < m * wc; i += BT) {
-            const int r = i / wc, w = i - r * wc;
-            M[i] = A[(size_t)r * d.lda + w];
+        {
+            constexpr int CU = 8;  // words per thread with loads in flight together
+            const int mw = m * wc;
+            for (int i0 = tid; i0 < mw; i0 += BT * CU) {
+                uint64_t v[CU];
+#pragma unroll
+                for (int k = 0; k < CU; ++k) {
+                    const int i = i0 + k * BT < mw ? i0 + k * BT : i0;
+                    const int r = i / wc, w = i - r * wc;
+                    v[k] = A[(size_t)r * d.lda + w];
+                }
+#pragma unroll
+                for (int k = 0; k < CU; ++k)
+                    if (i0 + k * BT < mw) M[i0 + k * BT] = v[k];
+            }
         }
         for (int i = tid; i < m; i += BT) L.act[i] = (uint16_t)i;
         for (int i = tid; i < wc; i += BT) L.pmask[i] = 0;
@@ -103,91 +138,136 @@ k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order, i
             const uint64_t pm = po.pm;
             if (pm == 0) continue;  // no pivot in this panel (uniform)
             const int rho = po.rho;
-            // pivot rows Q[r0 + j], words g .. wc-1 (words below g are zero)
-            for (int i = tid; i < rho * wc; i += BT) {
-                const int j = i / wc, w = i - j * wc;
-                uint64_t t = pm;
-                for (int q = 0; q < j; ++q) t &= t - 1;
-                const int c = dev::ffs64(t);
-                uint64_t acc = 0;
-                if (w >= g)
-                    for (uint64_t s = L.comb[c]; s; s &= s - 1)
-                        acc ^= M[(size_t)L.selrow[dev::ffs64(s)] * wc + w];
-                Q[(size_t)(r0 + j) * wc + w] = acc;
-            }
             if (tid == 0) {
                 L.pmask[g] = pm;
                 L.s_r = r0 + rho;
                 L.s_nact = po.n_act_new;
             }
-            __syncthreads();
-            // sweep: active rows and (for the null basis) earlier pivot rows
+            // the new pivot rows are zero left of their panel
+            for (int e = tid; e < rho * g; e += BT) {
+                const int j = e / g, w = e - j * g;
+                Q[(size_t)(r0 + j) * wc + w] = 0ull;
+            }
+            // active rows and (for the null basis) earlier pivot rows
             const int nact = po.n_act_new;
             const int total = nact + (want_null ? r0 : 0);
-            // 4 rows per wave pass, 16 lanes (words) per row
-            const int sub = lane >> 4, wl = lane & 15;
-            for (int w0 = g + 1; w0 < wc; w0 += TW) {
-                const int w = w0 + wl;
-                const bool wok = w < wc;
-                if (d.mode == 1) {
-                    for (int e = tid; e < 16 * TW; e += BT) {
-                        const int t = e / TW, ww = e - t * TW;
-                        uint64_t q[4];
+            constexpr int RPP = 64 / TW;  // rows per wave pass, TW lanes (words) per row
+            const int sub = lane / TW, wl = lane % TW;
+            for (int w0 = g; w0 < wc; w0 += TW) {
+                // selected rows' words of the tile (loads issued together;
+                // slots past rho and words past wc read row 0 and store 0)
+                {
+                    constexpr int SE = 64 * TW / BT;
+                    uint64_t sv[SE];
 #pragma unroll
-                        for (int b = 0; b < 4; ++b) {
-                            const int c = 4 * t + b;
-                            q[b] = 0;
-                            if (w0 + ww < wc && ((pm >> c) & 1ull))
-                                q[b] = Q[(size_t)(r0 + dev::popc64(pm & ((1ull << c) - 1ull))) * wc +
-                                         w0 + ww];
-                        }
+                    for (int k = 0; k < SE; ++k) {
+                        const int e = tid + k * BT, slot = e / TW, ww = e - slot * TW;
+                        const bool ok = slot < rho && w0 + ww < wc;
+                        sv[k] = M[ok ? (size_t)L.selrow[slot] * wc + w0 + ww : 0];
+                    }
+#pragma unroll
+                    for (int k = 0; k < SE; ++k) {
+                        const int e = tid + k * BT, slot = e / TW, ww = e - slot * TW;
+                        L.u.sp.S[slot][ww] = (slot < rho && w0 + ww < wc) ? sv[k] : 0ull;
+                    }
+                }
+                __syncthreads();
+                // pivot rows of the tile: XOR of the selected rows in comb[c]
+                for (int e = tid; e < 64 * TW; e += BT) {
+                    const int c = e / TW, ww = e - c * TW;
+                    const int w = w0 + ww;
+                    uint64_t v = 0;
+                    if ((pm >> c) & 1ull) {
+                        for (uint64_t y = L.comb[c]; y; y &= y - 1) v ^= L.u.sp.S[dev::ffs64(y)][ww];
+                        if (w < wc) Q[(size_t)(r0 + dev::popc64(pm & ((1ull << c) - 1ull))) * wc + w] = v;
+                    }
+                    L.u.sp.P[c][ww] = v;
+                    if (w == g) L.pw[c] = v;
+                }
+                __syncthreads();
+                if (d.mode == 1) {
+                    // P -> registers, then the tables over the same LDS
+                    constexpr int TPT = (16 * TW + BT - 1) / BT;  // tables per thread
+                    uint64_t q[TPT][4];
+#pragma unroll
+                    for (int k = 0; k < TPT; ++k) {
+                        const int e = tid + k * BT, t = e / TW, ww = e - t * TW;
+#pragma unroll
+                        for (int b = 0; b < 4; ++b) q[k][b] = e < 16 * TW ? L.u.sp.P[4 * t + b][ww] : 0ull;
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int k = 0; k < TPT; ++k) {
+                        const int e = tid + k * BT, t = e / TW, ww = e - t * TW;
+                        if (e >= 16 * TW) continue;
                         uint64_t cur = 0;
                         L.u.T[t][0][ww] = 0;
 #pragma unroll
                         for (int v = 1; v < 16; ++v) {
-                            cur ^= q[__builtin_ctz(v)];
+                            cur ^= q[k][__builtin_ctz(v)];
                             L.u.T[t][v ^ (v >> 1)][ww] = cur;
                         }
                     }
-                } else {
-                    for (int e = tid; e < 64 * TW; e += BT) {
-                        const int c = e / TW, ww = e - c * TW;
-                        uint64_t v = 0;
-                        if (w0 + ww < wc && ((pm >> c) & 1ull))
-                            v = Q[(size_t)(r0 + dev::popc64(pm & ((1ull << c) - 1ull))) * wc + w0 + ww];
-                        L.u.P[c][ww] = v;
-                    }
+                    __syncthreads();
                 }
-                __syncthreads();
-                for (int q0 = wave * 4 + sub; q0 < total; q0 += BT_WAVES * 4) {
-                    uint64_t *rp = q0 < nact ? M + (size_t)L.act[q0] * wc : Q + (size_t)(q0 - nact) * wc;
-                    uint64_t x = rp[g] & pm;
-                    if (x == 0 || !wok) continue;
-                    uint64_t a = rp[w];
-                    if (d.mode == 1) {
+                // sweep words > g of the tile; SW_PASSES passes' loads together
+                const int w = w0 + wl;
+                const bool wok = w < wc && w > g;
+                const int wsafe = wok ? w : g;
+                for (int q0 = wave * RPP + sub; q0 < total; q0 += BT_WAVES * RPP * SW_PASSES) {
+                    // every pass loads unconditionally (rows past the end
+                    // re-read row 0 of M) so all loads are in flight together
+                    uint64_t *rp[SW_PASSES];
+                    uint64_t x[SW_PASSES], a[SW_PASSES];
 #pragma unroll
-                        for (int t = 0; t < 16; ++t) a ^= L.u.T[t][(x >> (4 * t)) & 15u][wl];
-                    } else {
-                        while (x) {
-                            a ^= L.u.P[dev::ffs64(x)][wl];
-                            x &= x - 1;
-                        }
+                    for (int j = 0; j < SW_PASSES; ++j) {
+                        const int q = q0 + j * BT_WAVES * RPP;
+                        const int qa = q < nact ? q : 0;
+                        uint64_t *row = q < nact ? M + (size_t)L.act[qa] * wc : Q + (size_t)(q - nact) * wc;
+                        rp[j] = q < total ? row : M;
                     }
-                    rp[w] = a;
+#pragma unroll
+                    for (int j = 0; j < SW_PASSES; ++j) x[j] = rp[j][g];
+#pragma unroll
+                    for (int j = 0; j < SW_PASSES; ++j) a[j] = rp[j][wsafe];
+#pragma unroll
+                    for (int j = 0; j < SW_PASSES; ++j) {
+                        const int q = q0 + j * BT_WAVES * RPP;
+                        uint64_t xx = q < total ? x[j] & pm : 0ull;
+                        if (xx == 0 || !wok) continue;
+                        uint64_t aa = a[j];
+                        if (d.mode == 1) {
+#pragma unroll
+                            for (int t = 0; t < 16; ++t) aa ^= L.u.T[t][(xx >> (4 * t)) & 15u][wl];
+                        } else {
+                            while (xx) {
+                                aa ^= L.u.sp.P[dev::ffs64(xx)][wl];
+                                xx &= xx - 1;
+                            }
+                        }
+                        rp[j][w] = aa;
+                    }
                 }
                 __syncthreads();
             }
             // word g of the earlier pivot rows (after every tile read it)
             if (want_null) {
-                for (int q = tid; q < r0; q += BT) {
-                    const uint64_t x = Q[(size_t)q * wc + g];
-                    uint64_t a = x, h = x & pm;
-                    while (h) {
-                        const int c = dev::ffs64(h);
-                        h &= h - 1;
-                        a ^= Q[(size_t)(r0 + dev::popc64(pm & ((1ull << c) - 1ull))) * wc + g];
+                constexpr int PU = 4;  // rows per thread with loads in flight together
+                for (int q0 = tid; q0 < r0; q0 += BT * PU) {
+                    uint64_t xv[PU];
+#pragma unroll
+                    for (int k = 0; k < PU; ++k) {
+                        const int q = q0 + k * BT < r0 ? q0 + k * BT : q0;
+                        xv[k] = Q[(size_t)q * wc + g];
                     }
-                    Q[(size_t)q * wc + g] = a;
+#pragma unroll
+                    for (int k = 0; k < PU; ++k) {
+                        const int q = q0 + k * BT;
+                        if (q >= r0) continue;
+                        uint64_t a = xv[k];
+                        for (uint64_t h = xv[k] & pm; h; h &= h - 1) a ^= L.pw[dev::ffs64(h)];
+                        Q[(size_t)q * wc + g] = a;
+                    }
                 }
             }
             __syncthreads();
@@ -197,7 +277,7 @@ k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order, i
         if (tid == 0) ranks[mi] = (uint32_t)r;
         if (want_null) {
             const int f = n - r;
-            // prefix pivot counts per word, free column list
+            // prefix pivot counts per column word
             if (tid == 0) {
                 int acc = 0;
                 for (int w = 0; w < wc; ++w) {
@@ -206,31 +286,80 @@ k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order, i
                 }
             }
             __syncthreads();
-            for (int p = tid; p < n; p += BT)
-                if (!((L.pmask[p >> 6] >> (p & 63)) & 1ull)) L.u.freecol[p - pivots_before(L, p)] = (uint16_t)p;
+            // per column word: free mask, compress masks, free columns before
+            // it; ws[j] = the column word that holds free column 64 j
+            for (int w = tid; w < wc; w += BT) {
+                const int valid = n - 64 * w < 64 ? n - 64 * w : 64;
+                const uint64_t fm = ~L.pmask[w] & (valid == 64 ? ~0ull : ((1ull << valid) - 1ull));
+                dev::compress_masks(fm, L.u.nl.mv[w]);
+                L.u.nl.fm[w] = fm;
+                const int fb = 64 * w - L.pbefore[w];
+                L.u.nl.fb[w] = fb;
+                const int j = (fb + 63) >> 6;
+                if (64 * j < fb + dev::popc64(fm)) L.u.nl.ws[j] = w;
+            }
             __syncthreads();
             uint64_t *Z = zarena + d.z_off;
             const int ldz = (int)d.ldz;
             const int fw = (f + 63) >> 6;
-            for (int i = tid; i < n * ldz; i += BT) {
-                const int p = i / ldz, jw = i - p * ldz;
-                uint64_t word = 0;
-                if (jw < fw) {
-                    const bool isp = (L.pmask[p >> 6] >> (p & 63)) & 1ull;
-                    const int pb = pivots_before(L, p);
-                    if (isp) {
-                        const uint64_t *qrow = Q + (size_t)pb * wc;
-                        const int jn = f - jw * 64 < 64 ? f - jw * 64 : 64;
-                        for (int b = 0; b < jn; ++b) {
-                            const int fc = L.u.freecol[jw * 64 + b];
-                            word |= ((qrow[fc >> 6] >> (fc & 63)) & 1ull) << b;
+            // Z row p: a pivot column's row is its pivot row's free columns
+            // (word jw gathers the compressed column words ws[jw]..ws[jw+1]),
+            // a free column's row is its unit vector.  Four words per thread
+            // per step, their row-word loads issued together.
+            constexpr int NZ = 4;
+            for (int i0 = tid; i0 < n * ldz; i0 += BT * NZ) {
+                const uint64_t *qrow[NZ];
+                int lo[NZ], wa[NZ], wb[NZ];
+                uint64_t word[NZ];
+#pragma unroll
+                for (int k = 0; k < NZ; ++k) {
+                    const int i = i0 + k * BT;
+                    const int p = i / ldz, jw = i - p * ldz;
+                    qrow[k] = nullptr;
+                    word[k] = 0;
+                    wa[k] = 0;
+                    wb[k] = -1;
+                    lo[k] = 64 * jw;
+                    if (i < n * ldz && jw < fw) {
+                        const int pb = pivots_before(L, p);
+                        if ((L.pmask[p >> 6] >> (p & 63)) & 1ull) {
+                            qrow[k] = Q + (size_t)pb * wc;
+                            wa[k] = L.u.nl.ws[jw];
+                            wb[k] = jw + 1 < fw ? L.u.nl.ws[jw + 1] : wc - 1;
+                        } else {
+                            const int fi = p - pb;
+                            if ((fi >> 6) == jw) word[k] = 1ull << (fi & 63);
                         }
-                    } else {
-                        const int fi = p - pb;
-                        if ((fi >> 6) == jw) word = 1ull << (fi & 63);
                     }
                 }
-                Z[i] = word;
+                // the first three column words of each output word, loaded
+                // together (unit rows and words past the end re-read Q)
+                uint64_t x0[NZ], x1[NZ], x2[NZ];
+#pragma unroll
+                for (int k = 0; k < NZ; ++k) {
+                    const uint64_t *src = qrow[k] ? qrow[k] : Q;
+                    x0[k] = src[wa[k] < wc ? wa[k] : wc - 1];
+                    x1[k] = src[wa[k] + 1 < wc ? wa[k] + 1 : wc - 1];
+                    x2[k] = src[wa[k] + 2 < wc ? wa[k] + 2 : wc - 1];
+                }
+#pragma unroll
+                for (int k = 0; k < NZ; ++k) {
+                    // column word w holds free columns fb[w] .. fb[w] + popc - 1
+                    auto put = [&](int w, uint64_t xv) {
+                        const int fb = L.u.nl.fb[w];
+                        if (fb >= lo[k] + 64) return;
+                        const uint64_t c = dev::compress_apply(xv, L.u.nl.fm[w], L.u.nl.mv[w]);
+                        const int sh = fb - lo[k];
+                        word[k] |= sh >= 0 ? c << sh : c >> (-sh);
+                    };
+                    const int nw = wb[k] - wa[k] + 1;
+                    if (nw > 0) put(wa[k], x0[k]);
+                    if (nw > 1) put(wa[k] + 1, x1[k]);
+                    if (nw > 2) put(wa[k] + 2, x2[k]);
+                    for (int w = wa[k] + 3; w <= wb[k]; ++w) put(w, qrow[k][w]);
+                    const int i = i0 + k * BT;
+                    if (i < n * ldz) Z[i] = word[k];
+                }
             }
         }
         __syncthreads();
@@ -256,7 +385,7 @@ k_batch_popcount(const BatchMat *__restrict__ mats, const uint64_t *const *__res
 
 }  // namespace
 
-size_t batch_lds_bytes() { return sizeof(BatchLDS); }
+size_t batch_lds_bytes(int tw) { return tw == 32 ? sizeof(BatchLDS<32>) : sizeof(BatchLDS<16>); }
 
 }  // namespace mela
 
@@ -280,6 +409,8 @@ struct mela_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int tuned = -1;  // index of the autotuned mode assignment
+    int tw = 16;     // update tile width in words (16 or 32, autotuned)
+    int grid_tw[2] = {0, 0};  // persistent grid for tile widths 16 and 32
     bool ctx_scratch = false;  // arenas borrowed from the context (not freed)
 };
 
@@ -293,9 +424,15 @@ namespace mela {
 
 mela_status batch_launch_raw(mela_batch *b, hipStream_t s) {
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, b->d_counter);
-    hipLaunchKernelGGL(k_batch_rref, dim3(b->grid), dim3(BT), batch_lds_bytes(), s, b->d_mats,
-                       b->d_order, (int)b->mats.size(), b->d_counter, b->d_inputs, b->d_work, b->d_z, b->d_ranks,
-                       (b->flags & MELA_BATCH_NULL) ? 1 : 0);
+    const int want_null = (b->flags & MELA_BATCH_NULL) ? 1 : 0;
+    if (b->tw == 32)
+        hipLaunchKernelGGL(k_batch_rref<32>, dim3(b->grid_tw[1]), dim3(BT), batch_lds_bytes(32), s,
+                           b->d_mats, b->d_order, (int)b->mats.size(), b->d_counter, b->d_inputs,
+                           b->d_work, b->d_z, b->d_ranks, want_null);
+    else
+        hipLaunchKernelGGL(k_batch_rref<16>, dim3(b->grid_tw[0]), dim3(BT), batch_lds_bytes(16), s,
+                           b->d_mats, b->d_order, (int)b->mats.size(), b->d_counter, b->d_inputs,
+                           b->d_work, b->d_z, b->d_ranks, want_null);
     MELA_HIP(hipGetLastError());
     return MELA_OK;
 }
@@ -336,9 +473,12 @@ mela_status mela_batch_create(mela_ctx *ctx, const mela_bitmat *mats, size_t cou
     (void)hipSetDevice(ctx->device);
     static bool attr_set = false;
     if (!attr_set) {
-        MELA_HIP(hipFuncSetAttribute((const void *)k_batch_rref,
+        MELA_HIP(hipFuncSetAttribute((const void *)k_batch_rref<16>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)batch_lds_bytes()));
+                                     (int)batch_lds_bytes(16)));
+        MELA_HIP(hipFuncSetAttribute((const void *)k_batch_rref<32>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)batch_lds_bytes(32)));
         attr_set = true;
     }
     mela_batch *b = new mela_batch();
@@ -424,11 +564,14 @@ mela_status mela_batch_create(mela_ctx *ctx, const mela_bitmat *mats, size_t cou
         MELA_HIP(hipMemcpyAsync(b->d_order, b->order.data(), nb * sizeof(int), hipMemcpyHostToDevice, s));
         MELA_HIP(hipMemcpyAsync(b->d_inputs, b->inputs.data(), nb * sizeof(void *), hipMemcpyHostToDevice, s));
     }
-    int occ = 0;
-    MELA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)k_batch_rref, BT,
-                                                          batch_lds_bytes()));
-    if (occ < 1) occ = 1;
-    b->grid = std::max(1, std::min<int>((int)nb, occ * ctx->cu_count));
+    for (int v = 0; v < 2; ++v) {
+        int occ = 0;
+        const void *fn = v ? (const void *)k_batch_rref<32> : (const void *)k_batch_rref<16>;
+        MELA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, BT, batch_lds_bytes(v ? 32 : 16)));
+        if (occ < 1) occ = 1;
+        b->grid_tw[v] = std::max(1, std::min<int>((int)nb, occ * ctx->cu_count));
+    }
+    b->grid = b->grid_tw[0];
     MELA_HIP(hipStreamSynchronize(s));
     *out = b;
     return MELA_OK;
@@ -484,31 +627,47 @@ mela_status mela_batch_autotune(mela_batch *b, float *best_ms) {
     hipEvent_t e0, e1;
     MELA_HIP(hipEventCreate(&e0));
     MELA_HIP(hipEventCreate(&e1));
+    // search: tile width (16 / 32 words) x per-matrix mode assignment
     float best = 1e30f;
     size_t bi = 0;
-    for (size_t c = 0; c < cands.size(); ++c) {
-        mela_status st = mela_batch_set_modes(b, cands[c].data());
-        if (st != MELA_OK) return st;
-        float tot = 0.f;
-        for (int rep = 0; rep < 2; ++rep) {
-            MELA_HIP(hipEventRecord(e0, s));
-            if ((st = batch_launch_raw(b, s)) != MELA_OK) return st;
-            MELA_HIP(hipEventRecord(e1, s));
-            MELA_HIP(hipEventSynchronize(e1));
-            float ms = 0.f;
-            MELA_HIP(hipEventElapsedTime(&ms, e0, e1));
-            if (rep) tot = ms;  // second run (warm)
-        }
-        if (tot < best) {
-            best = tot;
-            bi = c;
+    int best_tw = 16;
+    for (int tw : {16, 32}) {
+        b->tw = tw;
+        for (size_t c = 0; c < cands.size(); ++c) {
+            mela_status st = mela_batch_set_modes(b, cands[c].data());
+            if (st != MELA_OK) return st;
+            float tot = 0.f;
+            for (int rep = 0; rep < 2; ++rep) {
+                MELA_HIP(hipEventRecord(e0, s));
+                if ((st = batch_launch_raw(b, s)) != MELA_OK) return st;
+                MELA_HIP(hipEventRecord(e1, s));
+                MELA_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                MELA_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (rep) tot = ms;  // second run (warm)
+            }
+            if (tot < best) {
+                best = tot;
+                bi = c;
+                best_tw = tw;
+            }
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    b->tw = best_tw;
     mela_status st = mela_batch_set_modes(b, cands[bi].data());
     if (st != MELA_OK) return st;
     b->tuned = (int)bi;
+    // a graph captured before tuning launches the old tile width
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+    }
+    if (b->graph) {
+        (void)hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
     if (best_ms) *best_ms = best;
     return MELA_OK;
 }

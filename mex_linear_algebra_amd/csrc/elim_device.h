@@ -61,6 +61,38 @@ __device__ __forceinline__ void wave_gj(uint64_t &v, uint64_t &comb, bool valid,
     pbit = piv ? ffs64(v) : -1;
 }
 
+// Bit compress (x's bits at the set positions of mask m, packed to the low
+// end in order) by the six-step parallel-suffix method of Hacker's Delight
+// (2nd ed., section 7-4, "compress").  The six move masks depend on m only, so
+// a null-basis extraction computes them once per column word and applies them
+// to every row word of that column.
+__host__ __device__ __forceinline__ void compress_masks(uint64_t m, uint64_t mv[6]) {
+    uint64_t mk = ~m << 1;  // bit j: count of zeros of m below j, parity by prefix XOR
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        uint64_t mp = mk ^ (mk << 1);
+        mp ^= mp << 2;
+        mp ^= mp << 4;
+        mp ^= mp << 8;
+        mp ^= mp << 16;
+        mp ^= mp << 32;
+        const uint64_t v = mp & m;  // bits that move right by 2^i in this step
+        mv[i] = v;
+        m = (m ^ v) | (v >> (1 << i));
+        mk &= ~mp;
+    }
+}
+
+__host__ __device__ __forceinline__ uint64_t compress_apply(uint64_t x, uint64_t m, const uint64_t mv[6]) {
+    x &= m;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const uint64_t t = x & mv[i];
+        x = (x ^ t) | (t >> (1 << i));
+    }
+    return x;
+}
+
 // XOR of v over the 64 lanes of the wave (every lane receives the result).
 __device__ __forceinline__ uint64_t wave_xor64(uint64_t v) {
     uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
@@ -165,10 +197,17 @@ __device__ PanelOut wg_panel_select(SelectLDS<NT> &L, ACT *act, int n_act, int g
         }
         __syncthreads();
     }
+    // chunks of NT active rows; the next chunk's words are loaded while the
+    // current chunk is reduced (positions past the end re-read a valid row)
+    uint64_t xnext = 0;
+    if (!L.fast && n_act > 0) xnext = row((int)act[tid < n_act ? tid : n_act - 1])[g];
     for (int base = 0; base < n_act && !L.fast; base += NT) {
         const int pos = base + tid;
-        uint64_t x = 0;
-        if (pos < n_act) x = row((int)act[pos])[g];
+        uint64_t x = pos < n_act ? xnext : 0ull;
+        {
+            const int pn = pos + NT;
+            xnext = row((int)act[pn < n_act ? pn : n_act - 1])[g];
+        }
         const uint64_t pm = L.pm;
         uint64_t y = x;
         uint64_t hit = x & pm;
