@@ -11,9 +11,10 @@
 // update tile width (16 or 32 words) is chosen by the autotuner per batch.
 //
 // Per panel and tile of TW words: the selected rows' words are staged in LDS,
-// the tile's pivot rows are formed from them in LDS (and stored to Q), the
-// M4RI tables are built from those, and the sweep keeps 4 row passes' loads
-// in flight per wave.
+// the tile's pivot rows are formed from 4-bit tables of them (and stored to
+// Q), the M4RI tables are built from those, and the sweep runs over the
+// panel's list of rows with a nonzero multiplier, 16 row passes per wave with
+// their loads in flight together, looked up table by table.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -50,20 +51,17 @@ struct BatchLDS {
     uint64_t comb[64];
     int selrow[64];
     uint16_t act[MAX_M];
+    uint16_t upd[MAX_M];     // sweep list of the panel (rows with a nonzero multiplier)
     uint64_t pmask[MAX_WC];  // pivot column bitmap
     int pbefore[MAX_WC];     // pivots in columns below 64*w
     uint64_t pw[64];         // word g of the panel's pivot rows, by pivot bit
-    // S (selected rows of the tile) and P (its pivot rows, by pivot bit) sit
-    // at the two ends of the region the M4RI tables T are built in: P is read
-    // into registers before T overwrites it; the null extraction's column
-    // word tables reuse the region after the elimination.
+    // One region, used in turn per tile: the selected rows' words, their
+    // tables, the pivot rows' tables (M4RI) or the pivot rows (GE); the null
+    // extraction's column word tables reuse it after the elimination.
     union {
-        struct {
-            uint64_t S[64][TW];
-            uint64_t gap[128 * TW];
-            uint64_t P[64][TW];
-        } sp;
-        uint64_t T[16][16][TW];
+        uint64_t S[64][TW];      // selected rows' words of the tile
+        uint64_t T[16][16][TW];  // 4-bit tables (of the selected rows, then of the pivot rows)
+        uint64_t P[64][TW];      // pivot rows of the tile, by pivot bit (GE sweep)
         struct {
             uint64_t fm[MAX_WC];     // free-column mask of each column word
             uint64_t mv[MAX_WC][6];  // its compress masks
@@ -151,6 +149,55 @@ k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order, i
             // active rows and (for the null basis) earlier pivot rows
             const int nact = po.n_act_new;
             const int total = nact + (want_null ? r0 : 0);
+            // rows of sweep positions: active rows by row id, earlier pivot
+            // rows (Q) by 0x8000 | index
+            auto sweep_row = [&](uint32_t code) -> uint64_t * {
+                return code & 0x8000u ? Q + (size_t)(code & 0x7fffu) * wc : M + (size_t)code * wc;
+            };
+            // list of the positions whose multiplier (word g & pm) is nonzero
+            int nupd = 0;
+            {
+                constexpr int CK = 4;  // positions per thread per round, loads together
+                for (int b0 = 0; b0 < total; b0 += BT * CK) {
+                    uint32_t code[CK];
+                    bool f[CK];
+#pragma unroll
+                    for (int k = 0; k < CK; ++k) {
+                        const int q = b0 + k * BT + tid;
+                        const int qa = q < nact ? q : 0;
+                        code[k] = q < nact ? (uint32_t)L.act[qa] : 0x8000u | (uint32_t)(q - nact);
+                        if (q >= total) code[k] = 0;
+                    }
+                    uint64_t xv[CK];
+#pragma unroll
+                    for (int k = 0; k < CK; ++k) xv[k] = sweep_row(code[k])[g];
+                    // wave-local slots, k-major: entry k of lane l goes to
+                    // (entries of rounds < k) + (lanes < l with entry k)
+                    int wsum = 0, off[CK];
+                    const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+                    for (int k = 0; k < CK; ++k) {
+                        f[k] = b0 + k * BT + tid < total && (xv[k] & pm) != 0ull;
+                        const uint64_t bal = __ballot(f[k]);
+                        off[k] = wsum + dev::popc64(bal & below);
+                        wsum += dev::popc64(bal);
+                    }
+                    if (lane == 0) L.sel.wcnt[wave] = wsum;
+                    __syncthreads();
+                    int base = nupd, all = 0;
+#pragma unroll
+                    for (int v = 0; v < BT_WAVES; ++v) {
+                        const int c = L.sel.wcnt[v];
+                        if (v < wave) base += c;
+                        all += c;
+                    }
+#pragma unroll
+                    for (int k = 0; k < CK; ++k)
+                        if (f[k]) L.upd[base + off[k]] = (uint16_t)code[k];
+                    nupd += all;
+                    __syncthreads();
+                }
+            }
             constexpr int RPP = 64 / TW;  // rows per wave pass, TW lanes (words) per row
             const int sub = lane / TW, wl = lane % TW;
             for (int w0 = g; w0 < wc; w0 += TW) {
@@ -168,34 +215,24 @@ k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order, i
 #pragma unroll
                     for (int k = 0; k < SE; ++k) {
                         const int e = tid + k * BT, slot = e / TW, ww = e - slot * TW;
-                        L.u.sp.S[slot][ww] = (slot < rho && w0 + ww < wc) ? sv[k] : 0ull;
+                        L.u.S[slot][ww] = (slot < rho && w0 + ww < wc) ? sv[k] : 0ull;
                     }
                 }
                 __syncthreads();
-                // pivot rows of the tile: XOR of the selected rows in comb[c]
-                for (int e = tid; e < 64 * TW; e += BT) {
-                    const int c = e / TW, ww = e - c * TW;
-                    const int w = w0 + ww;
-                    uint64_t v = 0;
-                    if ((pm >> c) & 1ull) {
-                        for (uint64_t y = L.comb[c]; y; y &= y - 1) v ^= L.u.sp.S[dev::ffs64(y)][ww];
-                        if (w < wc) Q[(size_t)(r0 + dev::popc64(pm & ((1ull << c) - 1ull))) * wc + w] = v;
-                    }
-                    L.u.sp.P[c][ww] = v;
-                    if (w == g) L.pw[c] = v;
+                // Thread (t, ww) owns table t of tile word ww: first the 4-bit
+                // tables of the selected rows 4t..4t+3, from which it forms
+                // the pivot rows 4t..4t+3 (pivot row c = XOR of the selected
+                // rows in comb[c], 16 lookups), then the tables of those.
+                constexpr int TPT = (16 * TW + BT - 1) / BT;  // tables per thread
+                uint64_t q[TPT][4];
+#pragma unroll
+                for (int k = 0; k < TPT; ++k) {
+                    const int e = tid + k * BT, t = e / TW, ww = e - t * TW;
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) q[k][b] = e < 16 * TW ? L.u.S[4 * t + b][ww] : 0ull;
                 }
                 __syncthreads();
-                if (d.mode == 1) {
-                    // P -> registers, then the tables over the same LDS
-                    constexpr int TPT = (16 * TW + BT - 1) / BT;  // tables per thread
-                    uint64_t q[TPT][4];
-#pragma unroll
-                    for (int k = 0; k < TPT; ++k) {
-                        const int e = tid + k * BT, t = e / TW, ww = e - t * TW;
-#pragma unroll
-                        for (int b = 0; b < 4; ++b) q[k][b] = e < 16 * TW ? L.u.sp.P[4 * t + b][ww] : 0ull;
-                    }
-                    __syncthreads();
+                auto build_tables = [&]() {
 #pragma unroll
                     for (int k = 0; k < TPT; ++k) {
                         const int e = tid + k * BT, t = e / TW, ww = e - t * TW;
@@ -205,26 +242,56 @@ k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order, i
 #pragma unroll
                         for (int v = 1; v < 16; ++v) {
                             cur ^= q[k][__builtin_ctz(v)];
-                            L.u.T[t][v ^ (v >> 1)][ww] = cur;
+                            L.u.T[t][v ^ (v >> 1)][ww] = cur;  // Gray-code order
                         }
                     }
-                    __syncthreads();
+                };
+                build_tables();
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < TPT; ++k) {
+                    const int e = tid + k * BT, t = e / TW, ww = e - t * TW;
+                    const int w = w0 + ww;
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        const int c = 4 * t + b;
+                        const bool pc = e < 16 * TW && ((pm >> c) & 1ull);
+                        const uint64_t cb = pc ? L.comb[c] : 0ull;
+                        uint64_t v = 0;
+#pragma unroll
+                        for (int u = 0; u < 16; ++u) v ^= L.u.T[u][(cb >> (4 * u)) & 15u][e < 16 * TW ? ww : 0];
+                        q[k][b] = v;
+                        if (pc && w < wc) Q[(size_t)(r0 + dev::popc64(pm & ((1ull << c) - 1ull))) * wc + w] = v;
+                        if (pc && w == g) L.pw[c] = v;
+                    }
                 }
+                __syncthreads();
+                if (d.mode == 1) {
+                    build_tables();
+                } else {
+#pragma unroll
+                    for (int k = 0; k < TPT; ++k) {
+                        const int e = tid + k * BT, t = e / TW, ww = e - t * TW;
+                        if (e >= 16 * TW) continue;
+#pragma unroll
+                        for (int b = 0; b < 4; ++b) L.u.P[4 * t + b][ww] = q[k][b];
+                    }
+                }
+                __syncthreads();
                 // sweep words > g of the tile; SW_PASSES passes' loads together
                 const int w = w0 + wl;
                 const bool wok = w < wc && w > g;
                 const int wsafe = wok ? w : g;
-                for (int q0 = wave * RPP + sub; q0 < total; q0 += BT_WAVES * RPP * SW_PASSES) {
-                    // every pass loads unconditionally (rows past the end
+#pragma unroll 1
+                for (int i0 = wave * RPP + sub; i0 < nupd; i0 += BT_WAVES * RPP * SW_PASSES) {
+                    // every pass loads unconditionally (entries past the end
                     // re-read row 0 of M) so all loads are in flight together
                     uint64_t *rp[SW_PASSES];
                     uint64_t x[SW_PASSES], a[SW_PASSES];
 #pragma unroll
                     for (int j = 0; j < SW_PASSES; ++j) {
-                        const int q = q0 + j * BT_WAVES * RPP;
-                        const int qa = q < nact ? q : 0;
-                        uint64_t *row = q < nact ? M + (size_t)L.act[qa] * wc : Q + (size_t)(q - nact) * wc;
-                        rp[j] = q < total ? row : M;
+                        const int i = i0 + j * BT_WAVES * RPP;
+                        rp[j] = i < nupd ? sweep_row(L.upd[i]) : M;
                     }
 #pragma unroll
                     for (int j = 0; j < SW_PASSES; ++j) x[j] = rp[j][g];
@@ -232,21 +299,25 @@ k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order, i
                     for (int j = 0; j < SW_PASSES; ++j) a[j] = rp[j][wsafe];
 #pragma unroll
                     for (int j = 0; j < SW_PASSES; ++j) {
-                        const int q = q0 + j * BT_WAVES * RPP;
-                        uint64_t xx = q < total ? x[j] & pm : 0ull;
-                        if (xx == 0 || !wok) continue;
-                        uint64_t aa = a[j];
-                        if (d.mode == 1) {
-#pragma unroll
-                            for (int t = 0; t < 16; ++t) aa ^= L.u.T[t][(xx >> (4 * t)) & 15u][wl];
-                        } else {
-                            while (xx) {
-                                aa ^= L.u.sp.P[dev::ffs64(xx)][wl];
-                                xx &= xx - 1;
-                            }
-                        }
-                        rp[j][w] = aa;
+                        const int i = i0 + j * BT_WAVES * RPP;
+                        x[j] = (i < nupd && wok) ? x[j] & pm : 0ull;
                     }
+                    if (d.mode == 1) {
+                        // table-major: one lookup of every pass per table, so
+                        // the passes' LDS reads overlap (a zero multiplier
+                        // reads the zero entry)
+#pragma unroll
+                        for (int t = 0; t < 16; ++t)
+#pragma unroll
+                            for (int j = 0; j < SW_PASSES; ++j) a[j] ^= L.u.T[t][(x[j] >> (4 * t)) & 15u][wl];
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < SW_PASSES; ++j)
+                            for (uint64_t xx = x[j]; xx; xx &= xx - 1) a[j] ^= L.u.P[dev::ffs64(xx)][wl];
+                    }
+#pragma unroll
+                    for (int j = 0; j < SW_PASSES; ++j)
+                        if (x[j]) rp[j][w] = a[j];
                 }
                 __syncthreads();
             }

@@ -104,6 +104,10 @@ __device__ __forceinline__ uint64_t wave_xor64(uint64_t v) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// Pivots the first 64 active rows must give for the selection to complete
+// them by single insertions instead of the general chunked path.
+constexpr int FAST_MIN_PIVOTS = 48;
+
 // LDS state of one panel selection.
 template <int NT>
 struct SelectLDS {
@@ -160,7 +164,9 @@ __device__ PanelOut wg_panel_select(SelectLDS<NT> &L, ACT *act, int n_act, int g
             wave_gj<true>(v, comb, true, piv, pbit, lane);
             uint64_t pivl = __ballot(piv);  // pivot lanes = selection slots
             int slotpos = lane;              // active-list position of slot `lane`
-            const int lim = n0 < 64 + 128 ? n0 : 64 + 128;
+            // single insertions only when the first 64 rows came close to 64
+            // pivots (sparse panels would scan 128 rows and then fail)
+            const int lim = popc64(pivl) < FAST_MIN_PIVOTS ? 0 : n0 < 64 + 128 ? n0 : 64 + 128;
             for (int pos = 64; pos < lim && pivl != ~0ull; ++pos) {
                 const uint64_t x = L.cy[pos];
                 if (x == 0) continue;
