@@ -217,7 +217,15 @@ mela_status mela_batch_create(mela_ctx *ctx, const mela_bitmat *mats, size_t cou
                               int flags, mela_batch **batch);
 /* per-matrix elimination mode: 0 = GE, 1 = M4RI 4-bit tables */
 mela_status mela_batch_set_modes(mela_batch *batch, const int *modes);
-/* times candidate mode assignments on the batch itself and keeps the fastest;
+/* per-matrix workgroups (1..8) that eliminate the matrix together: the
+ * tiles of each panel are dealt out among them, one device-scope barrier per
+ * panel.  Every workgroup of the launch is resident at once, so the group
+ * barriers only wait for running partners; on a device shared with other
+ * work a barrier that waits over 0.5 s fails the run (MELA_ERR_DEVICE from
+ * mela_batch_results).  Default 1. */
+mela_status mela_batch_set_groups(mela_batch *batch, const int *groups);
+/* times candidate tile widths, mode assignments and workgroups-per-matrix
+ * for the largest matrices on the batch itself and keeps the fastest;
  * *best_ms receives its time (may be NULL) */
 mela_status mela_batch_autotune(mela_batch *batch, float *best_ms);
 mela_status mela_batch_run(mela_batch *batch);

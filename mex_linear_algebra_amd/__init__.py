@@ -159,6 +159,8 @@ def load():
         L.mela_batch_create.restype = st
         L.mela_batch_set_modes.argtypes = [bp, C.POINTER(C.c_int)]
         L.mela_batch_set_modes.restype = st
+        L.mela_batch_set_groups.argtypes = [bp, C.POINTER(C.c_int)]
+        L.mela_batch_set_groups.restype = st
         L.mela_batch_autotune.argtypes = [bp, C.POINTER(C.c_float)]
         L.mela_batch_autotune.restype = st
         L.mela_batch_run.argtypes = [bp]
@@ -345,6 +347,11 @@ class Batch:
     def set_modes(self, modes):
         arr = (C.c_int * max(self.count, 1))(*[int(x) for x in modes])
         _check(self.lib.mela_batch_set_modes(self.handle, arr))
+
+    def set_groups(self, groups):
+        """Workgroups (1..8) that eliminate each matrix together."""
+        arr = (C.c_int * max(self.count, 1))(*[int(x) for x in groups])
+        _check(self.lib.mela_batch_set_groups(self.handle, arr))
 
     def autotune(self) -> float:
         ms = C.c_float()

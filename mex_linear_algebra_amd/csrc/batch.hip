@@ -32,8 +32,8 @@ struct BatchMat {
     uint64_t z_off;  // words: packed null basis (n x ldz) in the Z arena
     uint32_t m, n;
     uint32_t lda, ldz;
-    uint32_t mode;  // 0 = GE, 1 = M4RI (4-bit tables)
-    uint32_t pad;
+    uint32_t mode;    // 0 = GE, 1 = M4RI (4-bit tables)
+    uint32_t groups;  // workgroups eliminating the matrix together (1..MAX_GROUP)
 };
 
 namespace {
@@ -44,6 +44,10 @@ constexpr int MAX_M = 4096;
 constexpr int MAX_N = 8192;
 constexpr int MAX_WC = MAX_N / 64;
 constexpr int SW_PASSES = 16;    // row passes per wave with loads in flight together
+constexpr int MAX_GROUP = 8;     // workgroups per matrix at most
+// a group member that waits this long (wall clock ticks, 100 MHz: 0.5 s)
+// for its partners gives up and flags the batch as failed
+constexpr uint64_t GROUP_WAIT_TICKS = 50000000ull;
 
 template <int TW>
 struct BatchLDS {
@@ -71,6 +75,7 @@ struct BatchLDS {
     } u;
     int s_idx;
     int s_r, s_nact;
+    int s_ok;
 };
 
 template <int TW>
@@ -79,23 +84,73 @@ __device__ __forceinline__ int pivots_before(const BatchLDS<TW> &L, int p) {
     return L.pbefore[w] + dev::popc64(L.pmask[w] & ((1ull << (p & 63)) - 1ull));
 }
 
+// Barrier of the G workgroups that eliminate one matrix together.  *bar
+// counts arrivals (zeroed before the launch, never reset); the phase-th
+// barrier waits for G * phase of them.  Every wave releases its stores to
+// device scope before arriving (L2 write-back, the members may sit on
+// different XCDs) and acquires after the wait (L1/L2 invalidate).  A member
+// that waits longer than GROUP_WAIT_TICKS flags *err and returns false, and
+// once *err is set every group barrier returns false at once, so the launch
+// drains quickly and every wave ends; the host then reports the batch as
+// failed.  With G == 1 it is a workgroup barrier.
+__device__ __forceinline__ bool group_sync(unsigned *bar, int G, unsigned phase, int *err, int &s_ok) {
+    if (G == 1) {
+        __syncthreads();
+        return true;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned target = (unsigned)G * phase;
+        const uint64_t t0 = wall_clock64();
+        int ok = 1;
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(2);
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                ok = 0;
+                break;
+            }
+            if (wall_clock64() - t0 > GROUP_WAIT_TICKS) {
+                atomicOr(err, 1);
+                ok = 0;
+                break;
+            }
+        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return s_ok != 0;
+}
+
 template <int TW>
 __global__ void __launch_bounds__(BT)
-k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order, int count,
-             int *__restrict__ counter, const uint64_t *const *__restrict__ inputs,
+k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order,
+             const int2 *__restrict__ slots, int nslots, int *__restrict__ counter,
+             unsigned *__restrict__ gbar, const uint64_t *const *__restrict__ inputs,
              uint64_t *__restrict__ work, uint64_t *__restrict__ zarena, uint32_t *__restrict__ ranks,
              int want_null) {
     extern __shared__ __align__(16) unsigned char smem[];
     BatchLDS<TW> &L = *reinterpret_cast<BatchLDS<TW> *>(smem);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
+    // queue of slots: slot s = member slots[s].y of queue entry slots[s].x;
+    // a matrix's G slots are consecutive, so at most one matrix at a time is
+    // partly staffed and every member is taken by a running workgroup
+    int *err = counter + 1;
     for (;;) {
         if (tid == 0) L.s_idx = atomicAdd(counter, 1);
         __syncthreads();
-        const int qi = L.s_idx;
-        if (qi >= count) break;
+        const int si = L.s_idx;
+        if (si >= nslots) break;
+        const int2 sl = slots[si];
+        const int qi = sl.x, mk = sl.y;  // queue entry, member index
         const int mi = order[qi];
         const BatchMat d = mats[mi];
+        const int G = (int)d.groups;
+        unsigned *bar = gbar + qi;
+        unsigned phase = 0;
         const int m = (int)d.m, n = (int)d.n;
         const int wc = (n + 63) >> 6;
         const int rmax = m < n ? m : n;
@@ -104,9 +159,11 @@ k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order, i
         const uint64_t *A = inputs[mi];
 
         {
+            // member mk copies its share of the words
             constexpr int CU = 8;  // words per thread with loads in flight together
-            const int mw = m * wc;
-            for (int i0 = tid; i0 < mw; i0 += BT * CU) {
+            const int share = (m * wc + G - 1) / G;
+            const int mw = min(m * wc, (mk + 1) * share);
+            for (int i0 = mk * share + tid; i0 < mw; i0 += BT * CU) {
                 uint64_t v[CU];
 #pragma unroll
                 for (int k = 0; k < CU; ++k) {
@@ -125,8 +182,14 @@ k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order, i
             L.s_r = 0;
             L.s_nact = m;
         }
-        __syncthreads();
+        if (!group_sync(bar, G, ++phase, err, L.s_ok)) continue;
 
+        // Members run the same panel selection on the same words (identical
+        // results); the tiles of each panel are dealt out round robin; one
+        // group barrier per panel.  Member 0 owns tile 0 (word g), so it
+        // updates word g of the earlier pivot rows, after the barrier (when
+        // every member has read it).
+        bool dropped = false;
         for (int g = 0; g < wc; ++g) {
             const int n_act = L.s_nact, r0 = L.s_r;
             if (n_act == 0 || r0 >= rmax) break;
@@ -142,7 +205,7 @@ k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order, i
                 L.s_nact = po.n_act_new;
             }
             // the new pivot rows are zero left of their panel
-            for (int e = tid; e < rho * g; e += BT) {
+            for (int e = mk == 0 ? tid : rho * g; e < rho * g; e += BT) {
                 const int j = e / g, w = e - j * g;
                 Q[(size_t)(r0 + j) * wc + w] = 0ull;
             }
@@ -200,7 +263,7 @@ k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order, i
             }
             constexpr int RPP = 64 / TW;  // rows per wave pass, TW lanes (words) per row
             const int sub = lane / TW, wl = lane % TW;
-            for (int w0 = g; w0 < wc; w0 += TW) {
+            for (int w0 = g + mk * TW; w0 < wc; w0 += G * TW) {
                 // selected rows' words of the tile (loads issued together;
                 // slots past rho and words past wc read row 0 and store 0)
                 {
@@ -321,8 +384,12 @@ k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order, i
                 }
                 __syncthreads();
             }
-            // word g of the earlier pivot rows (after every tile read it)
-            if (want_null) {
+            if (!group_sync(bar, G, ++phase, err, L.s_ok)) {
+                dropped = true;
+                break;
+            }
+            // word g of the earlier pivot rows (every member has read it)
+            if (want_null && mk == 0) {
                 constexpr int PU = 4;  // rows per thread with loads in flight together
                 for (int q0 = tid; q0 < r0; q0 += BT * PU) {
                     uint64_t xv[PU];
@@ -344,8 +411,11 @@ k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order, i
             __syncthreads();
         }
 
+        if (dropped) continue;
+        // the last panel's pivot row words before any member extracts
+        if (!group_sync(bar, G, ++phase, err, L.s_ok)) continue;
         const int r = L.s_r;
-        if (tid == 0) ranks[mi] = (uint32_t)r;
+        if (tid == 0 && mk == 0) ranks[mi] = (uint32_t)r;
         if (want_null) {
             const int f = n - r;
             // prefix pivot counts per column word
@@ -378,7 +448,9 @@ k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order, i
             // a free column's row is its unit vector.  Four words per thread
             // per step, their row-word loads issued together.
             constexpr int NZ = 4;
-            for (int i0 = tid; i0 < n * ldz; i0 += BT * NZ) {
+            const int zshare = (n * ldz + G - 1) / G;
+            const int zend = min(n * ldz, (mk + 1) * zshare);
+            for (int i0 = mk * zshare + tid; i0 < zend; i0 += BT * NZ) {
                 const uint64_t *qrow[NZ];
                 int lo[NZ], wa[NZ], wb[NZ];
                 uint64_t word[NZ];
@@ -391,7 +463,7 @@ k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order, i
                     wa[k] = 0;
                     wb[k] = -1;
                     lo[k] = 64 * jw;
-                    if (i < n * ldz && jw < fw) {
+                    if (i < zend && jw < fw) {
                         const int pb = pivots_before(L, p);
                         if ((L.pmask[p >> 6] >> (p & 63)) & 1ull) {
                             qrow[k] = Q + (size_t)pb * wc;
@@ -429,7 +501,7 @@ k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order, i
                     if (nw > 2) put(wa[k] + 2, x2[k]);
                     for (int w = wa[k] + 3; w <= wb[k]; ++w) put(w, qrow[k][w]);
                     const int i = i0 + k * BT;
-                    if (i < n * ldz) Z[i] = word[k];
+                    if (i < zend) Z[i] = word[k];
                 }
             }
         }
@@ -437,7 +509,12 @@ k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order, i
     }
 }
 
-__global__ void k_reset_counter(int *c) { *c = 0; }
+// queue counter, error flag and the group barrier counters start at zero
+__global__ void k_reset_batch(int *counter, unsigned *gbar, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 2) counter[i] = 0;
+    if (i < count) gbar[i] = 0u;
+}
 
 // ones[i] += popcount of matrix i (one block per (matrix, row chunk))
 __global__ void __launch_bounds__(256)
@@ -470,18 +547,20 @@ struct mela_batch {
     std::vector<const uint64_t *> inputs;
     mela::BatchMat *d_mats = nullptr;
     int *d_order = nullptr;
-    int *d_counter = nullptr;
+    int *d_counter = nullptr;  // [0] queue slot counter, [1] group wait error flag
+    int2 *d_slots = nullptr;   // queue slots (entry, member), up to MAX_GROUP per matrix
+    unsigned *d_gbar = nullptr;  // group barrier counter per queue entry
+    int nslots = 0;
     const uint64_t **d_inputs = nullptr;
     uint64_t *d_work = nullptr;
     uint64_t *d_z = nullptr;
     uint32_t *d_ranks = nullptr;
     size_t work_words = 0, z_words = 0;
-    int grid = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int tuned = -1;  // index of the autotuned mode assignment
     int tw = 16;     // update tile width in words (16 or 32, autotuned)
-    int grid_tw[2] = {0, 0};  // persistent grid for tile widths 16 and 32
+    int occ_tw[2] = {1, 1};   // resident workgroups per CU for tile widths 16 and 32
     bool ctx_scratch = false;  // arenas borrowed from the context (not freed)
 };
 
@@ -493,18 +572,57 @@ using namespace mela;
 
 namespace mela {
 
+// Persistent grid: every workgroup resident at once (the group barriers wait
+// for partners that must be running), never more than the queue's slots.
+int batch_grid(const mela_batch *b) {
+    const int occ = b->occ_tw[b->tw == 32 ? 1 : 0];
+    return std::max(1, std::min(b->nslots, occ * b->ctx->cu_count));
+}
+
 mela_status batch_launch_raw(mela_batch *b, hipStream_t s) {
-    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, b->d_counter);
+    const int nb = (int)b->mats.size();
+    hipLaunchKernelGGL(k_reset_batch, dim3((unsigned)((std::max(nb, 2) + 255) / 256)), dim3(256), 0, s,
+                       b->d_counter, b->d_gbar, nb);
     const int want_null = (b->flags & MELA_BATCH_NULL) ? 1 : 0;
+    const int grid = batch_grid(b);
     if (b->tw == 32)
-        hipLaunchKernelGGL(k_batch_rref<32>, dim3(b->grid_tw[1]), dim3(BT), batch_lds_bytes(32), s,
-                           b->d_mats, b->d_order, (int)b->mats.size(), b->d_counter, b->d_inputs,
+        hipLaunchKernelGGL(k_batch_rref<32>, dim3(grid), dim3(BT), batch_lds_bytes(32), s, b->d_mats,
+                           b->d_order, b->d_slots, b->nslots, b->d_counter, b->d_gbar, b->d_inputs,
                            b->d_work, b->d_z, b->d_ranks, want_null);
     else
-        hipLaunchKernelGGL(k_batch_rref<16>, dim3(b->grid_tw[0]), dim3(BT), batch_lds_bytes(16), s,
-                           b->d_mats, b->d_order, (int)b->mats.size(), b->d_counter, b->d_inputs,
+        hipLaunchKernelGGL(k_batch_rref<16>, dim3(grid), dim3(BT), batch_lds_bytes(16), s, b->d_mats,
+                           b->d_order, b->d_slots, b->nslots, b->d_counter, b->d_gbar, b->d_inputs,
                            b->d_work, b->d_z, b->d_ranks, want_null);
     MELA_HIP(hipGetLastError());
+    return MELA_OK;
+}
+
+// Queue slots from the largest-first order and the per-matrix group sizes
+// (a matrix's slots are consecutive), uploaded with the descriptors.
+mela_status batch_upload_plan(mela_batch *b) {
+    std::vector<int2> sl;
+    sl.reserve(b->mats.size() * 2);
+    for (size_t qi = 0; qi < b->order.size(); ++qi) {
+        const int G = (int)b->mats[b->order[qi]].groups;
+        for (int k = 0; k < G; ++k) sl.push_back(make_int2((int)qi, k));
+    }
+    b->nslots = (int)sl.size();
+    hipStream_t s = b->ctx->stream;
+    if (!b->mats.empty()) {
+        MELA_HIP(hipMemcpyAsync(b->d_mats, b->mats.data(), b->mats.size() * sizeof(BatchMat),
+                                hipMemcpyHostToDevice, s));
+        MELA_HIP(hipMemcpyAsync(b->d_slots, sl.data(), sl.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+    }
+    MELA_HIP(hipStreamSynchronize(s));
+    // a captured graph holds the old grid size
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+    }
+    if (b->graph) {
+        (void)hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
     return MELA_OK;
 }
 
@@ -514,7 +632,8 @@ namespace {
 
 void free_batch_device(mela_batch *b) {
     if (!b->ctx_scratch) {
-        void *ptrs[] = {b->d_mats, b->d_order, b->d_counter, b->d_inputs, b->d_work, b->d_z, b->d_ranks};
+        void *ptrs[] = {b->d_mats,   b->d_order, b->d_counter, b->d_slots, b->d_gbar,
+                        b->d_inputs, b->d_work,  b->d_z,       b->d_ranks};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
     }
@@ -583,6 +702,7 @@ mela_status mela_batch_create(mela_ctx *ctx, const mela_bitmat *mats, size_t cou
         d.z_off = z;
         z += M.cols * wc;
         d.mode = (flags & MELA_BATCH_GE) ? 0 : 1;
+        d.groups = 1;
         b->mats.push_back(d);
         b->inputs.push_back(M.data);
     }
@@ -602,7 +722,9 @@ mela_status mela_batch_create(mela_ctx *ctx, const mela_bitmat *mats, size_t cou
         // one batch at a time on this context: borrow its cached arenas
         b->ctx_scratch = true;
         void *desc = nullptr, *wk = nullptr, *za = nullptr, *rk = nullptr;
-        const size_t desc_bytes = nb * (sizeof(BatchMat) + sizeof(int) + sizeof(void *)) + 64;
+        const size_t desc_bytes =
+            nb * (sizeof(BatchMat) + sizeof(void *) + sizeof(int) + MAX_GROUP * sizeof(int2) + sizeof(unsigned)) +
+            64;
         if ((st = scratch_get(ctx, SLOT_BATCH_DESC, desc_bytes, &desc)) ||
             (st = scratch_get(ctx, SLOT_BATCH_WORK, work * 8, &wk)) ||
             (st = scratch_get(ctx, SLOT_BATCH_Z, z * 8, &za)) ||
@@ -615,14 +737,19 @@ mela_status mela_batch_create(mela_ctx *ctx, const mela_bitmat *mats, size_t cou
         dp += nb * sizeof(BatchMat);
         b->d_inputs = (const uint64_t **)dp;
         dp += nb * sizeof(void *);
+        b->d_slots = (int2 *)dp;
+        dp += nb * MAX_GROUP * sizeof(int2);
         b->d_order = (int *)dp;
         dp += nb * sizeof(int);
+        b->d_gbar = (unsigned *)dp;
+        dp += nb * sizeof(unsigned);
         b->d_counter = (int *)(((uintptr_t)dp + 15) & ~(uintptr_t)15);
         b->d_work = (uint64_t *)wk;
         b->d_z = (uint64_t *)za;
         b->d_ranks = (uint32_t *)rk;
     } else if ((st = dalloc(&b->d_mats, nb)) || (st = dalloc(&b->d_order, nb)) ||
-               (st = dalloc(&b->d_counter, 1)) || (st = dalloc(&b->d_inputs, nb)) ||
+               (st = dalloc(&b->d_counter, 2)) || (st = dalloc(&b->d_inputs, nb)) ||
+               (st = dalloc(&b->d_slots, nb * MAX_GROUP)) || (st = dalloc(&b->d_gbar, nb)) ||
                (st = dalloc(&b->d_work, work)) || (st = dalloc(&b->d_z, z)) ||
                (st = dalloc(&b->d_ranks, nb))) {
         free_batch_device(b);
@@ -631,7 +758,6 @@ mela_status mela_batch_create(mela_ctx *ctx, const mela_bitmat *mats, size_t cou
     }
     hipStream_t s = ctx->stream;
     if (nb) {
-        MELA_HIP(hipMemcpyAsync(b->d_mats, b->mats.data(), nb * sizeof(BatchMat), hipMemcpyHostToDevice, s));
         MELA_HIP(hipMemcpyAsync(b->d_order, b->order.data(), nb * sizeof(int), hipMemcpyHostToDevice, s));
         MELA_HIP(hipMemcpyAsync(b->d_inputs, b->inputs.data(), nb * sizeof(void *), hipMemcpyHostToDevice, s));
     }
@@ -639,13 +765,24 @@ mela_status mela_batch_create(mela_ctx *ctx, const mela_bitmat *mats, size_t cou
         int occ = 0;
         const void *fn = v ? (const void *)k_batch_rref<32> : (const void *)k_batch_rref<16>;
         MELA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, BT, batch_lds_bytes(v ? 32 : 16)));
-        if (occ < 1) occ = 1;
-        b->grid_tw[v] = std::max(1, std::min<int>((int)nb, occ * ctx->cu_count));
+        b->occ_tw[v] = occ < 1 ? 1 : occ;
     }
-    b->grid = b->grid_tw[0];
-    MELA_HIP(hipStreamSynchronize(s));
+    if ((st = batch_upload_plan(b)) != MELA_OK) {
+        free_batch_device(b);
+        delete b;
+        return st;
+    }
     *out = b;
     return MELA_OK;
+}
+
+mela_status mela_batch_set_groups(mela_batch *b, const int *groups) {
+    if (!b || (b->count && !groups)) return set_error(MELA_ERR_INVALID, "null argument");
+    for (size_t i = 0; i < b->count; ++i)
+        if (groups[i] < 1 || groups[i] > MAX_GROUP)
+            return set_error(MELA_ERR_INVALID, "workgroups per matrix must be 1..8");
+    for (size_t i = 0; i < b->count; ++i) b->mats[i].groups = (uint32_t)groups[i];
+    return batch_upload_plan(b);
 }
 
 mela_status mela_batch_set_modes(mela_batch *b, const int *modes) {
@@ -698,47 +835,66 @@ mela_status mela_batch_autotune(mela_batch *b, float *best_ms) {
     hipEvent_t e0, e1;
     MELA_HIP(hipEventCreate(&e0));
     MELA_HIP(hipEventCreate(&e1));
-    // search: tile width (16 / 32 words) x per-matrix mode assignment
+    // time of the second of two runs (warm); a group wait time-out counts as
+    // a failed candidate
+    auto timed = [&](float *ms) -> mela_status {
+        mela_status st;
+        for (int rep = 0; rep < 2; ++rep) {
+            MELA_HIP(hipEventRecord(e0, s));
+            if ((st = batch_launch_raw(b, s)) != MELA_OK) return st;
+            MELA_HIP(hipEventRecord(e1, s));
+            MELA_HIP(hipEventSynchronize(e1));
+            MELA_HIP(hipEventElapsedTime(ms, e0, e1));
+        }
+        int qerr[2] = {0, 0};
+        MELA_HIP(hipMemcpy(qerr, b->d_counter, sizeof(qerr), hipMemcpyDeviceToHost));
+        if (qerr[1]) *ms = 1e30f;
+        return MELA_OK;
+    };
+    mela_status st;
+    std::vector<int> ones(nb, 1);
+    if ((st = mela_batch_set_groups(b, ones.data())) != MELA_OK) return st;
+    // search 1: tile width (16 / 32 words) x per-matrix mode assignment
     float best = 1e30f;
     size_t bi = 0;
     int best_tw = 16;
     for (int tw : {16, 32}) {
         b->tw = tw;
         for (size_t c = 0; c < cands.size(); ++c) {
-            mela_status st = mela_batch_set_modes(b, cands[c].data());
-            if (st != MELA_OK) return st;
-            float tot = 0.f;
-            for (int rep = 0; rep < 2; ++rep) {
-                MELA_HIP(hipEventRecord(e0, s));
-                if ((st = batch_launch_raw(b, s)) != MELA_OK) return st;
-                MELA_HIP(hipEventRecord(e1, s));
-                MELA_HIP(hipEventSynchronize(e1));
-                float ms = 0.f;
-                MELA_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (rep) tot = ms;  // second run (warm)
-            }
-            if (tot < best) {
-                best = tot;
+            if ((st = mela_batch_set_modes(b, cands[c].data())) != MELA_OK) return st;
+            float ms = 0.f;
+            if ((st = timed(&ms)) != MELA_OK) return st;
+            if (ms < best) {
+                best = ms;
                 bi = c;
                 best_tw = tw;
             }
         }
     }
+    b->tw = best_tw;
+    if ((st = mela_batch_set_modes(b, cands[bi].data())) != MELA_OK) return st;
+    b->tuned = (int)bi;
+    // search 2: workgroups per matrix for the head of the queue (the largest
+    // matrices, whose single-workgroup time bounds the launch)
+    std::vector<int> gbest = ones;
+    for (double frac : {0.005, 0.02, 0.06}) {
+        for (int G : {2, 4, 8}) {
+            std::vector<int> gr(nb, 1);
+            const size_t top = std::max<size_t>(1, (size_t)(frac * (double)nb));
+            for (size_t q = 0; q < top && q < nb; ++q) gr[b->order[q]] = G;
+            if ((st = mela_batch_set_groups(b, gr.data())) != MELA_OK) return st;
+            float ms = 0.f;
+            if ((st = timed(&ms)) != MELA_OK) return st;
+            if (ms < best) {
+                best = ms;
+                gbest = gr;
+            }
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    b->tw = best_tw;
-    mela_status st = mela_batch_set_modes(b, cands[bi].data());
-    if (st != MELA_OK) return st;
-    b->tuned = (int)bi;
-    // a graph captured before tuning launches the old tile width
-    if (b->exec) {
-        (void)hipGraphExecDestroy(b->exec);
-        b->exec = nullptr;
-    }
-    if (b->graph) {
-        (void)hipGraphDestroy(b->graph);
-        b->graph = nullptr;
-    }
+    // also drops a graph captured before tuning (old tile width and grid)
+    if ((st = mela_batch_set_groups(b, gbest.data())) != MELA_OK) return st;
     if (best_ms) *best_ms = best;
     return MELA_OK;
 }
@@ -780,9 +936,15 @@ mela_status mela_batch_results(mela_batch *b, uint32_t *ranks, mela_bitmat *null
     std::vector<uint32_t> r(b->mats.size());
     if (!r.empty()) {
         std::lock_guard<std::mutex> lk(ctx->mu);
+        int qerr[2] = {0, 0};
         MELA_HIP(hipMemcpyAsync(r.data(), b->d_ranks, r.size() * sizeof(uint32_t),
                                 hipMemcpyDeviceToHost, ctx->stream));
+        MELA_HIP(hipMemcpyAsync(qerr, b->d_counter, sizeof(qerr), hipMemcpyDeviceToHost, ctx->stream));
         MELA_HIP(hipStreamSynchronize(ctx->stream));
+        if (qerr[1])
+            return set_error(MELA_ERR_DEVICE,
+                             "a workgroup group of the batch timed out waiting for its partners "
+                             "(the device was shared; rerun with one workgroup per matrix)");
     }
     for (size_t i = 0; i < b->count; ++i) {
         if (ranks) ranks[i] = r[i];

@@ -156,7 +156,9 @@ __device__ PanelOut wg_panel_select(SelectLDS<NT> &L, ACT *act, int n_act, int g
     if (n_act >= 64) {
         if (tid < n0) L.cy[tid] = row((int)act[tid])[g];
         __syncthreads();
-        if (wave == 0) {
+        // the rank of the first 64 words is at most the number of nonzero
+        // ones: with fewer than FAST_MIN_PIVOTS the general path runs at once
+        if (wave == 0 && popc64(__ballot(L.cy[lane] != 0ull)) >= FAST_MIN_PIVOTS) {
             uint64_t v = L.cy[lane];
             uint64_t comb = 1ull << lane;
             bool piv;

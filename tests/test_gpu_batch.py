@@ -31,19 +31,24 @@ def _oracle_null(orc, P, m, n):
     return orc.rank(A, "gf2"), orc.null(A, "gf2")
 
 
-@pytest.mark.parametrize("flags_extra,modes", [(0, None), (4, None), (4, "ge"), (0, "mix")])
-def test_batch_matches_oracle(mela, ctx, orc, flags_extra, modes):
+@pytest.mark.parametrize("flags_extra,modes,groups", [(0, None, None), (4, None, None), (4, "ge", None),
+                                                      (0, "mix", None), (4, None, 4), (0, "mix", "mix")])
+def test_batch_matches_oracle(mela, ctx, orc, flags_extra, modes, groups):
     items = W.batch_c4(120, seed=7, mmin=8, mmax=384)
     items += [(W.dense_matrix(np.random.default_rng(3), 70, 64), 70, 64, "dense"),
               (np.zeros((5, 2), np.uint64), 5, 100, "zero"),
               (W.dense_matrix(np.random.default_rng(4), 1, 65), 1, 65, "dense")]
-    print(f"[batch] {len(items)} matrices flags+{flags_extra} modes={modes}", flush=True)
+    print(f"[batch] {len(items)} matrices flags+{flags_extra} modes={modes} groups={groups}", flush=True)
     dev = [ctx.upload(P, n) for P, m, n, _ in items]
     b = mela.Batch(ctx, dev, mela.BATCH_NULL | flags_extra)
     if modes == "ge":
         b.set_modes([0] * len(items))
     elif modes == "mix":
         b.set_modes([i % 2 for i in range(len(items))])
+    if groups == "mix":  # 1..8 workgroups per matrix
+        b.set_groups([1 + i % 8 for i in range(len(items))])
+    elif groups:
+        b.set_groups([groups] * len(items))
     for rep in range(2):  # graph replays must repeat the result
         b.run()
         ranks, nulls = b.results()
@@ -76,15 +81,19 @@ def test_batch_same_alone_as_in_batch(mela, ctx):
         ctx.free(M)
 
 
-def test_batch_full_c4_shapes_property(mela, ctx):
+@pytest.mark.parametrize("groups", [1, 3, 8])
+def test_batch_full_c4_shapes_property(mela, ctx, groups):
     # largest shapes of the uneven batch: 2048-row parity-check matrices;
-    # checked by A*Z = 0 on the device and nullity = n - rank
+    # checked by A*Z = 0 on the device and nullity = n - rank, with 1, 3 and
+    # 8 workgroups per matrix
+    print(f"[batch full shapes] groups={groups}", flush=True)
     rng = np.random.default_rng(5)
     items = [(W.parity_check_matrix(rng, 2048, 4096, 3), 2048, 4096),
              (W.dense_matrix(rng, 2048, 3000), 2048, 3000),
              (W.parity_check_matrix(rng, 1500, 2000, 6), 1500, 2000)]
     dev = [ctx.upload(P, n) for P, m, n in items]
     b = mela.Batch(ctx, dev, mela.BATCH_NULL)
+    b.set_groups([groups] * len(items))
     b.run()
     ranks, nulls = b.results()
     for i, (P, m, n) in enumerate(items):
@@ -94,6 +103,18 @@ def test_batch_full_c4_shapes_property(mela, ctx):
         ctx.matmul_packed(dev[i], nulls[i], Cm)
         assert not ctx.download(Cm).any()
         ctx.free(Cm)
+    b.close()
+    for M in dev:
+        ctx.free(M)
+
+
+def test_batch_groups_rejects_bad_counts(mela, ctx):
+    items = W.batch_c4(4, seed=2, mmin=16, mmax=64)
+    dev = [ctx.upload(P, n) for P, m, n, _ in items]
+    b = mela.Batch(ctx, dev, mela.BATCH_NULL)
+    for bad in (0, 9, -1):
+        with pytest.raises(mela.MelaError):
+            b.set_groups([bad] * len(items))
     b.close()
     for M in dev:
         ctx.free(M)
