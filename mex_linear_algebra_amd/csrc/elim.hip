@@ -132,23 +132,31 @@ struct RowBatch {
     uint64_t a[R];
 };
 
+// Positions past the end resolve to the row of position 0 (always valid when
+// the kernel runs) and get a zero multiplier, so every load is unconditional
+// and all of a batch's loads are in flight together (a load under a branch
+// is waited for before the branch closes).
 template <int R>
 __device__ __forceinline__ void batch_ptrs(RowBatch<R> &b, uint64_t *M, size_t ld,
                                            const uint32_t *act, uint64_t *Q, size_t ldq,
                                            int n_act, int total, int q0) {
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-        const int q = q0 + j;
-        b.rp[j] = q < total ? row_ptr(M, ld, act, Q, ldq, n_act, q) : nullptr;
+        const int q = q0 + j < total ? q0 + j : 0;
+        b.rp[j] = row_ptr(M, ld, act, Q, ldq, n_act, q);
     }
 }
 
 template <int R>
-__device__ __forceinline__ void batch_words(RowBatch<R> &b, int g, int w, bool wok, uint64_t pm) {
+__device__ __forceinline__ void batch_words(RowBatch<R> &b, int g, int w, bool wok, uint64_t pm,
+                                            int total, int q0) {
+    const int ws = wok ? w : g;
 #pragma unroll
-    for (int j = 0; j < R; ++j) b.x[j] = b.rp[j] ? (b.rp[j][g] & pm) : 0ull;
+    for (int j = 0; j < R; ++j) b.x[j] = b.rp[j][g];
 #pragma unroll
-    for (int j = 0; j < R; ++j) b.a[j] = (b.rp[j] && wok) ? b.rp[j][w] : 0ull;
+    for (int j = 0; j < R; ++j) b.a[j] = b.rp[j][ws];
+#pragma unroll
+    for (int j = 0; j < R; ++j) b.x[j] = (q0 + j < total && wok) ? b.x[j] & pm : 0ull;
 }
 
 // LDS-only workgroup barrier: row loads already in flight stay in flight.
@@ -165,6 +173,8 @@ __global__ void __launch_bounds__(UP_WAVES * 64)
 k_update_m4ri(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
               uint64_t *__restrict__ Q, size_t ldq, const ElimMeta *__restrict__ meta, int g,
               int wc, int rref) {
+    // T[entry][table][word]: the table index lands in the ds_read immediate
+    // offset (t * 512 bytes), the entry index is the only per-lookup address
     __shared__ uint64_t T[16][16][UP_TILE];
     const uint64_t pm = meta->pm;
     if (pm == 0) return;
@@ -190,32 +200,35 @@ k_update_m4ri(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ 
         }
     RowBatch<UP_ROWS_M4RI> rb;
     batch_ptrs(rb, M, ld, act, Q, ldq, n_act, total, q0);
-    batch_words(rb, g, w, wok, pm);
+    batch_words(rb, g, w, wok, pm, total, q0);
 #pragma unroll
     for (int tt = 0; tt < 16 / UP_WAVES; ++tt) {
         const int t = wave * (16 / UP_WAVES) + tt;
         uint64_t cur = 0;
-        T[t][0][lane] = 0;
+        T[0][t][lane] = 0;
 #pragma unroll
         for (int v = 1; v < 16; ++v) {
             cur ^= pq[tt][__builtin_ctz(v)];
-            T[t][v ^ (v >> 1)][lane] = cur;
+            T[v ^ (v >> 1)][t][lane] = cur;
         }
     }
     lds_barrier();
     for (; q0 < total; q0 += stride) {
+        // rows whose multiplier is zero on the whole wave are skipped (a
+        // wave-uniform branch); stores come after all lookups, so no row
+        // waits for another row's store
 #pragma unroll
         for (int j = 0; j < UP_ROWS_M4RI; ++j) {
-            const uint64_t x = rb.x[j];
-            if (x == 0 || !wok) continue;
-            uint64_t a = rb.a[j];
+            if (__ballot(rb.x[j] != 0ull) == 0ull) continue;
 #pragma unroll
-            for (int t = 0; t < 16; ++t) a ^= T[t][(x >> (4 * t)) & 15u][lane];
-            rb.rp[j][w] = a;
+            for (int t = 0; t < 16; ++t) rb.a[j] ^= T[(rb.x[j] >> (4 * t)) & 15u][t][lane];
         }
+#pragma unroll
+        for (int j = 0; j < UP_ROWS_M4RI; ++j)
+            if (rb.x[j]) rb.rp[j][w] = rb.a[j];
         if (q0 + stride < total) {
             batch_ptrs(rb, M, ld, act, Q, ldq, n_act, total, q0 + stride);
-            batch_words(rb, g, w, wok, pm);
+            batch_words(rb, g, w, wok, pm, total, q0 + stride);
         }
     }
 }
@@ -247,7 +260,7 @@ k_update_ge(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ ac
     }
     RowBatch<UP_ROWS_GE> rb;
     batch_ptrs(rb, M, ld, act, Q, ldq, n_act, total, q0);
-    batch_words(rb, g, w, wok, pm);
+    batch_words(rb, g, w, wok, pm, total, q0);
 #pragma unroll
     for (int k = 0; k < 64 / UP_WAVES; ++k) P[wave + k * UP_WAVES][lane] = pv[k];
     lds_barrier();
@@ -255,17 +268,21 @@ k_update_ge(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ ac
 #pragma unroll
         for (int j = 0; j < UP_ROWS_GE; ++j) {
             uint64_t x = rb.x[j];
-            if (x == 0 || !wok) continue;
             uint64_t a = rb.a[j];
+            const bool upd = x != 0;
             while (x) {
                 a ^= P[ffs64(x)][lane];
                 x &= x - 1;
             }
-            rb.rp[j][w] = a;
+            rb.a[j] = a;
+            rb.x[j] = upd;
         }
+#pragma unroll
+        for (int j = 0; j < UP_ROWS_GE; ++j)
+            if (rb.x[j]) rb.rp[j][w] = rb.a[j];
         if (q0 + stride < total) {
             batch_ptrs(rb, M, ld, act, Q, ldq, n_act, total, q0 + stride);
-            batch_words(rb, g, w, wok, pm);
+            batch_words(rb, g, w, wok, pm, total, q0 + stride);
         }
     }
 }
@@ -418,9 +435,12 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
             const size_t rem = wc - g - 1;
             if (rem > 0) {
                 const unsigned tx = (unsigned)((rem + UP_TILE - 1) / UP_TILE);
-                // rows are strided over gridDim.y; size it to fill the chip ~2x
+                // rows are strided over gridDim.y: M4RI one round of one
+                // workgroup per CU (its tables take 128 KiB of LDS), GE two
+                // rounds (32 KiB of pivot rows, several workgroups per CU)
                 const size_t rows_hint = m + (rref ? rmax : 0);
-                unsigned ty = (unsigned)((2 * cu + tx - 1) / tx);
+                const unsigned per_cu = variant == MELA_ELIM_GE ? 2u : 1u;
+                unsigned ty = (unsigned)((per_cu * cu + tx - 1) / tx);
                 const unsigned per = UP_WAVES * (variant == MELA_ELIM_GE ? UP_ROWS_GE : UP_ROWS_M4RI);
                 const unsigned ty_max = (unsigned)((rows_hint + per - 1) / per);
                 if (ty > ty_max) ty = ty_max;
