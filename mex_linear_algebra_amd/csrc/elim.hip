@@ -15,13 +15,15 @@
 //                   ballot/readlane), removes them from the active list and
 //                   computes the RREF basis of the panel plus, for each basis
 //                   vector c, the set comb[c] of selected rows that XOR to it;
-//   k_build_pivots  forms the new pivot rows Q (words g .. wc-1): the selected
-//                   rows of a 64-word tile are staged in LDS and each pivot row
-//                   XORs the ones named by comb[c];
+//   k_build_pivots  (GE) forms the new pivot rows Q (words g .. wc-1): the
+//                   selected rows of a 64-word tile are staged in LDS and each
+//                   pivot row XORs the ones named by comb[c];
 //   k_update        clears the panel's pivot columns from every active row
 //                   (and, for RREF, from every earlier pivot row) with one
 //                   table XOR per 4 pivot columns (M4RI) or one XOR per pivot
 //                   row (GE), sweeping words g+1 .. wc-1 -- the HBM-bound part;
+//                   the M4RI kernel forms its tile's pivot rows itself from
+//                   4-bit tables of the selected rows (and writes them to Q);
 //   k_update_panelword (RREF) word g of the earlier pivot rows.
 #include "elim_device.h"
 
@@ -165,10 +167,15 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_s_barrier();
 }
 
-// M4RI form: 16 nibble tables of the panel's pivot rows (slot c = pivot bit c,
-// zero where c is not a pivot) in LDS; each row XORs 16 table rows selected
-// by the 16 nibbles of its panel word.  Words g+1 .. wc-1 only (word g is
-// handled by k_update_panelword so that every tile reads an unmodified x).
+// M4RI form, pivot rows included: the workgroup's 64-word tile starts at
+// word g.  Thread (t, lane) loads the selected rows 4t..4t+3 of its word and
+// builds their 4-bit table t; pivot row c = XOR of the selected rows in
+// comb[c] is then 16 lookups, and the four pivot rows 4t..4t+3 it holds give
+// the sweep's table t (slot c = pivot bit c, zero where c is not a pivot).
+// Row-stripe 0 of each tile writes the tile's pivot-row words to Q.  Each
+// active (and for RREF earlier pivot) row XORs 16 table entries selected by
+// the nibbles of its panel word into words g+1 .. wc-1; word g is left to
+// k_update_panelword so that every tile reads an unmodified multiplier.
 __global__ void __launch_bounds__(UP_WAVES * 64)
 k_update_m4ri(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
               uint64_t *__restrict__ Q, size_t ldq, const ElimMeta *__restrict__ meta, int g,
@@ -178,40 +185,70 @@ k_update_m4ri(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ 
     __shared__ uint64_t T[16][16][UP_TILE];
     const uint64_t pm = meta->pm;
     if (pm == 0) return;
-    const int n_act = meta->n_act, r0 = meta->r0;
+    const int n_act = meta->n_act, r0 = meta->r0, rho = meta->rho;
     const int total = n_act + (rref ? r0 : 0);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int w = g + 1 + blockIdx.x * UP_TILE + lane;
-    const bool wok = w < wc;
+    const int w = g + blockIdx.x * UP_TILE + lane;
+    const bool win = w < wc;            // a word of the matrix
+    const bool wok = win && w > g;      // a word the sweep updates
+    const int wl = win ? w : g;         // loads of words past the end re-read word g
     const int stride = gridDim.y * UP_WAVES * UP_ROWS_M4RI;
     int q0 = (blockIdx.y * UP_WAVES + wave) * UP_ROWS_M4RI;
-    if ((int)blockIdx.y * UP_WAVES * UP_ROWS_M4RI >= total) return;
-    // table inputs first, then the first row batch, whose word loads stay in
-    // flight while the tables are built
-    uint64_t pq[16 / UP_WAVES][4];
+    const bool rows_here = (int)blockIdx.y * UP_WAVES * UP_ROWS_M4RI < total;
+    if (!rows_here && blockIdx.y != 0) return;
+    constexpr int TPW = 16 / UP_WAVES;  // tables per wave
+    // selected rows' words first, then the first row batch, whose loads stay
+    // in flight while the tables are built
+    uint64_t sq[TPW][4];
 #pragma unroll
-    for (int tt = 0; tt < 16 / UP_WAVES; ++tt)
+    for (int tt = 0; tt < TPW; ++tt)
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const int c = 4 * (wave * (16 / UP_WAVES) + tt) + b;
-            pq[tt][b] = (wok && ((pm >> c) & 1ull))
-                            ? Q[(size_t)(r0 + popc64(pm & ((1ull << c) - 1ull))) * ldq + w]
-                            : 0ull;
+            const int slot = 4 * (wave * TPW + tt) + b;
+            sq[tt][b] = M[(size_t)meta->sel_row[slot < rho ? slot : 0] * ld + wl];
         }
     RowBatch<UP_ROWS_M4RI> rb;
-    batch_ptrs(rb, M, ld, act, Q, ldq, n_act, total, q0);
-    batch_words(rb, g, w, wok, pm, total, q0);
-#pragma unroll
-    for (int tt = 0; tt < 16 / UP_WAVES; ++tt) {
-        const int t = wave * (16 / UP_WAVES) + tt;
-        uint64_t cur = 0;
-        T[0][t][lane] = 0;
-#pragma unroll
-        for (int v = 1; v < 16; ++v) {
-            cur ^= pq[tt][__builtin_ctz(v)];
-            T[v ^ (v >> 1)][t][lane] = cur;
-        }
+    if (rows_here) {
+        batch_ptrs(rb, M, ld, act, Q, ldq, n_act, total, q0);
+        batch_words(rb, g, w, wok, pm, total, q0);
     }
+    auto build = [&](const uint64_t (&v)[TPW][4]) {
+#pragma unroll
+        for (int tt = 0; tt < TPW; ++tt) {
+            const int t = wave * TPW + tt;
+            uint64_t cur = 0;
+            T[0][t][lane] = 0;
+#pragma unroll
+            for (int e = 1; e < 16; ++e) {
+                cur ^= v[tt][__builtin_ctz(e)];
+                T[e ^ (e >> 1)][t][lane] = cur;  // Gray-code order
+            }
+        }
+    };
+#pragma unroll
+    for (int tt = 0; tt < TPW; ++tt)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            if (4 * (wave * TPW + tt) + b >= rho || !win) sq[tt][b] = 0ull;
+    build(sq);  // tables of the selected rows
+    lds_barrier();
+    uint64_t pv[TPW][4];
+#pragma unroll
+    for (int tt = 0; tt < TPW; ++tt)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int c = 4 * (wave * TPW + tt) + b;
+            const uint64_t cb = ((pm >> c) & 1ull) ? meta->comb[c] : 0ull;
+            uint64_t v = 0;
+#pragma unroll
+            for (int u = 0; u < 16; ++u) v ^= T[(cb >> (4 * u)) & 15u][u][lane];
+            pv[tt][b] = v;
+            if (blockIdx.y == 0 && win && ((pm >> c) & 1ull))
+                Q[(size_t)(r0 + popc64(pm & ((1ull << c) - 1ull))) * ldq + w] = v;
+        }
+    lds_barrier();
+    if (!rows_here) return;
+    build(pv);  // tables of the pivot rows
     lds_barrier();
     for (; q0 < total; q0 += stride) {
         // rows whose multiplier is zero on the whole wave are skipped (a
@@ -430,9 +467,12 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
         for (size_t g = 0; g < wc; ++g) {
             hipLaunchKernelGGL(k_panel_select, dim3(1), dim3(PS_THREADS), 0, s, Mw, ld, act, meta,
                                pivcols, (int)g, (int)rmax);
-            hipLaunchKernelGGL(k_build_pivots, dim3((unsigned)((wc - g + 63) / 64), 64 / BP_PIV),
-                               dim3(512), 0, s, Mw, ld, Q, ld, meta, (int)g, (int)wc);
-            const size_t rem = wc - g - 1;
+            // GE builds the pivot rows in a kernel of their own; the M4RI
+            // sweep forms them from the selected rows' tables itself
+            if (variant == MELA_ELIM_GE)
+                hipLaunchKernelGGL(k_build_pivots, dim3((unsigned)((wc - g + 63) / 64), 64 / BP_PIV),
+                                   dim3(512), 0, s, Mw, ld, Q, ld, meta, (int)g, (int)wc);
+            const size_t rem = variant == MELA_ELIM_GE ? wc - g - 1 : wc - g;  // words per grid
             if (rem > 0) {
                 const unsigned tx = (unsigned)((rem + UP_TILE - 1) / UP_TILE);
                 // rows are strided over gridDim.y: M4RI one round of one
