@@ -123,6 +123,7 @@ constexpr int UP_TILE = 64;   // words per workgroup (one per lane)
 constexpr int UP_WAVES = 8;   // 512 threads
 constexpr int UP_ROWS_M4RI = 16;  // rows in flight per wave (M4RI sweep)
 constexpr int UP_ROWS_GE = 4;     // rows in flight per wave (GE sweep: fewer registers)
+constexpr int M4RI_TILE = 32;     // words per M4RI sweep workgroup (32 or 64)
 
 // R rows of one wave: row pointers, panel words x and this lane's word w.
 // The pointers are resolved first (one active-list load each); the word loads
@@ -167,85 +168,111 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_s_barrier();
 }
 
-// M4RI form, pivot rows included: the workgroup's 64-word tile starts at
-// word g.  Thread (t, lane) loads the selected rows 4t..4t+3 of its word and
-// builds their 4-bit table t; pivot row c = XOR of the selected rows in
-// comb[c] is then 16 lookups, and the four pivot rows 4t..4t+3 it holds give
-// the sweep's table t (slot c = pivot bit c, zero where c is not a pivot).
-// Row-stripe 0 of each tile writes the tile's pivot-row words to Q.  Each
-// active (and for RREF earlier pivot) row XORs 16 table entries selected by
-// the nibbles of its panel word into words g+1 .. wc-1; word g is left to
+// M4RI form, pivot rows included: the workgroup's TILE-word tile starts at
+// word g (TILE 64: 128 KiB of tables, one workgroup per CU; TILE 32: two).
+// Each thread owns TILE/32 (table t, word) pairs: it loads the selected rows
+// 4t..4t+3 of its word and builds their 4-bit table t; pivot row c = XOR of
+// the selected rows in comb[c] is then 16 lookups, and the four pivot rows
+// 4t..4t+3 it holds give the sweep's table t (slot c = pivot bit c, zero
+// where c is not a pivot).  Row-stripe 0 of each tile writes the tile's
+// pivot-row words to Q.  Each active (and for RREF earlier pivot) row XORs 16
+// table entries selected by the nibbles of its panel word into words
+// g+1 .. wc-1 (64 / TILE rows per wave instruction); word g is left to
 // k_update_panelword so that every tile reads an unmodified multiplier.
-__global__ void __launch_bounds__(UP_WAVES * 64)
+template <int TILE>
+__global__ void __launch_bounds__(UP_WAVES * 64, 64 / TILE)
 k_update_m4ri(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
               uint64_t *__restrict__ Q, size_t ldq, const ElimMeta *__restrict__ meta, int g,
               int wc, int rref) {
     // T[entry][table][word]: the table index lands in the ds_read immediate
-    // offset (t * 512 bytes), the entry index is the only per-lookup address
-    __shared__ uint64_t T[16][16][UP_TILE];
+    // offset, the entry index is the only per-lookup address
+    __shared__ uint64_t T[16][16][TILE];
+    constexpr int NT = UP_WAVES * 64;
+    constexpr int RPW = 64 / TILE;          // rows per wave instruction
+    constexpr int R = UP_ROWS_M4RI / RPW;   // row batch per lane
+    constexpr int TPT = 16 * TILE / NT;     // (table, word) pairs per thread
     const uint64_t pm = meta->pm;
     if (pm == 0) return;
     const int n_act = meta->n_act, r0 = meta->r0, rho = meta->rho;
     const int total = n_act + (rref ? r0 : 0);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int w = g + blockIdx.x * UP_TILE + lane;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wl = lane % TILE, sub = lane / TILE;
+    const int w = g + blockIdx.x * TILE + wl;
     const bool win = w < wc;            // a word of the matrix
     const bool wok = win && w > g;      // a word the sweep updates
-    const int wl = win ? w : g;         // loads of words past the end re-read word g
-    const int stride = gridDim.y * UP_WAVES * UP_ROWS_M4RI;
-    int q0 = (blockIdx.y * UP_WAVES + wave) * UP_ROWS_M4RI;
-    const bool rows_here = (int)blockIdx.y * UP_WAVES * UP_ROWS_M4RI < total;
+    const int wsafe = win ? w : g;      // loads of words past the end re-read word g
+    const int per_wave = R * RPW;
+    const int stride = gridDim.y * UP_WAVES * per_wave;
+    int q0 = (blockIdx.y * UP_WAVES + wave) * per_wave + sub;
+    const bool rows_here = (int)blockIdx.y * UP_WAVES * per_wave < total;
     if (!rows_here && blockIdx.y != 0) return;
-    constexpr int TPW = 16 / UP_WAVES;  // tables per wave
+    // (table, word) pairs of this thread: table e / TILE, word e % TILE
+    auto pair_t = [&](int k) { return (tid + k * NT) / TILE; };
+    auto pair_w = [&](int k) { return g + blockIdx.x * TILE + (tid + k * NT) % TILE; };
     // selected rows' words first, then the first row batch, whose loads stay
     // in flight while the tables are built
-    uint64_t sq[TPW][4];
+    uint64_t sq[TPT][4];
 #pragma unroll
-    for (int tt = 0; tt < TPW; ++tt)
+    for (int k = 0; k < TPT; ++k) {
+        const int pw = pair_w(k) < wc ? pair_w(k) : g;
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const int slot = 4 * (wave * TPW + tt) + b;
-            sq[tt][b] = M[(size_t)meta->sel_row[slot < rho ? slot : 0] * ld + wl];
+            const int slot = 4 * pair_t(k) + b;
+            sq[k][b] = M[(size_t)meta->sel_row[slot < rho ? slot : 0] * ld + pw];
         }
-    RowBatch<UP_ROWS_M4RI> rb;
-    if (rows_here) {
-        batch_ptrs(rb, M, ld, act, Q, ldq, n_act, total, q0);
-        batch_words(rb, g, w, wok, pm, total, q0);
     }
-    auto build = [&](const uint64_t (&v)[TPW][4]) {
+    uint64_t *rp[R];
+    uint64_t x[R], a[R];
+    auto load_rows = [&](int qb) {
 #pragma unroll
-        for (int tt = 0; tt < TPW; ++tt) {
-            const int t = wave * TPW + tt;
+        for (int j = 0; j < R; ++j) {
+            const int q = qb + j * RPW;
+            rp[j] = row_ptr(M, ld, act, Q, ldq, n_act, q < total ? q : 0);
+        }
+#pragma unroll
+        for (int j = 0; j < R; ++j) x[j] = rp[j][g];
+#pragma unroll
+        for (int j = 0; j < R; ++j) a[j] = rp[j][wsafe];
+#pragma unroll
+        for (int j = 0; j < R; ++j) x[j] = (qb + j * RPW < total && wok) ? x[j] & pm : 0ull;
+    };
+    if (rows_here) load_rows(q0);
+    auto build = [&](const uint64_t (&v)[TPT][4]) {
+#pragma unroll
+        for (int k = 0; k < TPT; ++k) {
+            const int t = pair_t(k), ww = (tid + k * NT) % TILE;
             uint64_t cur = 0;
-            T[0][t][lane] = 0;
+            T[0][t][ww] = 0;
 #pragma unroll
             for (int e = 1; e < 16; ++e) {
-                cur ^= v[tt][__builtin_ctz(e)];
-                T[e ^ (e >> 1)][t][lane] = cur;  // Gray-code order
+                cur ^= v[k][__builtin_ctz(e)];
+                T[e ^ (e >> 1)][t][ww] = cur;  // Gray-code order
             }
         }
     };
 #pragma unroll
-    for (int tt = 0; tt < TPW; ++tt)
+    for (int k = 0; k < TPT; ++k)
 #pragma unroll
         for (int b = 0; b < 4; ++b)
-            if (4 * (wave * TPW + tt) + b >= rho || !win) sq[tt][b] = 0ull;
+            if (4 * pair_t(k) + b >= rho || pair_w(k) >= wc) sq[k][b] = 0ull;
     build(sq);  // tables of the selected rows
     lds_barrier();
-    uint64_t pv[TPW][4];
+    uint64_t pv[TPT][4];
 #pragma unroll
-    for (int tt = 0; tt < TPW; ++tt)
+    for (int k = 0; k < TPT; ++k) {
+        const int ww = (tid + k * NT) % TILE;
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const int c = 4 * (wave * TPW + tt) + b;
+            const int c = 4 * pair_t(k) + b;
             const uint64_t cb = ((pm >> c) & 1ull) ? meta->comb[c] : 0ull;
             uint64_t v = 0;
 #pragma unroll
-            for (int u = 0; u < 16; ++u) v ^= T[(cb >> (4 * u)) & 15u][u][lane];
-            pv[tt][b] = v;
-            if (blockIdx.y == 0 && win && ((pm >> c) & 1ull))
-                Q[(size_t)(r0 + popc64(pm & ((1ull << c) - 1ull))) * ldq + w] = v;
+            for (int u = 0; u < 16; ++u) v ^= T[(cb >> (4 * u)) & 15u][u][ww];
+            pv[k][b] = v;
+            if (blockIdx.y == 0 && pair_w(k) < wc && ((pm >> c) & 1ull))
+                Q[(size_t)(r0 + popc64(pm & ((1ull << c) - 1ull))) * ldq + pair_w(k)] = v;
         }
+    }
     lds_barrier();
     if (!rows_here) return;
     build(pv);  // tables of the pivot rows
@@ -255,18 +282,15 @@ k_update_m4ri(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ 
         // wave-uniform branch); stores come after all lookups, so no row
         // waits for another row's store
 #pragma unroll
-        for (int j = 0; j < UP_ROWS_M4RI; ++j) {
-            if (__ballot(rb.x[j] != 0ull) == 0ull) continue;
+        for (int j = 0; j < R; ++j) {
+            if (__ballot(x[j] != 0ull) == 0ull) continue;
 #pragma unroll
-            for (int t = 0; t < 16; ++t) rb.a[j] ^= T[(rb.x[j] >> (4 * t)) & 15u][t][lane];
+            for (int t = 0; t < 16; ++t) a[j] ^= T[(x[j] >> (4 * t)) & 15u][t][wl];
         }
 #pragma unroll
-        for (int j = 0; j < UP_ROWS_M4RI; ++j)
-            if (rb.x[j]) rb.rp[j][w] = rb.a[j];
-        if (q0 + stride < total) {
-            batch_ptrs(rb, M, ld, act, Q, ldq, n_act, total, q0 + stride);
-            batch_words(rb, g, w, wok, pm, total, q0 + stride);
-        }
+        for (int j = 0; j < R; ++j)
+            if (x[j]) rp[j][w] = a[j];
+        if (q0 + stride < total) load_rows(q0 + stride);
     }
 }
 
@@ -474,12 +498,13 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
                                    dim3(512), 0, s, Mw, ld, Q, ld, meta, (int)g, (int)wc);
             const size_t rem = variant == MELA_ELIM_GE ? wc - g - 1 : wc - g;  // words per grid
             if (rem > 0) {
-                const unsigned tx = (unsigned)((rem + UP_TILE - 1) / UP_TILE);
-                // rows are strided over gridDim.y: M4RI one round of one
-                // workgroup per CU (its tables take 128 KiB of LDS), GE two
-                // rounds (32 KiB of pivot rows, several workgroups per CU)
+                const size_t tile = variant == MELA_ELIM_GE ? UP_TILE : M4RI_TILE;
+                const unsigned tx = (unsigned)((rem + tile - 1) / tile);
+                // rows are strided over gridDim.y: M4RI one round of the
+                // resident workgroups (64 / M4RI_TILE per CU), GE two rounds
+                // (32 KiB of pivot rows, several workgroups per CU)
                 const size_t rows_hint = m + (rref ? rmax : 0);
-                const unsigned per_cu = variant == MELA_ELIM_GE ? 2u : 1u;
+                const unsigned per_cu = variant == MELA_ELIM_GE ? 2u : (unsigned)(64 / M4RI_TILE);
                 unsigned ty = (unsigned)((per_cu * cu + tx - 1) / tx);
                 const unsigned per = UP_WAVES * (variant == MELA_ELIM_GE ? UP_ROWS_GE : UP_ROWS_M4RI);
                 const unsigned ty_max = (unsigned)((rows_hint + per - 1) / per);
@@ -489,7 +514,7 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
                     hipLaunchKernelGGL(k_update_ge, dim3(tx, ty), dim3(UP_WAVES * 64), 0, s, Mw, ld,
                                        act, Q, ld, meta, (int)g, (int)wc, rref ? 1 : 0);
                 else
-                    hipLaunchKernelGGL(k_update_m4ri, dim3(tx, ty), dim3(UP_WAVES * 64), 0, s, Mw,
+                    hipLaunchKernelGGL(k_update_m4ri<M4RI_TILE>, dim3(tx, ty), dim3(UP_WAVES * 64), 0, s, Mw,
                                        ld, act, Q, ld, meta, (int)g, (int)wc, rref ? 1 : 0);
             }
             if (rref)
