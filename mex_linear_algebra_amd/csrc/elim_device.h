@@ -93,14 +93,22 @@ __host__ __device__ __forceinline__ uint64_t compress_apply(uint64_t x, uint64_t
     return x;
 }
 
-// XOR of v over the 64 lanes of the wave (every lane receives the result).
+// XOR of v over the 64 lanes of the wave (every lane receives the result;
+// all 64 lanes must be active).  DPP row shifts give the inclusive XOR of
+// each 16-lane row, two row broadcasts carry rows 0-2 into lane 63, whose
+// value is read out: six VALU steps per half instead of six LDS permutes.
+__device__ __forceinline__ uint32_t wave_xor32(uint32_t v) {
+    v ^= __builtin_amdgcn_update_dpp(0u, v, 0x111, 0xF, 0xF, true);   // row_shr:1
+    v ^= __builtin_amdgcn_update_dpp(0u, v, 0x112, 0xF, 0xF, true);   // row_shr:2
+    v ^= __builtin_amdgcn_update_dpp(0u, v, 0x114, 0xF, 0xF, true);   // row_shr:4
+    v ^= __builtin_amdgcn_update_dpp(0u, v, 0x118, 0xF, 0xF, true);   // row_shr:8
+    v ^= __builtin_amdgcn_update_dpp(0u, v, 0x142, 0xA, 0xF, false);  // row_bcast:15 -> rows 1, 3
+    v ^= __builtin_amdgcn_update_dpp(0u, v, 0x143, 0xC, 0xF, false);  // row_bcast:31 -> rows 2, 3
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
 __device__ __forceinline__ uint64_t wave_xor64(uint64_t v) {
-    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
-#pragma unroll
-    for (int s = 1; s < 64; s <<= 1) {
-        lo ^= __shfl_xor(lo, s, 64);
-        hi ^= __shfl_xor(hi, s, 64);
-    }
+    const uint32_t lo = wave_xor32((uint32_t)v), hi = wave_xor32((uint32_t)(v >> 32));
     return ((uint64_t)hi << 32) | lo;
 }
 
