@@ -457,9 +457,8 @@ mela_status mela_null_packed(mela_ctx *ctx, const mela_bitmat *A, int variant, m
         if ((st = scratch_get(ctx, SLOT_ELIM_Q, 8, &dummy)) != MELA_OK) return st;
         Q = (uint64_t *)dummy;
     }
-    st = launch_null_packed(ctx->stream, Q, ldq, d_rowmap, d_free, n, f, Z->data, Z->ld);
+    st = launch_null_packed(ctx, Q, ldq, d_rowmap, piv, n, f, Z->data, Z->ld);
     if (st != MELA_OK) return st;
-    MELA_HIP(hipStreamSynchronize(ctx->stream));
     return MELA_OK;
 }
 

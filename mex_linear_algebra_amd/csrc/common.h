@@ -94,7 +94,7 @@ enum ScratchSlot {
     SLOT_A = 0, SLOT_B, SLOT_C, SLOT_BT, SLOT_IN_A, SLOT_IN_B, SLOT_OUT,
     SLOT_ELIM_M, SLOT_ELIM_Q, SLOT_ELIM_META, SLOT_ELIM_ACT, SLOT_ELIM_TAB,
     SLOT_NULL_Z, SLOT_XA, SLOT_XB,
-    SLOT_BATCH_DESC, SLOT_BATCH_WORK, SLOT_BATCH_Z, SLOT_BATCH_RANK, SLOT_COUNT
+    SLOT_BATCH_DESC, SLOT_BATCH_WORK, SLOT_BATCH_Z, SLOT_BATCH_RANK, SLOT_NULL_TAB, SLOT_COUNT
 };
 
 // ------------------------------------------------- kernel launchers ----
@@ -137,8 +137,8 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m,
 mela_status launch_null_logical(hipStream_t s, const uint64_t *Q, size_t ldq,
                                 const int *rowmap_dev, const int *free_dev,
                                 size_t n, size_t f, uint8_t *Z);
-mela_status launch_null_packed(hipStream_t s, const uint64_t *Q, size_t ldq,
-                               const int *rowmap_dev, const int *free_dev,
-                               size_t n, size_t f, uint64_t *Z, size_t ldz);
+mela_status launch_null_packed(mela_ctx *ctx, const uint64_t *Q, size_t ldq, const int *rowmap,
+                               const std::vector<int> &piv, size_t n, size_t f, uint64_t *Z,
+                               size_t ldz);
 
 }  // namespace mela

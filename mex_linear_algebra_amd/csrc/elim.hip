@@ -29,12 +29,23 @@
 
 namespace mela {
 
+// Column word of the null-basis gather (k_null_packed).
+struct NullWord {
+    uint64_t fm;     // free-column mask of the word
+    uint64_t mv[6];  // its compress masks
+    long fb;         // free columns before the word
+    int ws;          // (entry j) the column word holding free column 64 j
+    int pad;
+};
+
 struct ElimMeta {
     int n_act;  // active (not yet pivot) rows
     int r;      // pivot rows found so far
     int r0;     // pivot rows before the current panel
     int rho;    // pivots in the current panel
     uint64_t pm;        // pivot bits (columns 64g + c) of the current panel
+    int finished;       // every active row is zero: the remaining panels have no pivot
+    int last_swept;     // last panel with pivots (its sweep ran), -1 before the first
     int sel_row[64];           // selected row ids (selection slot -> row)
     uint64_t comb[64];  // by pivot bit c: selection slots XORed into pivot row c
 };
@@ -45,20 +56,38 @@ using dev::ffs64;
 using dev::popc64;
 
 constexpr int PS_THREADS = 1024;
+// Sweep workgroups that record whether they left a nonzero active-row word
+// past their panel: workgroup b of the sweep of panel g stores g + 1 into
+// swept[b] (no reset needed between panels: older entries hold smaller tags).
+constexpr int NSWEPT = 4096;
 
 // One workgroup: selects the panel's pivot rows (elim_device.h) and records
 // the panel in meta and pivcols.
 __global__ void __launch_bounds__(PS_THREADS)
 k_panel_select(const uint64_t *__restrict__ M, size_t ld, uint32_t *__restrict__ act,
-               ElimMeta *__restrict__ meta, int *__restrict__ pivcols, int g, int rmax) {
+               ElimMeta *__restrict__ meta, int *__restrict__ pivcols, const unsigned *swept,
+               int g, int rmax) {
     __shared__ dev::SelectLDS<PS_THREADS> L;
     const int n_act = meta->n_act;
     const int r_before = meta->r;
-    if (n_act == 0 || r_before >= rmax) {
+    // no workgroup of the last sweep saw a nonzero active-row word past its
+    // panel: no panel from here on has a pivot.  Checked after a panel with
+    // fewer than 64 pivots (a full panel rarely ends the rank).
+    bool finished = meta->finished != 0;
+    const int ls = meta->last_swept;
+    if (!finished && ls >= 0 && (meta->pm == 0 || meta->rho < 64)) {
+        bool seen = false;
+#pragma unroll
+        for (int k = 0; k < NSWEPT / PS_THREADS; ++k)
+            seen |= swept[threadIdx.x + k * PS_THREADS] == (unsigned)ls + 1u;
+        finished = !__syncthreads_or(seen);
+    }
+    if (n_act == 0 || r_before >= rmax || finished) {
         if (threadIdx.x == 0) {
             meta->r0 = r_before;
             meta->rho = 0;
             meta->pm = 0;
+            meta->finished = finished ? 1 : 0;
         }
         return;
     }
@@ -74,7 +103,16 @@ k_panel_select(const uint64_t *__restrict__ M, size_t ld, uint32_t *__restrict__
         meta->pm = po.pm;
         meta->r = r_before + po.rho;
         meta->n_act = po.n_act_new;
+        if (po.pm) meta->last_swept = g;
     }
+}
+
+// A wave with a nonzero active-row word past panel g tags its workgroup's
+// entry (plain stores of one value: no atomics, no contention; grids larger
+// than NSWEPT share entries, which only merges tags of the same panel).
+__device__ __forceinline__ void mark_rest_nonzero(unsigned *swept, uint64_t nz, int g) {
+    const unsigned b = (blockIdx.y * gridDim.x + blockIdx.x) % NSWEPT;
+    if (__ballot(nz != 0ull) != 0ull && (threadIdx.x & 63) == 0) swept[b] = (unsigned)g + 1u;
 }
 
 constexpr int BP_PIV = 8;  // pivot rows per k_build_pivots workgroup
@@ -182,8 +220,8 @@ __device__ __forceinline__ void lds_barrier() {
 template <int TILE>
 __global__ void __launch_bounds__(UP_WAVES * 64, 64 / TILE)
 k_update_m4ri(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
-              uint64_t *__restrict__ Q, size_t ldq, const ElimMeta *__restrict__ meta, int g,
-              int wc, int rref) {
+              uint64_t *__restrict__ Q, size_t ldq, const ElimMeta *__restrict__ meta,
+              unsigned *__restrict__ swept, int g, int wc, int rref) {
     // T[entry][table][word]: the table index lands in the ds_read immediate
     // offset, the entry index is the only per-lookup address
     __shared__ uint64_t T[16][16][TILE];
@@ -277,6 +315,7 @@ k_update_m4ri(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ 
     if (!rows_here) return;
     build(pv);  // tables of the pivot rows
     lds_barrier();
+    uint64_t nz = 0;  // active rows' words past the panel, ORed
     for (; q0 < total; q0 += stride) {
         // rows whose multiplier is zero on the whole wave are skipped (a
         // wave-uniform branch); stores come after all lookups, so no row
@@ -288,18 +327,21 @@ k_update_m4ri(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ 
             for (int t = 0; t < 16; ++t) a[j] ^= T[(x[j] >> (4 * t)) & 15u][t][wl];
         }
 #pragma unroll
-        for (int j = 0; j < R; ++j)
+        for (int j = 0; j < R; ++j) {
             if (x[j]) rp[j][w] = a[j];
+            if (q0 + j * RPW < n_act && wok) nz |= a[j];
+        }
         if (q0 + stride < total) load_rows(q0 + stride);
     }
+    mark_rest_nonzero(swept, nz, g);
 }
 
 // GE form: the panel's pivot rows themselves in LDS; each row XORs one pivot
 // row per set pivot bit of its panel word.
 __global__ void __launch_bounds__(UP_WAVES * 64)
 k_update_ge(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
-            uint64_t *__restrict__ Q, size_t ldq, const ElimMeta *__restrict__ meta, int g, int wc,
-            int rref) {
+            uint64_t *__restrict__ Q, size_t ldq, const ElimMeta *__restrict__ meta,
+            unsigned *__restrict__ swept, int g, int wc, int rref) {
     __shared__ uint64_t P[64][UP_TILE];
     const uint64_t pm = meta->pm;
     if (pm == 0) return;
@@ -325,6 +367,7 @@ k_update_ge(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ ac
 #pragma unroll
     for (int k = 0; k < 64 / UP_WAVES; ++k) P[wave + k * UP_WAVES][lane] = pv[k];
     lds_barrier();
+    uint64_t nz = 0;  // active rows' words past the panel, ORed
     for (; q0 < total; q0 += stride) {
 #pragma unroll
         for (int j = 0; j < UP_ROWS_GE; ++j) {
@@ -339,13 +382,16 @@ k_update_ge(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ ac
             rb.x[j] = upd;
         }
 #pragma unroll
-        for (int j = 0; j < UP_ROWS_GE; ++j)
+        for (int j = 0; j < UP_ROWS_GE; ++j) {
             if (rb.x[j]) rb.rp[j][w] = rb.a[j];
+            if (q0 + j < n_act && wok) nz |= rb.a[j];
+        }
         if (q0 + stride < total) {
             batch_ptrs(rb, M, ld, act, Q, ldq, n_act, total, q0 + stride);
             batch_words(rb, g, w, wok, pm, total, q0 + stride);
         }
     }
+    mark_rest_nonzero(swept, nz, g);
 }
 
 // RREF only: word g of the earlier pivot rows (active rows' word g is dead),
@@ -380,6 +426,8 @@ __global__ void k_elim_init(ElimMeta *meta, uint32_t *act, int m) {
         meta->r0 = 0;
         meta->rho = 0;
         meta->pm = 0;
+        meta->finished = 0;
+        meta->last_swept = -1;
     }
 }
 
@@ -414,32 +462,39 @@ k_null_logical(const uint64_t *__restrict__ Q, size_t ldq, const int *__restrict
     }
 }
 
-// Packed Z (n x f, row-major, ldz words): word (p, jw) gathers bits
-// freecols[64 jw .. 64 jw + 63] of pivot row rowmap[p] (or the unit bit).
+// Packed Z (n x f, row-major, ldz words), one thread per word (p, jw) so
+// the stores are coalesced: a free column's row is its unit vector; a pivot
+// column's row gathers the free columns of its RREF row rowmap[p]: output
+// word jw takes the compressed column words tab[jw].ws .. tab[jw+1].ws, each
+// word's free bits packed by the six-step compress and shifted to their place
+// (column word w holds free columns tab[w].fb ..).
 __global__ void __launch_bounds__(256)
 k_null_packed(const uint64_t *__restrict__ Q, size_t ldq, const int *__restrict__ rowmap,
-              const int *__restrict__ freecols, size_t n, size_t f, uint64_t *__restrict__ Z,
-              size_t ldz) {
-    const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (p >= n) return;
-    const int rm = rowmap[p];
+              const NullWord *__restrict__ tab, size_t n, size_t wc, size_t f,
+              uint64_t *__restrict__ Z, size_t ldz) {
     const size_t fw = (f + 63) / 64;
-    for (size_t jw = blockIdx.y; jw < ldz; jw += gridDim.y) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n * ldz; i += (size_t)gridDim.x * 256) {
+        const size_t p = i / ldz, jw = i - p * ldz;
         uint64_t word = 0;
         if (jw < fw) {
+            const int rm = rowmap[p];
             if (rm >= 0) {
                 const uint64_t *row = Q + (size_t)rm * ldq;
-                const size_t jn = f - jw * 64 < 64 ? f - jw * 64 : 64;
-                for (size_t b = 0; b < jn; ++b) {
-                    const int fc = freecols[jw * 64 + b];
-                    word |= ((row[fc >> 6] >> (fc & 63)) & 1ull) << b;
+                const long lo = 64 * (long)jw;
+                const int wa = tab[jw].ws, wb = jw + 1 < fw ? tab[jw + 1].ws : (int)wc - 1;
+                for (int w = wa; w <= wb; ++w) {
+                    const long fb = tab[w].fb;
+                    if (fb >= lo + 64) break;
+                    const uint64_t c = dev::compress_apply(row[w], tab[w].fm, tab[w].mv);
+                    const long sh = fb - lo;
+                    word |= sh >= 0 ? c << sh : c >> (-sh);
                 }
             } else {
                 const size_t fi = (size_t)(-rm - 1);
                 if (fi / 64 == jw) word = 1ull << (fi & 63);
             }
         }
-        Z[p * ldz + jw] = word;
+        Z[i] = word;
     }
 }
 
@@ -464,14 +519,15 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
     mela_status st;
     if ((st = scratch_get(ctx, SLOT_ELIM_M, m * ld * 8, &pM)) != MELA_OK) return st;
     if ((st = scratch_get(ctx, SLOT_ELIM_Q, rmax * ld * 8, &pQ)) != MELA_OK) return st;
-    if ((st = scratch_get(ctx, SLOT_ELIM_META, sizeof(ElimMeta) + (rmax + 1) * sizeof(int), &pMeta)) !=
-        MELA_OK)
+    const size_t swept_off = (sizeof(ElimMeta) + (rmax + 1) * sizeof(int) + 15) & ~(size_t)15;
+    if ((st = scratch_get(ctx, SLOT_ELIM_META, swept_off + NSWEPT * sizeof(unsigned), &pMeta)) != MELA_OK)
         return st;
     if ((st = scratch_get(ctx, SLOT_ELIM_ACT, m * sizeof(uint32_t), &pAct)) != MELA_OK) return st;
     uint64_t *Mw = (uint64_t *)pM, *Q = (uint64_t *)pQ;
     ElimMeta *meta = (ElimMeta *)pMeta;
     int *pivcols = (int *)((char *)pMeta + sizeof(ElimMeta));
     uint32_t *act = (uint32_t *)pAct;
+    unsigned *swept = (unsigned *)((char *)pMeta + swept_off);
 
     const size_t tot = m * wc;
     hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, A, lda,
@@ -488,9 +544,11 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
             mela_status zs = launch_fill_zero(s, Q, rmax * ld * sizeof(uint64_t));
             if (zs != MELA_OK) return zs;
         }
+        mela_status zs = launch_fill_zero(s, swept, NSWEPT * sizeof(unsigned));
+        if (zs != MELA_OK) return zs;
         for (size_t g = 0; g < wc; ++g) {
             hipLaunchKernelGGL(k_panel_select, dim3(1), dim3(PS_THREADS), 0, s, Mw, ld, act, meta,
-                               pivcols, (int)g, (int)rmax);
+                               pivcols, swept, (int)g, (int)rmax);
             // GE builds the pivot rows in a kernel of their own; the M4RI
             // sweep forms them from the selected rows' tables itself
             if (variant == MELA_ELIM_GE)
@@ -512,10 +570,11 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
                 if (ty < 1) ty = 1;
                 if (variant == MELA_ELIM_GE)
                     hipLaunchKernelGGL(k_update_ge, dim3(tx, ty), dim3(UP_WAVES * 64), 0, s, Mw, ld,
-                                       act, Q, ld, meta, (int)g, (int)wc, rref ? 1 : 0);
+                                       act, Q, ld, meta, swept, (int)g, (int)wc, rref ? 1 : 0);
                 else
-                    hipLaunchKernelGGL(k_update_m4ri<M4RI_TILE>, dim3(tx, ty), dim3(UP_WAVES * 64), 0, s, Mw,
-                                       ld, act, Q, ld, meta, (int)g, (int)wc, rref ? 1 : 0);
+                    hipLaunchKernelGGL(k_update_m4ri<M4RI_TILE>, dim3(tx, ty), dim3(UP_WAVES * 64), 0, s,
+                                       Mw, ld, act, Q, ld, meta, swept, (int)g, (int)wc,
+                                       rref ? 1 : 0);
             }
             if (rref)
                 hipLaunchKernelGGL(k_update_panelword, dim3((unsigned)((rmax + 255) / 256)),
@@ -587,13 +646,41 @@ mela_status launch_null_logical(hipStream_t s, const uint64_t *Q, size_t ldq, co
     return MELA_OK;
 }
 
-mela_status launch_null_packed(hipStream_t s, const uint64_t *Q, size_t ldq, const int *rowmap,
-                               const int *freecols, size_t n, size_t f, uint64_t *Z, size_t ldz) {
+mela_status launch_null_packed(mela_ctx *ctx, const uint64_t *Q, size_t ldq, const int *rowmap,
+                               const std::vector<int> &piv, size_t n, size_t f, uint64_t *Z,
+                               size_t ldz) {
     if (n == 0 || ldz == 0) return MELA_OK;
-    const unsigned gy = (unsigned)(ldz < 65535 ? ldz : 65535);
-    hipLaunchKernelGGL(k_null_packed, dim3((unsigned)((n + 255) / 256), gy), dim3(256), 0, s, Q,
-                       ldq, rowmap, freecols, n, f, Z, ldz);
+    // per column word: free mask, its compress masks, free columns before it;
+    // tab[j].ws = the column word holding free column 64 j
+    const size_t wc = words64(n);
+    std::vector<NullWord> tab(wc + 1);
+    std::vector<uint64_t> pmask(wc, 0);
+    for (int c : piv) pmask[(size_t)c >> 6] |= 1ull << (c & 63);
+    long fb = 0;
+    for (size_t w = 0; w < wc; ++w) {
+        const size_t valid = n - 64 * w < 64 ? n - 64 * w : 64;
+        const uint64_t fm = ~pmask[w] & (valid == 64 ? ~0ull : ((1ull << valid) - 1ull));
+        NullWord &t = tab[w];
+        t.fm = fm;
+        dev::compress_masks(fm, t.mv);
+        t.fb = fb;
+        const long cnt = __builtin_popcountll(fm);
+        const long j = (fb + 63) / 64;
+        if (cnt && 64 * j < fb + cnt) tab[(size_t)j].ws = (int)w;
+        fb += cnt;
+    }
+    void *p = nullptr;
+    mela_status st = scratch_get(ctx, SLOT_NULL_TAB, tab.size() * sizeof(NullWord), &p);
+    if (st != MELA_OK) return st;
+    MELA_HIP(hipMemcpyAsync(p, tab.data(), tab.size() * sizeof(NullWord), hipMemcpyHostToDevice,
+                            ctx->stream));
+    const size_t total = n * ldz;
+    const unsigned blocks = (unsigned)std::min<size_t>((total + 255) / 256, (size_t)1 << 20);
+    hipLaunchKernelGGL(k_null_packed, dim3(blocks), dim3(256), 0, ctx->stream, Q, ldq, rowmap,
+                       (const NullWord *)p, n, wc, f, Z, ldz);
     MELA_HIP(hipGetLastError());
+    // the table is host memory until the copy has run
+    MELA_HIP(hipStreamSynchronize(ctx->stream));
     return MELA_OK;
 }
 

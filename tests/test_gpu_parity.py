@@ -219,6 +219,41 @@ def test_null_low_rank(mela, orc, m, n, r):
         assert np.array_equal(fn(H), want), fn.__name__
 
 
+def _blocks(rng, orc, m, parts):
+    """Columns side by side: ("lr", n, r) a rank-r product, ("zero", n), ("rand", n)."""
+    cols = []
+    for p in parts:
+        if p[0] == "lr":
+            cols.append(orc.matmul(rng.integers(0, 2, (m, p[2])).astype(bool),
+                                   rng.integers(0, 2, (p[2], p[1])).astype(bool), "gf2"))
+        elif p[0] == "zero":
+            cols.append(np.zeros((m, p[1]), dtype=bool))
+        else:
+            cols.append(rng.integers(0, 2, (m, p[1])).astype(bool))
+    return np.concatenate(cols, axis=1)
+
+
+# rank stalls (a low-rank block, then zero panels) and resumes, leading zero
+# panels, and a rank exhausted exactly at a panel boundary followed by zeros:
+# the elimination skips panels once every active row is zero, and must not
+# do so while later columns still hold pivots
+@pytest.mark.parametrize("parts", [
+    [("lr", 256, 64), ("zero", 320), ("rand", 100)],
+    [("zero", 200), ("rand", 150)],
+    [("rand", 128), ("zero", 400)],
+    [("lr", 300, 37), ("zero", 70), ("lr", 200, 20), ("zero", 130)],
+])
+def test_rank_null_block_structured(mela, orc, parts):
+    print(f"[block structured] {parts}", flush=True)
+    H = _blocks(np.random.default_rng(len(parts) * 7 + parts[0][1]), orc, 500, parts)
+    want_r = orc.rank(H, "gf2")
+    want_z = orc.null(H, "gf2")
+    assert mela.mela_rank_gf2(H) == want_r
+    assert mela.mela_rank_m4ri(H) == want_r
+    for fn in (mela.mela_null_gf2, mela.mela_null_m4ri):
+        assert np.array_equal(fn(H), want_z), fn.__name__
+
+
 def test_null_double_input(mela, orc):
     rng = np.random.default_rng(9)
     H = rng.integers(-5, 6, (40, 70)).astype(np.float64)
