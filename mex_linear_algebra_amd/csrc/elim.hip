@@ -15,15 +15,12 @@
 //                   ballot/readlane), removes them from the active list and
 //                   computes the RREF basis of the panel plus, for each basis
 //                   vector c, the set comb[c] of selected rows that XOR to it;
-//   k_build_pivots  (GE) forms the new pivot rows Q (words g .. wc-1): the
-//                   selected rows of a 64-word tile are staged in LDS and each
-//                   pivot row XORs the ones named by comb[c];
-//   k_update        clears the panel's pivot columns from every active row
-//                   (and, for RREF, from every earlier pivot row) with one
-//                   table XOR per 4 pivot columns (M4RI) or one XOR per pivot
-//                   row (GE), sweeping words g+1 .. wc-1 -- the HBM-bound part;
-//                   the M4RI kernel forms its tile's pivot rows itself from
-//                   4-bit tables of the selected rows (and writes them to Q);
+//   k_sweep         forms the panel's pivot rows of its tile from 4-bit tables
+//                   of the selected rows (and writes them to Q), then clears
+//                   the panel's pivot columns from every active row (and, for
+//                   RREF, from every earlier pivot row) with one table XOR per
+//                   4 pivot columns (M4RI) or one XOR per pivot row (GE),
+//                   sweeping words g+1 .. wc-1;
 //   k_update_panelword (RREF) word g of the earlier pivot rows.
 #include "elim_device.h"
 
@@ -115,41 +112,6 @@ __device__ __forceinline__ void mark_rest_nonzero(unsigned *swept, uint64_t nz, 
     if (__ballot(nz != 0ull) != 0ull && (threadIdx.x & 63) == 0) swept[b] = (unsigned)g + 1u;
 }
 
-constexpr int BP_PIV = 8;  // pivot rows per k_build_pivots workgroup
-
-// Q[r0 + j][w], w in one 64-word tile from g, for BP_PIV pivots j = the XOR of
-// the selected rows named by comb[c] (c = j-th pivot bit).  The tile's 64
-// selected-row words per lane are loaded together and staged in LDS, so the
-// XORs read LDS instead of chaining dependent global loads.
-__global__ void __launch_bounds__(512)
-k_build_pivots(const uint64_t *__restrict__ M, size_t ld, uint64_t *__restrict__ Q, size_t ldq,
-               const ElimMeta *__restrict__ meta, int g, int wc) {
-    __shared__ uint64_t S[64][64];
-    const uint64_t pm = meta->pm;
-    const int rho = meta->rho;
-    const int j0 = blockIdx.y * BP_PIV;
-    if (pm == 0 || j0 >= rho) return;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int w = g + blockIdx.x * 64 + lane;
-    const bool wok = w < wc;
-    uint64_t v[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const int slot = wave + 8 * k;
-        v[k] = (wok && slot < rho) ? M[(size_t)meta->sel_row[slot] * ld + w] : 0ull;
-    }
-#pragma unroll
-    for (int k = 0; k < 8; ++k) S[wave + 8 * k][lane] = v[k];
-    __syncthreads();
-    const int j = j0 + wave;  // one pivot per wave
-    if (j >= rho || !wok) return;
-    uint64_t t = pm;
-    for (int q = 0; q < j; ++q) t &= t - 1;
-    uint64_t acc = 0;
-    for (uint64_t y = meta->comb[ffs64(t)]; y; y &= y - 1) acc ^= S[ffs64(y)][lane];
-    Q[(size_t)(meta->r0 + j) * ldq + w] = acc;
-}
-
 // Row pointer of unified row index q: active rows first, then (RREF) the
 // earlier pivot rows.
 __device__ __forceinline__ uint64_t *row_ptr(uint64_t *M, size_t ld, const uint32_t *act,
@@ -157,48 +119,9 @@ __device__ __forceinline__ uint64_t *row_ptr(uint64_t *M, size_t ld, const uint3
     return q < n_act ? M + (size_t)act[q] * ld : Q + (size_t)(q - n_act) * ldq;
 }
 
-constexpr int UP_TILE = 64;   // words per workgroup (one per lane)
 constexpr int UP_WAVES = 8;   // 512 threads
-constexpr int UP_ROWS_M4RI = 16;  // rows in flight per wave (M4RI sweep)
-constexpr int UP_ROWS_GE = 4;     // rows in flight per wave (GE sweep: fewer registers)
-constexpr int M4RI_TILE = 32;     // words per M4RI sweep workgroup (32 or 64)
-
-// R rows of one wave: row pointers, panel words x and this lane's word w.
-// The pointers are resolved first (one active-list load each); the word loads
-// are issued together afterwards, so all of them are in flight at once.
-template <int R>
-struct RowBatch {
-    uint64_t *rp[R];
-    uint64_t x[R];
-    uint64_t a[R];
-};
-
-// Positions past the end resolve to the row of position 0 (always valid when
-// the kernel runs) and get a zero multiplier, so every load is unconditional
-// and all of a batch's loads are in flight together (a load under a branch
-// is waited for before the branch closes).
-template <int R>
-__device__ __forceinline__ void batch_ptrs(RowBatch<R> &b, uint64_t *M, size_t ld,
-                                           const uint32_t *act, uint64_t *Q, size_t ldq,
-                                           int n_act, int total, int q0) {
-#pragma unroll
-    for (int j = 0; j < R; ++j) {
-        const int q = q0 + j < total ? q0 + j : 0;
-        b.rp[j] = row_ptr(M, ld, act, Q, ldq, n_act, q);
-    }
-}
-
-template <int R>
-__device__ __forceinline__ void batch_words(RowBatch<R> &b, int g, int w, bool wok, uint64_t pm,
-                                            int total, int q0) {
-    const int ws = wok ? w : g;
-#pragma unroll
-    for (int j = 0; j < R; ++j) b.x[j] = b.rp[j][g];
-#pragma unroll
-    for (int j = 0; j < R; ++j) b.a[j] = b.rp[j][ws];
-#pragma unroll
-    for (int j = 0; j < R; ++j) b.x[j] = (q0 + j < total && wok) ? b.x[j] & pm : 0ull;
-}
+constexpr int UP_ROWS = 16;       // rows in flight per wave in the sweep
+constexpr int SWEEP_TILE = 32;    // words per sweep workgroup (32 or 64)
 
 // LDS-only workgroup barrier: row loads already in flight stay in flight.
 __device__ __forceinline__ void lds_barrier() {
@@ -206,28 +129,29 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_s_barrier();
 }
 
-// M4RI form, pivot rows included: the workgroup's TILE-word tile starts at
-// word g (TILE 64: 128 KiB of tables, one workgroup per CU; TILE 32: two).
+// Sweep of panel g, pivot rows included: the workgroup's TILE-word tile starts
+// at word g (TILE 64: 128 KiB of tables, one workgroup per CU; TILE 32: two).
 // Each thread owns TILE/32 (table t, word) pairs: it loads the selected rows
 // 4t..4t+3 of its word and builds their 4-bit table t; pivot row c = XOR of
-// the selected rows in comb[c] is then 16 lookups, and the four pivot rows
-// 4t..4t+3 it holds give the sweep's table t (slot c = pivot bit c, zero
-// where c is not a pivot).  Row-stripe 0 of each tile writes the tile's
-// pivot-row words to Q.  Each active (and for RREF earlier pivot) row XORs 16
-// table entries selected by the nibbles of its panel word into words
-// g+1 .. wc-1 (64 / TILE rows per wave instruction); word g is left to
+// the selected rows in comb[c] is then 16 lookups.  Row-stripe 0 of each
+// tile writes the tile's pivot-row words to Q.  Each active (and for RREF
+// earlier pivot) row then clears the panel's pivot columns from its words
+// g+1 .. wc-1 (64 / TILE rows per wave instruction): M4RI XORs 16 entries of
+// the pivot rows' 4-bit tables (the four pivot rows 4t..4t+3 a thread holds
+// give table t; slot c = pivot bit c, zero where c is not a pivot), GE XORs
+// the pivot row of every set bit of its panel word.  Word g is left to
 // k_update_panelword so that every tile reads an unmodified multiplier.
-template <int TILE>
+template <int TILE, bool GE>
 __global__ void __launch_bounds__(UP_WAVES * 64, 64 / TILE)
-k_update_m4ri(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
-              uint64_t *__restrict__ Q, size_t ldq, const ElimMeta *__restrict__ meta,
-              unsigned *__restrict__ swept, int g, int wc, int rref) {
+k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
+        uint64_t *__restrict__ Q, size_t ldq, const ElimMeta *__restrict__ meta,
+        unsigned *__restrict__ swept, int g, int wc, int rref) {
     // T[entry][table][word]: the table index lands in the ds_read immediate
     // offset, the entry index is the only per-lookup address
     __shared__ uint64_t T[16][16][TILE];
     constexpr int NT = UP_WAVES * 64;
     constexpr int RPW = 64 / TILE;          // rows per wave instruction
-    constexpr int R = UP_ROWS_M4RI / RPW;   // row batch per lane
+    constexpr int R = UP_ROWS / RPW;        // row batch per lane
     constexpr int TPT = 16 * TILE / NT;     // (table, word) pairs per thread
     const uint64_t pm = meta->pm;
     if (pm == 0) return;
@@ -313,7 +237,16 @@ k_update_m4ri(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ 
     }
     lds_barrier();
     if (!rows_here) return;
-    build(pv);  // tables of the pivot rows
+    // GE: the pivot rows themselves, by pivot bit, over the tables' LDS
+    uint64_t(*P)[TILE] = reinterpret_cast<uint64_t(*)[TILE]>(&T[0][0][0]);
+    if (GE) {
+#pragma unroll
+        for (int k = 0; k < TPT; ++k)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) P[4 * pair_t(k) + b][(tid + k * NT) % TILE] = pv[k][b];
+    } else {
+        build(pv);  // tables of the pivot rows
+    }
     lds_barrier();
     uint64_t nz = 0;  // active rows' words past the panel, ORed
     for (; q0 < total; q0 += stride) {
@@ -323,8 +256,12 @@ k_update_m4ri(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ 
 #pragma unroll
         for (int j = 0; j < R; ++j) {
             if (__ballot(x[j] != 0ull) == 0ull) continue;
+            if (GE) {
+                for (uint64_t xx = x[j]; xx; xx &= xx - 1) a[j] ^= P[ffs64(xx)][wl];
+            } else {
 #pragma unroll
-            for (int t = 0; t < 16; ++t) a[j] ^= T[(x[j] >> (4 * t)) & 15u][t][wl];
+                for (int t = 0; t < 16; ++t) a[j] ^= T[(x[j] >> (4 * t)) & 15u][t][wl];
+            }
         }
 #pragma unroll
         for (int j = 0; j < R; ++j) {
@@ -332,64 +269,6 @@ k_update_m4ri(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ 
             if (q0 + j * RPW < n_act && wok) nz |= a[j];
         }
         if (q0 + stride < total) load_rows(q0 + stride);
-    }
-    mark_rest_nonzero(swept, nz, g);
-}
-
-// GE form: the panel's pivot rows themselves in LDS; each row XORs one pivot
-// row per set pivot bit of its panel word.
-__global__ void __launch_bounds__(UP_WAVES * 64)
-k_update_ge(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
-            uint64_t *__restrict__ Q, size_t ldq, const ElimMeta *__restrict__ meta,
-            unsigned *__restrict__ swept, int g, int wc, int rref) {
-    __shared__ uint64_t P[64][UP_TILE];
-    const uint64_t pm = meta->pm;
-    if (pm == 0) return;
-    const int n_act = meta->n_act, r0 = meta->r0;
-    const int total = n_act + (rref ? r0 : 0);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int w = g + 1 + blockIdx.x * UP_TILE + lane;
-    const bool wok = w < wc;
-    const int stride = gridDim.y * UP_WAVES * UP_ROWS_GE;
-    int q0 = (blockIdx.y * UP_WAVES + wave) * UP_ROWS_GE;
-    if ((int)blockIdx.y * UP_WAVES * UP_ROWS_GE >= total) return;
-    uint64_t pv[64 / UP_WAVES];
-#pragma unroll
-    for (int k = 0; k < 64 / UP_WAVES; ++k) {
-        const int c = wave + k * UP_WAVES;
-        pv[k] = (wok && ((pm >> c) & 1ull))
-                    ? Q[(size_t)(r0 + popc64(pm & ((1ull << c) - 1ull))) * ldq + w]
-                    : 0ull;
-    }
-    RowBatch<UP_ROWS_GE> rb;
-    batch_ptrs(rb, M, ld, act, Q, ldq, n_act, total, q0);
-    batch_words(rb, g, w, wok, pm, total, q0);
-#pragma unroll
-    for (int k = 0; k < 64 / UP_WAVES; ++k) P[wave + k * UP_WAVES][lane] = pv[k];
-    lds_barrier();
-    uint64_t nz = 0;  // active rows' words past the panel, ORed
-    for (; q0 < total; q0 += stride) {
-#pragma unroll
-        for (int j = 0; j < UP_ROWS_GE; ++j) {
-            uint64_t x = rb.x[j];
-            uint64_t a = rb.a[j];
-            const bool upd = x != 0;
-            while (x) {
-                a ^= P[ffs64(x)][lane];
-                x &= x - 1;
-            }
-            rb.a[j] = a;
-            rb.x[j] = upd;
-        }
-#pragma unroll
-        for (int j = 0; j < UP_ROWS_GE; ++j) {
-            if (rb.x[j]) rb.rp[j][w] = rb.a[j];
-            if (q0 + j < n_act && wok) nz |= rb.a[j];
-        }
-        if (q0 + stride < total) {
-            batch_ptrs(rb, M, ld, act, Q, ldq, n_act, total, q0 + stride);
-            batch_words(rb, g, w, wok, pm, total, q0 + stride);
-        }
     }
     mark_rest_nonzero(swept, nz, g);
 }
@@ -549,32 +428,26 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
         for (size_t g = 0; g < wc; ++g) {
             hipLaunchKernelGGL(k_panel_select, dim3(1), dim3(PS_THREADS), 0, s, Mw, ld, act, meta,
                                pivcols, swept, (int)g, (int)rmax);
-            // GE builds the pivot rows in a kernel of their own; the M4RI
-            // sweep forms them from the selected rows' tables itself
-            if (variant == MELA_ELIM_GE)
-                hipLaunchKernelGGL(k_build_pivots, dim3((unsigned)((wc - g + 63) / 64), 64 / BP_PIV),
-                                   dim3(512), 0, s, Mw, ld, Q, ld, meta, (int)g, (int)wc);
-            const size_t rem = variant == MELA_ELIM_GE ? wc - g - 1 : wc - g;  // words per grid
-            if (rem > 0) {
-                const size_t tile = variant == MELA_ELIM_GE ? UP_TILE : M4RI_TILE;
-                const unsigned tx = (unsigned)((rem + tile - 1) / tile);
-                // rows are strided over gridDim.y: M4RI one round of the
-                // resident workgroups (64 / M4RI_TILE per CU), GE two rounds
-                // (32 KiB of pivot rows, several workgroups per CU)
+            {
+                // the sweep forms the panel's pivot rows of its tile itself;
+                // tiles start at word g, rows strided over gridDim.y, one
+                // round of the resident workgroups (64 / TILE per CU)
+                const size_t rem = wc - g;
+                const unsigned tx = (unsigned)((rem + SWEEP_TILE - 1) / SWEEP_TILE);
                 const size_t rows_hint = m + (rref ? rmax : 0);
-                const unsigned per_cu = variant == MELA_ELIM_GE ? 2u : (unsigned)(64 / M4RI_TILE);
-                unsigned ty = (unsigned)((per_cu * cu + tx - 1) / tx);
-                const unsigned per = UP_WAVES * (variant == MELA_ELIM_GE ? UP_ROWS_GE : UP_ROWS_M4RI);
+                unsigned ty = (unsigned)(((64 / SWEEP_TILE) * cu + tx - 1) / tx);
+                const unsigned per = UP_WAVES * UP_ROWS;
                 const unsigned ty_max = (unsigned)((rows_hint + per - 1) / per);
                 if (ty > ty_max) ty = ty_max;
                 if (ty < 1) ty = 1;
                 if (variant == MELA_ELIM_GE)
-                    hipLaunchKernelGGL(k_update_ge, dim3(tx, ty), dim3(UP_WAVES * 64), 0, s, Mw, ld,
-                                       act, Q, ld, meta, swept, (int)g, (int)wc, rref ? 1 : 0);
+                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sweep<SWEEP_TILE, true>), dim3(tx, ty),
+                                       dim3(UP_WAVES * 64), 0, s, Mw, ld, act, Q, ld, meta, swept, (int)g,
+                                       (int)wc, rref ? 1 : 0);
                 else
-                    hipLaunchKernelGGL(k_update_m4ri<M4RI_TILE>, dim3(tx, ty), dim3(UP_WAVES * 64), 0, s,
-                                       Mw, ld, act, Q, ld, meta, swept, (int)g, (int)wc,
-                                       rref ? 1 : 0);
+                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sweep<SWEEP_TILE, false>), dim3(tx, ty),
+                                       dim3(UP_WAVES * 64), 0, s, Mw, ld, act, Q, ld, meta, swept, (int)g,
+                                       (int)wc, rref ? 1 : 0);
             }
             if (rref)
                 hipLaunchKernelGGL(k_update_panelword, dim3((unsigned)((rmax + 255) / 256)),
