@@ -503,7 +503,7 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=256)
     ap.add_argument("--bytes", type=int, default=1 << 30)
     ap.add_argument("--serve-rates", type=float, nargs="+",
-                    default=[50, 100, 200, 400, 800, 1600, 3200])
+                    default=[50, 100, 200, 400, 800, 1600, 3200, 6400, 12800])
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-pool", type=int, default=48)
     ap.add_argument("--serve-workers", type=int, default=4)

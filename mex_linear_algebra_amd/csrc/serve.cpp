@@ -13,6 +13,7 @@
 // the client releases them.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -56,8 +57,31 @@ size_t est_bytes(const mela_request &r) {
     return b;
 }
 
+// Time (ms) that one workgroup of the persistent batch kernel takes over
+// one matrix, end to end (M4RI mode), fitted to single-matrix batches on
+// MI355X over 128..4096 rows x 128..8192 columns (within 35 %): a fixed
+// cost, a cost per 64-pivot panel, per panel and row (the selection scans
+// the active rows), per row, pivot and column word (the sweep) and per word
+// of the null basis written.
+double batch_member_ms(const mela_request &r) {
+    const double m = (double)r.A.rows, n = (double)r.A.cols;
+    const double rk = std::min(m, n), wc = (double)words64(r.A.cols);
+    const double p = std::ceil(rk / 64.0);
+    if (r.op == MELA_OP_NULL)
+        return 0.098 + 0.0139 * p + 0.0142e-3 * m * p + 0.0201e-6 * m * rk * wc + 1.514e-6 * n * wc;
+    return 0.113 + 0.0054 * p + 0.0155e-3 * m * p + 0.0117e-6 * m * rk * wc + 0.1005e-6 * n * wc;
+}
+
+// A batch launch lasts as long as its slowest member, so a matrix joins a
+// micro-batch only while one workgroup finishes it about as fast as the
+// single-matrix path, which spreads each panel over the whole device
+// (0.06-0.4 ms for these small shapes, 1.4-2 ms at 4096 x 8192 where one
+// workgroup would take 28-52 ms).
+constexpr double kBatchMemberMaxMs = 0.35;
+
 bool batchable(const mela_request &r) {
-    return r.op != MELA_OP_MATMUL && r.A.rows <= kBatchMaxM && r.A.cols <= kBatchMaxN;
+    return r.op != MELA_OP_MATMUL && r.A.rows <= kBatchMaxM && r.A.cols <= kBatchMaxN &&
+           batch_member_ms(r) <= kBatchMemberMaxMs;
 }
 
 }  // namespace
@@ -193,6 +217,18 @@ void worker_main(mela_server *s) {
         if (s->jobs.empty() && s->stop) break;
         Job job = s->jobs.front();
         s->jobs.pop_front();
+        if (job.kind != Job::SINGLE) {
+            // continuous batching: fold the micro-batches of the same kind
+            // that queued up while every worker was busy into this launch
+            for (auto it = s->jobs.begin(); it != s->jobs.end();) {
+                if (it->kind == job.kind && job.ids.size() + it->ids.size() <= (size_t)s->max_batch) {
+                    job.ids.insert(job.ids.end(), it->ids.begin(), it->ids.end());
+                    it = s->jobs.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+        }
         ++s->busy_workers;
         ++s->n_jobs;
         // snapshot the requests (the map entries stay put while admitted)
@@ -222,27 +258,40 @@ void worker_main(mela_server *s) {
             std::vector<uint32_t> ranks(rq.size());
             std::vector<mela_bitmat> nulls(rq.size());
             if (st == MELA_OK) st = mela_batch_results(b, ranks.data(), nulls.data());
+            // copy each null basis out of the batch arena (issued together,
+            // one synchronisation for the whole batch)
+            std::vector<mela_bitmat> outs(rq.size());
             for (size_t i = 0; i < rq.size() && st == MELA_OK; ++i) {
                 rq[i]->res.rank = ranks[i];
-                if (want_null) {
-                    mela_bitmat Z{};
-                    mela_status s2 = mela_bitmat_alloc(ctx, nulls[i].rows, nulls[i].cols, &Z);
-                    if (s2 == MELA_OK && Z.rows && nulls[i].cols) {
-                        // the batch view has the row pitch of the input width
-                        hipError_t e = hipMemcpy2DAsync(
-                            Z.data, Z.ld * 8, nulls[i].data, nulls[i].ld * 8, Z.ld * 8, Z.rows,
-                            hipMemcpyDeviceToDevice, (hipStream_t)mela_ctx_stream(ctx));
-                        if (e != hipSuccess) s2 = hip_fail(e, "hipMemcpy2DAsync");
+                if (!want_null) continue;
+                mela_status s2 = mela_bitmat_alloc(ctx, nulls[i].rows, nulls[i].cols, &outs[i]);
+                const mela_bitmat &Z = outs[i];
+                if (s2 == MELA_OK && Z.rows && nulls[i].cols) {
+                    // the batch view has the row pitch of the input width
+                    hipError_t e = hipMemcpy2DAsync(
+                        Z.data, Z.ld * 8, nulls[i].data, nulls[i].ld * 8, Z.ld * 8, Z.rows,
+                        hipMemcpyDeviceToDevice, (hipStream_t)mela_ctx_stream(ctx));
+                    if (e != hipSuccess) s2 = hip_fail(e, "hipMemcpy2DAsync");
+                }
+                if (s2 != MELA_OK) {
+                    sts[i] = s2;
+                    errs[i] = mela_last_error();
+                }
+            }
+            if (st == MELA_OK && want_null) {
+                mela_status s3 = mela_ctx_sync(ctx);
+                const std::string e3 = s3 == MELA_OK ? "" : mela_last_error();
+                for (size_t i = 0; i < rq.size(); ++i) {
+                    if (s3 != MELA_OK && sts[i] == MELA_OK) {
+                        sts[i] = s3;
+                        errs[i] = e3;
                     }
-                    if (s2 == MELA_OK) s2 = mela_ctx_sync(ctx);
-                    if (s2 != MELA_OK) {
-                        sts[i] = s2;
-                        errs[i] = mela_last_error();
-                        mela_bitmat_free(ctx, &Z);
+                    if (sts[i] != MELA_OK) {
+                        if (outs[i].data) mela_bitmat_free(ctx, &outs[i]);
                         continue;
                     }
-                    rq[i]->res.out = Z;
-                    rq[i]->out_bytes = Z.rows * Z.ld * 8;
+                    rq[i]->res.out = outs[i];
+                    rq[i]->out_bytes = outs[i].rows * outs[i].ld * 8;
                 }
             }
             if (st != MELA_OK)
