@@ -36,6 +36,20 @@ constexpr int SMALL_STAGE = 128;   // bytes of k per row per stage of k_mfma_sma
 constexpr int KPAD = 256;          // row pitch multiple: lcm(RING * RING_STAGE, SMALL_STAGE)
 static_assert(KPAD % (RING * RING_STAGE) == 0 && KPAD % SMALL_STAGE == 0, "pitch multiple");
 
+// Byte offset of k byte b of row i in an expanded operand of row pitch
+// `pitch`.  Row-major for k_mfma_small.  Stage-tiled for k_mfma_ring: each
+// block of XROWS rows stores its RING_STAGE-byte k stages one after another,
+// a stage being XROWS rows x RING_STAGE bytes contiguous, so each of that
+// kernel's DMA instructions reads 1 KiB of consecutive bytes (16 rows of a
+// row-major operand sit 16 rows x pitch apart, and that form fed the matrix
+// cores at 0.68x the rate of this one at 32768^3).  Writers keep each 32- or
+// 64-byte chunk inside one stage segment (b % 32 == 0).
+__device__ __forceinline__ size_t xoff(size_t i, size_t b, size_t pitch, bool tiled) {
+    if (!tiled) return i * pitch + b;
+    return (i / XROWS) * (XROWS * pitch) + (b / RING_STAGE) * (XROWS * RING_STAGE) +
+           (i % XROWS) * RING_STAGE + b % RING_STAGE;
+}
+
 // bits -> expanded bytes, one thread per 32-byte output chunk.  FP4: each
 // source bit becomes one nibble (0x0 or 0x2), two per byte (element 2i in the
 // low nibble), one source word per chunk; INT8: one byte (0 or 1), half a
@@ -75,60 +89,104 @@ k_expand(const uint64_t *__restrict__ src, size_t lds, size_t rows, size_t kw, s
     p[1] = make_uint4(o[4], o[5], o[6], o[7]);
 }
 
-// B^T expanded straight from B: one wave per 64 x 64 bit block of B (64
-// k-rows, 64 columns).  After the in-wave transpose lane c holds column
-// 64*jb + c for those 64 k values and writes their expansion to row 64*jb + c
-// of the expanded B^T.  Columns >= n and k rows >= k come out as zeros, which
-// is the padding the product kernels expect.
+// One 64-bit source word -> its 32 (FP4) or 64 (INT8) expanded bytes as
+// dwords, in the byte order of k_expand.
 template <bool FP4>
-__global__ void __launch_bounds__(256)
-k_transpose_expand(const uint64_t *__restrict__ B, size_t ldb, size_t k, size_t n, size_t kwp,
-                   uint8_t *__restrict__ dst, size_t pitch) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const size_t kb = (size_t)blockIdx.x * 4 + wave;  // 64-row block of B (k)
-    const size_t jb = blockIdx.y;                      // 64-column block of B
-    if (kb >= kwp) return;
-    const size_t i = kb * 64 + lane;
-    const size_t nw = (n + 63) >> 6;
-    uint64_t v = 0;
-    if (i < k && jb < nw) {
-        v = B[i * ldb + jb];
-        if (jb == nw - 1 && (n & 63)) v &= (1ull << (n & 63)) - 1ull;
-    }
-    v = dev::transpose64(v, lane);
-    uint4 *out = reinterpret_cast<uint4 *>(dst + (jb * 64 + lane) * pitch + kb * (FP4 ? 32 : 64));
+__device__ __forceinline__ void expand_word(uint64_t x, uint32_t *o) {
     if constexpr (FP4) {
-        uint32_t o[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            const uint32_t b = (uint32_t)(v >> (8 * q)) & 0xffu;
+            const uint32_t b = (uint32_t)(x >> (8 * q)) & 0xffu;
             uint32_t d = 0;
 #pragma unroll
             for (int t = 0; t < 8; ++t) d |= ((b >> t) & 1u) << (4 * t + 1);
             o[q] = d;
         }
-        out[0] = make_uint4(o[0], o[1], o[2], o[3]);
-        out[1] = make_uint4(o[4], o[5], o[6], o[7]);
     } else {
-        uint32_t o[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
-            const uint32_t b = (uint32_t)(v >> (4 * q)) & 0xfu;
+            const uint32_t b = (uint32_t)(x >> (4 * q)) & 0xfu;
             o[q] = (b & 1u) | ((b & 2u) << 7) | ((b & 4u) << 14) | ((b & 8u) << 21);
         }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) out[q] = make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
     }
+}
+
+// bits -> stage-tiled expanded bytes (the layout of xoff with tiled = true).
+// One thread per RING_STAGE-byte segment of one row (two source words for
+// FP4, one for INT8).  Four consecutive lanes take four consecutive segments
+// of one row, so a wave reads 16 rows x 4 segments of source with each row's
+// part contiguous and writes four runs of 16 rows x 64 bytes, each 1 KiB of
+// consecutive output.  grid.x walks segment groups (neighbouring workgroups
+// read neighbouring parts of the same rows), grid.y 64-row groups.  Rows >=
+// rows and words >= kw are written as zeros.
+template <bool FP4>
+__global__ void __launch_bounds__(256)
+k_expand_tiled(const uint64_t *__restrict__ src, size_t lds, size_t rows, size_t kw, size_t pitch,
+               uint8_t *__restrict__ dst) {
+    static_assert(XROWS % 64 == 0 && RING_STAGE == 64, "64-row groups inside a row block");
+    const int l = threadIdx.x;
+    const size_t seg = (size_t)blockIdx.x * 4 + (l & 3);
+    const size_t i = (size_t)blockIdx.y * 64 + (l >> 2);
+    constexpr int WPS = FP4 ? 2 : 1;  // source words per segment
+    uint32_t o[16];
+#pragma unroll
+    for (int h = 0; h < WPS; ++h) {
+        const size_t w = seg * WPS + h;
+        const uint64_t x = (i < rows && w < kw) ? src[i * lds + w] : 0ull;
+        expand_word<FP4>(x, o + 8 * h);
+    }
+    uint4 *p = reinterpret_cast<uint4 *>(dst + (i / XROWS) * (XROWS * pitch) +
+                                         seg * (XROWS * RING_STAGE) + (i % XROWS) * RING_STAGE);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) p[q] = make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+}
+
+// B^T expanded straight from B: each wave transposes KB 64 x 64 bit blocks
+// of B (64 k-rows each, 64 columns).  After the in-wave transpose lane c
+// holds column 64*jb + c for those k values and writes their expansion to row
+// 64*jb + c of the expanded B^T; KB = 2 for stage-tiled FP4, so that each
+// lane writes one whole 64-byte stage segment.  Columns >= n and k rows >= k
+// come out as zeros, which is the padding the product kernels expect.
+template <bool FP4, int KB>
+__global__ void __launch_bounds__(256)
+k_transpose_expand(const uint64_t *__restrict__ B, size_t ldb, size_t k, size_t n, size_t kwp,
+                   uint8_t *__restrict__ dst, size_t pitch, bool tiled) {
+    static_assert(KB == 1 || (FP4 && KB == 2), "a lane writes at most 64 bytes");
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const size_t kb0 = ((size_t)blockIdx.x * 4 + wave) * KB;  // first 64-row block of B (k)
+    const size_t jb = blockIdx.y;                               // 64-column block of B
+    if (kb0 >= kwp) return;
+    const size_t nw = (n + 63) >> 6;
+    uint32_t o[16];
+#pragma unroll
+    for (int h = 0; h < KB; ++h) {
+        const size_t i = (kb0 + h) * 64 + lane;
+        uint64_t v = 0;
+        if (i < k && jb < nw) {
+            v = B[i * ldb + jb];
+            if (jb == nw - 1 && (n & 63)) v &= (1ull << (n & 63)) - 1ull;
+        }
+        v = dev::transpose64(v, lane);
+        expand_word<FP4>(v, o + 8 * h);
+    }
+    constexpr int NQ = FP4 ? 2 * KB : 4;  // 16-byte stores per lane
+    uint4 *out = reinterpret_cast<uint4 *>(dst + xoff(jb * 64 + lane, kb0 * (FP4 ? 32 : 64), pitch, tiled));
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) out[q] = make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
 }
 
 // Workgroup -> C tile.  Workgroups are dealt to the 8 XCDs round-robin, so
 // the first step gives each XCD a contiguous range of tile indices; the
-// second walks those in groups of 8 tile rows, column by column, so the
-// workgroups resident on one XCD share A and B^T strips in its L2.
+// second walks those in groups of GR tile rows, column by column, so the
+// workgroups resident on one XCD share A and B^T strips in its L2.  For
+// k_mfma_ring (32 workgroups per XCD) GR = 4 keeps 4 x 8 tiles in flight per
+// XCD: at 32768^3 its loads alone took 8.1 ms against 9.0 ms with GR = 8
+// (the A strips are private to an XCD and are re-read half as often, the
+// B^T strips are read by every XCD at about the same time).
+template <int GR>
 __device__ __forceinline__ void tile_of(int L, int tx, int ty, int &bx, int &by) {
     const int G = tx * ty;
     if ((G & 7) == 0) L = (L & 7) * (G >> 3) + (L >> 3);
-    constexpr int GR = 8;
     const int grp = L / (GR * ty);
     const int fx = grp * GR;
     const int gs = min(tx - fx, GR);
@@ -204,7 +262,7 @@ k_mfma_small(const uint8_t *__restrict__ A, const uint8_t *__restrict__ B, size_
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
     int bx, by;
-    tile_of(blockIdx.x, tx, ty, bx, by);
+    tile_of<8>(blockIdx.x, tx, ty, bx, by);
     const int i0 = bx * ROWS, j0 = by * ROWS;
     const int lu = tid & 7, lr = tid >> 3;
     const int fsw = lu ^ ((lu & 1) << 3);
@@ -301,8 +359,10 @@ k_mfma_small(const uint8_t *__restrict__ A, const uint8_t *__restrict__ B, size_
 // k_mfma_ring: 256 x 256 tiles, 4 waves in a 2 x 2 grid, each 128 x 128
 // (4 x 4 MFMA tiles, 256 accumulator registers), one workgroup per CU.
 //
-// Operands move from HBM straight into a RING-stage LDS ring with
-// global_load_lds_dwordx4 (no staging registers), so RING-2 stages are in
+// Operands are stage-tiled (xoff): the 256 rows x RING_STAGE bytes of one
+// stage of one operand are 16 KiB of consecutive bytes.  They move from HBM
+// straight into a RING-stage LDS ring with global_load_lds_dwordx4 (no
+// staging registers, 1 KiB contiguous per instruction), so RING-2 stages are in
 // flight while the MFMAs work on the current one.  A stage holds RING_STAGE
 // bytes of k for the 256 A rows and the 256 B^T rows; each wave issues
 // exactly 2U DMA instructions per stage, so "stage s landed" is a plain
@@ -311,9 +371,11 @@ k_mfma_small(const uint8_t *__restrict__ A, const uint8_t *__restrict__ B, size_
 // compiler does not make fragment reads wait for the ring's other stages.
 //
 // Stage layout: row-major, U = RING_STAGE/16 units per row; unit p of row r
-// holds global unit p ^ g(r) with g(r) = ((r>>1) ^ (r>>3)) & 3, which keeps
-// the 16 lanes of a fragment read on distinct banks (for 32 and 64 banks)
-// while four lanes still read one 64-byte row segment.
+// holds global unit p ^ g(r) with g(r) = (r>>2) & 3, so four lanes still
+// read one 64-byte row segment.  Measured on gfx950 (ds_read_b128, four
+// waves per CU): fragment reads with this g run at the rate of 64 contiguous
+// 16-byte slots (88 B/clk/CU), while g(r) = r & 3, no swizzle and
+// ((r>>1) ^ (r>>3)) & 3 all drop to 64 B/clk.
 //
 // Step s: wait until stage s+1 landed, barrier, refill the buffer of stage
 // s-1 with stage s+RING-1, then read k-block 1 of stage s and k-block 0 of
@@ -321,7 +383,7 @@ k_mfma_small(const uint8_t *__restrict__ A, const uint8_t *__restrict__ B, size_
 // read before.  Reads and DMA issues are interleaved with the first MFMAs of
 // each block (any LDS read waits for all outstanding LDS traffic once a DMA
 // is pending, so they are issued early).
-__device__ __forceinline__ int ring_swz(int r) { return ((r >> 1) ^ (r >> 3)) & 3; }
+__device__ __forceinline__ int ring_swz(int r) { return (r >> 2) & 3; }
 
 template <bool FP4>
 __global__ void __launch_bounds__(256, 1)
@@ -337,7 +399,7 @@ k_mfma_ring(const uint8_t *__restrict__ A, const uint8_t *__restrict__ B, size_t
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
     int bx, by;
-    tile_of(blockIdx.x, tx, ty, bx, by);
+    tile_of<4>(blockIdx.x, tx, ty, bx, by);
     const int i0 = bx * ROWS, j0 = by * ROWS;
     // DMA sources: instruction i (0..U-1) of this wave fills slots
     // (i*4 + wave)*64 + lane of a stage (row S/U, position S%U)
@@ -347,13 +409,13 @@ k_mfma_ring(const uint8_t *__restrict__ A, const uint8_t *__restrict__ B, size_t
         const int S = (i * 4 + wave) * 64 + lane;
         const int r = S / U, p = S % U;
         const int u = p ^ ring_swz(r);
-        da[i] = A + (size_t)(i0 + r) * pitch + u * 16;
-        db[i] = B + (size_t)(j0 + r) * pitch + u * 16;
+        da[i] = A + (size_t)i0 * pitch + r * RING_STAGE + u * 16;
+        db[i] = B + (size_t)j0 * pitch + r * RING_STAGE + u * 16;
     }
     const int nstage = (int)(pitch / RING_STAGE);  // a multiple of RING
 #define MELA_LDS(x) ((__attribute__((address_space(3))) void *)(x))
     auto dma = [&](uint4 *buf, int st) __attribute__((always_inline)) {
-        const size_t ko = (size_t)(st < nstage ? st : nstage - 1) * RING_STAGE;
+        const size_t ko = (size_t)(st < nstage ? st : nstage - 1) * (ROWS * RING_STAGE);
 #pragma unroll
         for (int i = 0; i < U; ++i) {
             __builtin_amdgcn_global_load_lds(da[i] + ko, MELA_LDS(buf + (i * 4 + wave) * 64), 16, 0, 0);
@@ -470,24 +532,46 @@ mela_status matmul_mfma(mela_ctx *ctx, const uint64_t *A, size_t lda, const uint
     const size_t kwp = pitch / (fp4 ? 32 : 64);  // 64-bit k words the padded pitch holds
     if (np / 64 > 65535 || (kwp + 3) / 4 > 0x7fffffffu)
         return set_error(MELA_ERR_INVALID, "too many columns for the matrix-core product");
-    const dim3 gt((unsigned)((kwp + 3) / 4), (unsigned)(np / 64));
+    // 256 x 256 tiles (stage-tiled operands) once they fill every CU, else
+    // 128 x 128 tiles, two per CU (row-major operands)
+    const size_t tx2 = mp / 256, ty2 = np / 256;
+    const bool ring = tx2 * ty2 >= (size_t)ctx->cu_count;
+    // B^T: one 64 x 64 bit block per wave, two for stage-tiled fp4 (kwp is a
+    // multiple of 8)
+    const size_t kbw = ring && fp4 ? 2 : 1;
+    const dim3 gt((unsigned)((kwp + 4 * kbw - 1) / (4 * kbw)), (unsigned)(np / 64));
+    // stage-tiled A: 64 rows x 4 stage segments per workgroup (pitch is a
+    // multiple of 4 segments)
+    const dim3 gx((unsigned)(pitch / RING_STAGE / 4), (unsigned)(mp / 64));
+    if (ring && mp / 64 > 65535)
+        return set_error(MELA_ERR_INVALID, "too many rows for the matrix-core product");
     if (fp4) {
-        hipLaunchKernelGGL(k_expand<true>, dim3((unsigned)((ca + 255) / 256)), dim3(256), 0, s, A,
-                           lda, m, kw, mp, pitch, (uint8_t *)pa);
-        hipLaunchKernelGGL(k_transpose_expand<true>, gt, dim3(256), 0, s, B, ldb, k, n, kwp,
-                           (uint8_t *)pb, pitch);
+        if (ring)
+            hipLaunchKernelGGL(k_expand_tiled<true>, gx, dim3(256), 0, s, A,
+                               lda, m, kw, pitch, (uint8_t *)pa);
+        else
+            hipLaunchKernelGGL(k_expand<true>, dim3((unsigned)((ca + 255) / 256)), dim3(256), 0, s,
+                               A, lda, m, kw, mp, pitch, (uint8_t *)pa);
+        if (ring)
+            hipLaunchKernelGGL((k_transpose_expand<true, 2>), gt, dim3(256), 0, s, B, ldb, k, n, kwp,
+                               (uint8_t *)pb, pitch, true);
+        else
+            hipLaunchKernelGGL((k_transpose_expand<true, 1>), gt, dim3(256), 0, s, B, ldb, k, n, kwp,
+                               (uint8_t *)pb, pitch, false);
     } else {
-        hipLaunchKernelGGL(k_expand<false>, dim3((unsigned)((ca + 255) / 256)), dim3(256), 0, s, A,
-                           lda, m, kw, mp, pitch, (uint8_t *)pa);
-        hipLaunchKernelGGL(k_transpose_expand<false>, gt, dim3(256), 0, s, B, ldb, k, n, kwp,
-                           (uint8_t *)pb, pitch);
+        if (ring)
+            hipLaunchKernelGGL(k_expand_tiled<false>, gx, dim3(256), 0, s,
+                               A, lda, m, kw, pitch, (uint8_t *)pa);
+        else
+            hipLaunchKernelGGL(k_expand<false>, dim3((unsigned)((ca + 255) / 256)), dim3(256), 0, s,
+                               A, lda, m, kw, mp, pitch, (uint8_t *)pa);
+        hipLaunchKernelGGL((k_transpose_expand<false, 1>), gt, dim3(256), 0, s, B, ldb, k, n, kwp,
+                           (uint8_t *)pb, pitch, ring);
     }
     MELA_HIP(hipGetLastError());
     const uint8_t *xa = (const uint8_t *)pa, *xb = (const uint8_t *)pb;
-    // 256 x 256 tiles once they fill every CU, else 128 x 128 (two per CU)
-    const size_t tx2 = mp / 256, ty2 = np / 256;
     prof_mark(ctx, true);
-    if (tx2 * ty2 >= (size_t)ctx->cu_count) {
+    if (ring) {
         if (tx2 * ty2 > (size_t)INT32_MAX) return set_error(MELA_ERR_INVALID, "too many tiles for one launch");
         const dim3 grid((unsigned)(tx2 * ty2));
         if (fp4)
