@@ -173,6 +173,13 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
     auto pair_w = [&](int k) { return g + blockIdx.x * TILE + (tid + k * NT) % TILE; };
     // selected rows' words first, then the first row batch, whose loads stay
     // in flight while the tables are built
+    // the pivot-row combinations are loaded with the selected rows, so their
+    // latency is spent while the first tables are built
+    uint64_t cbv[TPT][4];
+#pragma unroll
+    for (int k = 0; k < TPT; ++k)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) cbv[k][b] = meta->comb[4 * pair_t(k) + b];
     uint64_t sq[TPT][4];
 #pragma unroll
     for (int k = 0; k < TPT; ++k) {
@@ -226,7 +233,7 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             const int c = 4 * pair_t(k) + b;
-            const uint64_t cb = ((pm >> c) & 1ull) ? meta->comb[c] : 0ull;
+            const uint64_t cb = ((pm >> c) & 1ull) ? cbv[k][b] : 0ull;
             uint64_t v = 0;
 #pragma unroll
             for (int u = 0; u < 16; ++u) v ^= T[(cb >> (4 * u)) & 15u][u][ww];
