@@ -63,8 +63,13 @@ constexpr int NSWEPT = 4096;
 __global__ void __launch_bounds__(PS_THREADS)
 k_panel_select(const uint64_t *__restrict__ M, size_t ld, uint32_t *__restrict__ act,
                ElimMeta *__restrict__ meta, int *__restrict__ pivcols, const unsigned *swept,
-               int g, int rmax) {
+               const uint64_t *__restrict__ panelw, int g, int rmax) {
     __shared__ dev::SelectLDS<PS_THREADS> L;
+    // panel word g of active-list entry threadIdx.x as the last sweep left it
+    // (it writes them when it ran, i.e. when the last selection found pivots):
+    // loaded with the metadata instead of through act[] and the row
+    const uint64_t pre = panelw[threadIdx.x];
+    const bool have_pre = meta->pm != 0;
     const int n_act = meta->n_act;
     const int r_before = meta->r;
     // no workgroup of the last sweep saw a nonzero active-row word past its
@@ -89,8 +94,8 @@ k_panel_select(const uint64_t *__restrict__ M, size_t ld, uint32_t *__restrict__
         return;
     }
     auto rowfn = [&](int q) { return M + (size_t)q * ld; };
-    const dev::PanelOut po =
-        dev::wg_panel_select<PS_THREADS>(L, act, n_act, g, rowfn, meta->comb, meta->sel_row);
+    const dev::PanelOut po = dev::wg_panel_select<PS_THREADS>(L, act, n_act, g, rowfn, meta->comb,
+                                                              meta->sel_row, have_pre, pre);
     const int tid = threadIdx.x;
     if (tid < 64 && ((po.pm >> tid) & 1ull))
         pivcols[r_before + popc64(po.pm & ((1ull << tid) - 1ull))] = g * 64 + tid;
@@ -145,7 +150,7 @@ template <int TILE, bool GE>
 __global__ void __launch_bounds__(UP_WAVES * 64, 64 / TILE)
 k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
         uint64_t *__restrict__ Q, size_t ldq, const ElimMeta *__restrict__ meta,
-        unsigned *__restrict__ swept, int g, int wc, int rref) {
+        unsigned *__restrict__ swept, uint64_t *__restrict__ panelw, int g, int wc, int rref) {
     // T[entry][table][word]: the table index lands in the ds_read immediate
     // offset, the entry index is the only per-lookup address
     __shared__ uint64_t T[16][16][TILE];
@@ -275,6 +280,13 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
             if (x[j]) rp[j][w] = a[j];
             if (q0 + j * RPW < n_act && wok) nz |= a[j];
         }
+        // the next selection's panel words, in active-list order
+        if (blockIdx.x == 0 && wl == 1 && wok)
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const int q = q0 + j * RPW;
+                if (q < n_act && q < PS_THREADS) panelw[q] = a[j];
+            }
         if (q0 + stride < total) load_rows(q0 + stride);
     }
     mark_rest_nonzero(swept, nz, g);
@@ -406,7 +418,9 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
     if ((st = scratch_get(ctx, SLOT_ELIM_M, m * ld * 8, &pM)) != MELA_OK) return st;
     if ((st = scratch_get(ctx, SLOT_ELIM_Q, rmax * ld * 8, &pQ)) != MELA_OK) return st;
     const size_t swept_off = (sizeof(ElimMeta) + (rmax + 1) * sizeof(int) + 15) & ~(size_t)15;
-    if ((st = scratch_get(ctx, SLOT_ELIM_META, swept_off + NSWEPT * sizeof(unsigned), &pMeta)) != MELA_OK)
+    const size_t panelw_off = swept_off + NSWEPT * sizeof(unsigned);
+    if ((st = scratch_get(ctx, SLOT_ELIM_META, panelw_off + PS_THREADS * sizeof(uint64_t), &pMeta)) !=
+        MELA_OK)
         return st;
     if ((st = scratch_get(ctx, SLOT_ELIM_ACT, m * sizeof(uint32_t), &pAct)) != MELA_OK) return st;
     uint64_t *Mw = (uint64_t *)pM, *Q = (uint64_t *)pQ;
@@ -414,6 +428,7 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
     int *pivcols = (int *)((char *)pMeta + sizeof(ElimMeta));
     uint32_t *act = (uint32_t *)pAct;
     unsigned *swept = (unsigned *)((char *)pMeta + swept_off);
+    uint64_t *panelw = (uint64_t *)((char *)pMeta + panelw_off);
 
     const size_t tot = m * wc;
     hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, A, lda,
@@ -434,7 +449,7 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
         if (zs != MELA_OK) return zs;
         for (size_t g = 0; g < wc; ++g) {
             hipLaunchKernelGGL(k_panel_select, dim3(1), dim3(PS_THREADS), 0, s, Mw, ld, act, meta,
-                               pivcols, swept, (int)g, (int)rmax);
+                               pivcols, swept, panelw, (int)g, (int)rmax);
             {
                 // the sweep forms the panel's pivot rows of its tile itself;
                 // tiles start at word g, rows strided over gridDim.y, one
@@ -449,11 +464,11 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
                 if (ty < 1) ty = 1;
                 if (variant == MELA_ELIM_GE)
                     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sweep<SWEEP_TILE, true>), dim3(tx, ty),
-                                       dim3(UP_WAVES * 64), 0, s, Mw, ld, act, Q, ld, meta, swept, (int)g,
+                                       dim3(UP_WAVES * 64), 0, s, Mw, ld, act, Q, ld, meta, swept, panelw, (int)g,
                                        (int)wc, rref ? 1 : 0);
                 else
                     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sweep<SWEEP_TILE, false>), dim3(tx, ty),
-                                       dim3(UP_WAVES * 64), 0, s, Mw, ld, act, Q, ld, meta, swept, (int)g,
+                                       dim3(UP_WAVES * 64), 0, s, Mw, ld, act, Q, ld, meta, swept, panelw, (int)g,
                                        (int)wc, rref ? 1 : 0);
             }
             if (rref)

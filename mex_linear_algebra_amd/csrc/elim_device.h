@@ -146,9 +146,12 @@ struct PanelOut {
 // selrow[0..rho).  All NT threads of the workgroup must call it.
 //   row(q)   : pointer to the row of active-list entry q's row id
 //   act      : active list (global or LDS), ACT = its element type
+//   pre      : if have_pre, thread t's panel word of active-list entry t
+//              (t < min(n_act, NT)), already loaded by the caller
 template <int NT, typename ACT, typename RowFn>
 __device__ PanelOut wg_panel_select(SelectLDS<NT> &L, ACT *act, int n_act, int g, RowFn row,
-                                    uint64_t *comb_out, int *selrow_out) {
+                                    uint64_t *comb_out, int *selrow_out, bool have_pre = false,
+                                    uint64_t pre = 0) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid < 64) L.bas[tid] = 0;
     if (tid == 0) {
@@ -162,7 +165,7 @@ __device__ PanelOut wg_panel_select(SelectLDS<NT> &L, ACT *act, int n_act, int g
     // panel that does not reach 64 pivots this way takes the general path.
     const int n0 = n_act < NT ? n_act : NT;
     if (n_act >= 64) {
-        if (tid < n0) L.cy[tid] = row((int)act[tid])[g];
+        if (tid < n0) L.cy[tid] = have_pre ? pre : row((int)act[tid])[g];
         __syncthreads();
         // the rank of the first 64 words is at most the number of nonzero
         // ones: with fewer than FAST_MIN_PIVOTS the general path runs at once
@@ -216,7 +219,8 @@ __device__ PanelOut wg_panel_select(SelectLDS<NT> &L, ACT *act, int n_act, int g
     // chunks of NT active rows; the next chunk's words are loaded while the
     // current chunk is reduced (positions past the end re-read a valid row)
     uint64_t xnext = 0;
-    if (!L.fast && n_act > 0) xnext = row((int)act[tid < n_act ? tid : n_act - 1])[g];
+    if (!L.fast && n_act > 0)
+        xnext = have_pre && tid < n_act ? pre : row((int)act[tid < n_act ? tid : n_act - 1])[g];
     for (int base = 0; base < n_act && !L.fast; base += NT) {
         const int pos = base + tid;
         uint64_t x = pos < n_act ? xnext : 0ull;
