@@ -506,7 +506,7 @@ def main():
                     default=[50, 100, 200, 400, 800, 1600, 3200, 6400, 12800])
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-pool", type=int, default=48)
-    ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--serve-workers", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1"}))
