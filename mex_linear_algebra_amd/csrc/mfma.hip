@@ -380,9 +380,8 @@ k_mfma_small(const uint8_t *__restrict__ A, const uint8_t *__restrict__ B, size_
 // Step s: wait until stage s+1 landed, barrier, refill the buffer of stage
 // s-1 with stage s+RING-1, then read k-block 1 of stage s and k-block 0 of
 // stage s+1 from LDS into registers while the MFMAs consume the fragments
-// read before.  Reads and DMA issues are interleaved with the first MFMAs of
-// each block (any LDS read waits for all outstanding LDS traffic once a DMA
-// is pending, so they are issued early).
+// read before.  Fragment reads are interleaved with the first MFMAs of each
+// block, the refill's DMA instructions with both blocks.
 __device__ __forceinline__ int ring_swz(int r) { return (r >> 2) & 3; }
 
 template <bool FP4>
@@ -414,10 +413,11 @@ k_mfma_ring(const uint8_t *__restrict__ A, const uint8_t *__restrict__ B, size_t
     }
     const int nstage = (int)(pitch / RING_STAGE);  // a multiple of RING
 #define MELA_LDS(x) ((__attribute__((address_space(3))) void *)(x))
-    auto dma = [&](uint4 *buf, int st) __attribute__((always_inline)) {
+    // DMA instructions i0 .. i1-1 (of U) of stage st for both operands
+    auto dma = [&](uint4 *buf, int st, int i0, int i1) __attribute__((always_inline)) {
         const size_t ko = (size_t)(st < nstage ? st : nstage - 1) * (ROWS * RING_STAGE);
 #pragma unroll
-        for (int i = 0; i < U; ++i) {
+        for (int i = i0; i < i1; ++i) {
             __builtin_amdgcn_global_load_lds(da[i] + ko, MELA_LDS(buf + (i * 4 + wave) * 64), 16, 0, 0);
             __builtin_amdgcn_global_load_lds(db[i] + ko, MELA_LDS(buf + SS + (i * 4 + wave) * 64), 16, 0, 0);
         }
@@ -461,23 +461,28 @@ k_mfma_ring(const uint8_t *__restrict__ A, const uint8_t *__restrict__ B, size_t
     constexpr int WAIT0 = (V0 & 0xF) | ((V0 >> 4) << 14) | (7 << 4) | (0xF << 8);
     uint4 fa0[TR], fb0[TR], fa1[TR], fb1[TR];
     // step st: fa0/fb0 hold k-block 0 of stage st
+    // The refill's 2U DMA instructions are spread over both halves of the
+    // step, one after every second MFMA of the interleave, rather than issued
+    // in one burst: a wave stalls at issue once the address unit's queue is
+    // full, and its MFMAs wait behind it.
     auto step = [&](const uint4 *cur, const uint4 *nx1, uint4 *refill, int st) __attribute__((always_inline)) {
         __builtin_amdgcn_s_waitcnt(WAIT);
         __builtin_amdgcn_s_barrier();
         __builtin_amdgcn_sched_barrier(0);
-        dma(refill, st + RING - 1);
+        dma(refill, st + RING - 1, 0, U / 2);
         rdA(cur, 1, fa1);
         rdB(cur, 1, fb1);
         mmh(fa0, fb0, 0);
         mmh(fa0, fb0, 2);
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
-            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // LDS read
-            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read (DMA)
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                 // MFMA
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);                 // LDS read
+            if (i % 2 == 1) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0); // VMEM read (DMA)
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
         }
-        __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);
         __builtin_amdgcn_sched_barrier(0);
+        dma(refill, st + RING - 1, U / 2, U);
         rdA(nx1, 0, fa0);
         rdB(nx1, 0, fb0);
         mmh(fa1, fb1, 0);
@@ -486,13 +491,14 @@ k_mfma_ring(const uint8_t *__restrict__ A, const uint8_t *__restrict__ B, size_t
         for (int i = 0; i < 8; ++i) {
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 1);
             __builtin_amdgcn_sched_group_barrier(0x100, 1, 1);
+            if (i % 2 == 1) __builtin_amdgcn_sched_group_barrier(0x020, 1, 1);
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 1);
         }
-        __builtin_amdgcn_sched_group_barrier(0x008, 8, 1);
         __builtin_amdgcn_sched_barrier(0);
     };
-    dma(r0, 0);
-    dma(r1, 1);
-    dma(r2, 2);
+    dma(r0, 0, 0, U);
+    dma(r1, 1, 0, U);
+    dma(r2, 2, 0, U);
     __builtin_amdgcn_s_waitcnt(WAIT0);
     __builtin_amdgcn_s_barrier();
     __builtin_amdgcn_sched_barrier(0);
