@@ -100,3 +100,26 @@ def test_wait_zero_timeout():
     # not finished yet (TIMEOUT) or already failed for want of a device
     assert ei.value.status in ("TIMEOUT",) + FAILED
     s.close()
+
+
+def test_routing_by_batch_member_cost():
+    # the scheduler batches a rank / null request only while one workgroup of
+    # the persistent batch kernel finishes it about as fast as the
+    # single-matrix path; the device never exists here, so every request
+    # fails, but the routing is counted all the same
+    s = mela.Server(device=NO_DEVICE, hbm_budget=1 << 34, workers=1)
+    cases = [
+        (mela.OP_RANK, 256, 256, True),     # small: batched
+        (mela.OP_RANK, 128, 8192, True),    # short and wide rank: batched
+        (mela.OP_RANK, 4096, 8192, False),  # one workgroup would take ~28 ms
+        (mela.OP_NULL, 128, 8192, False),   # 8192 x 8064 null basis from one workgroup
+        (mela.OP_RANK, 5000, 100, False),   # beyond the batch kernel's 4096 rows
+    ]
+    for op, m, n, _ in cases:
+        t = s.submit(op, fake(m, n))
+        with pytest.raises(mela.MelaError):
+            s.wait(t, 30.0)
+    st = s.stats()
+    assert st["batched_requests"] == sum(1 for c in cases if c[3])
+    assert st["failed"] == len(cases)
+    s.close()
