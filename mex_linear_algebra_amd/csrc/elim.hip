@@ -65,6 +65,7 @@ k_panel_select(const uint64_t *__restrict__ M, size_t ld, uint32_t *__restrict__
                ElimMeta *__restrict__ meta, int *__restrict__ pivcols, const unsigned *swept,
                const uint64_t *__restrict__ panelw, int g, int rmax) {
     __shared__ dev::SelectLDS<PS_THREADS> L;
+    __shared__ uint32_t actc[dev::ACT_HEAD + 64];
     // panel word g of active-list entry threadIdx.x as the last sweep left it
     // (it writes them when it ran, i.e. when the last selection found pivots):
     // loaded with the metadata instead of through act[] and the row
@@ -95,7 +96,7 @@ k_panel_select(const uint64_t *__restrict__ M, size_t ld, uint32_t *__restrict__
     }
     auto rowfn = [&](int q) { return M + (size_t)q * ld; };
     const dev::PanelOut po = dev::wg_panel_select<PS_THREADS>(L, act, n_act, g, rowfn, meta->comb,
-                                                              meta->sel_row, have_pre, pre);
+                                                              meta->sel_row, have_pre, pre, actc);
     const int tid = threadIdx.x;
     if (tid < 64 && ((po.pm >> tid) & 1ull))
         pivcols[r_before + popc64(po.pm & ((1ull << tid) - 1ull))] = g * 64 + tid;

@@ -148,10 +148,17 @@ struct PanelOut {
 //   act      : active list (global or LDS), ACT = its element type
 //   pre      : if have_pre, thread t's panel word of active-list entry t
 //              (t < min(n_act, NT)), already loaded by the caller
+//   actc     : optional LDS copy of act (ACT_HEAD + 64 entries) for a global
+//              act: waves 1-4 fill it with act[0 .. ACT_HEAD) and the last 64
+//              entries while wave 0 runs the fast path, so the final
+//              reordering reads the selected and the moved entries from LDS
+constexpr int ACT_HEAD = 192;  // positions the fast path can select (64 + 128)
 template <int NT, typename ACT, typename RowFn>
 __device__ PanelOut wg_panel_select(SelectLDS<NT> &L, ACT *act, int n_act, int g, RowFn row,
                                     uint64_t *comb_out, int *selrow_out, bool have_pre = false,
-                                    uint64_t pre = 0) {
+                                    uint64_t pre = 0, uint32_t *actc = nullptr) {
+    // the copy needs waves 1-4 (workgroups of at least 320 threads)
+    if (NT < 64 + ACT_HEAD + 64) actc = nullptr;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid < 64) L.bas[tid] = 0;
     if (tid == 0) {
@@ -179,7 +186,7 @@ __device__ PanelOut wg_panel_select(SelectLDS<NT> &L, ACT *act, int n_act, int g
             int slotpos = lane;              // active-list position of slot `lane`
             // single insertions only when the first 64 rows came close to 64
             // pivots (sparse panels would scan 128 rows and then fail)
-            const int lim = popc64(pivl) < FAST_MIN_PIVOTS ? 0 : n0 < 64 + 128 ? n0 : 64 + 128;
+            const int lim = popc64(pivl) < FAST_MIN_PIVOTS ? 0 : n0 < ACT_HEAD ? n0 : ACT_HEAD;
             for (int pos = 64; pos < lim && pivl != ~0ull; ++pos) {
                 const uint64_t x = L.cy[pos];
                 if (x == 0) continue;
@@ -213,9 +220,15 @@ __device__ PanelOut wg_panel_select(SelectLDS<NT> &L, ACT *act, int n_act, int g
                     L.fast = 1;
                 }
             }
+        } else if (actc && tid >= 64 && tid < 64 + ACT_HEAD + 64) {
+            const int k = tid - 64;
+            const int p = k < ACT_HEAD ? k : n_act - 64 + (k - ACT_HEAD);
+            if (p < n_act) actc[k] = (uint32_t)act[p];
         }
         __syncthreads();
     }
+    // the LDS copy of act is filled when the block above ran (n_act >= 64)
+    const bool have_actc = actc != nullptr && n_act >= 64;
     // chunks of NT active rows; the next chunk's words are loaded while the
     // current chunk is reduced (positions past the end re-read a valid row)
     uint64_t xnext = 0;
@@ -309,7 +322,7 @@ __device__ PanelOut wg_panel_select(SelectLDS<NT> &L, ACT *act, int n_act, int g
         int my_pos = -1;
         if (lane < nsel) {
             my_pos = L.selpos[lane];
-            L.selrow[lane] = (int)act[my_pos];
+            L.selrow[lane] = have_actc && my_pos < ACT_HEAD ? (int)actc[my_pos] : (int)act[my_pos];
         }
         L.flag[lane] = 0;
         __threadfence_block();
@@ -319,7 +332,10 @@ __device__ PanelOut wg_panel_select(SelectLDS<NT> &L, ACT *act, int n_act, int g
         const bool is_mover = lane < nsel && L.flag[lane] == 0;
         const uint64_t hb = __ballot(is_hole), mb = __ballot(is_mover);
         const uint64_t below = (1ull << lane) - 1ull;
-        if (is_mover) L.mv[popc64(mb & below)] = (uint32_t)act[n_new + lane];
+        // position n_new + lane is tail entry lane + 64 - nsel of the copy
+        if (is_mover)
+            L.mv[popc64(mb & below)] =
+                have_actc ? actc[ACT_HEAD + lane + 64 - nsel] : (uint32_t)act[n_new + lane];
         __threadfence_block();
         if (is_hole) act[my_pos] = (ACT)L.mv[popc64(hb & below)];
         if (lane < nsel) selrow_out[lane] = L.selrow[lane];
