@@ -111,7 +111,8 @@ def test_matmul_packed_algos(mela, ctx, orc, algo, shape):
 
 
 @pytest.mark.parametrize("algo", [0, 3, 4])
-@pytest.mark.parametrize("shape", [(4100, 1000, 4000), (4096, 4160, 4293), (4352, 64, 4097)])
+@pytest.mark.parametrize("shape", [(4100, 1000, 4000), (4096, 4160, 4293), (4352, 64, 4097),
+                                   (4100, 1, 4100), (4096, 129, 4352)])
 def test_matmul_mfma_large_tiles_exact(mela, ctx, orc, algo, shape):
     # grids of >= 256 tiles of 256 x 256 take the LDS-ring kernel; ragged m,
     # k and n exercise the zero padding and the last-word mask
