@@ -20,8 +20,10 @@
 //                   the panel's pivot columns from every active row (and, for
 //                   RREF, from every earlier pivot row) with one table XOR per
 //                   4 pivot columns (M4RI) or one XOR per pivot row (GE),
-//                   sweeping words g+1 .. wc-1;
-//   k_update_panelword (RREF) word g of the earlier pivot rows.
+//                   sweeping words g+1 .. wc-1; for RREF its tile-0 column
+//                   also brings word g-1 of the earlier pivot rows up to date
+//                   with panel g-1's pivot rows;
+//   k_update_panelword (RREF) the same for the last panel's word.
 #include "elim_device.h"
 
 namespace mela {
@@ -43,6 +45,8 @@ struct ElimMeta {
     uint64_t pm;        // pivot bits (columns 64g + c) of the current panel
     int finished;       // every active row is zero: the remaining panels have no pivot
     int last_swept;     // last panel with pivots (its sweep ran), -1 before the first
+    uint64_t pm_prev;   // pm and r0 of the previous panel (RREF: its word update is
+    int r0_prev;        // applied by the next panel's sweep)
     int sel_row[64];           // selected row ids (selection slot -> row)
     uint64_t comb[64];  // by pivot bit c: selection slots XORed into pivot row c
 };
@@ -70,7 +74,9 @@ k_panel_select(const uint64_t *__restrict__ M, size_t ld, uint32_t *__restrict__
     // (it writes them when it ran, i.e. when the last selection found pivots):
     // loaded with the metadata instead of through act[] and the row
     const uint64_t pre = panelw[threadIdx.x];
-    const bool have_pre = meta->pm != 0;
+    const uint64_t pm_old = meta->pm;
+    const int r0_old = meta->r0;
+    const bool have_pre = pm_old != 0;
     const int n_act = meta->n_act;
     const int r_before = meta->r;
     // no workgroup of the last sweep saw a nonzero active-row word past its
@@ -87,6 +93,8 @@ k_panel_select(const uint64_t *__restrict__ M, size_t ld, uint32_t *__restrict__
     }
     if (n_act == 0 || r_before >= rmax || finished) {
         if (threadIdx.x == 0) {
+            meta->pm_prev = pm_old;
+            meta->r0_prev = r0_old;
             meta->r0 = r_before;
             meta->rho = 0;
             meta->pm = 0;
@@ -101,6 +109,8 @@ k_panel_select(const uint64_t *__restrict__ M, size_t ld, uint32_t *__restrict__
     if (tid < 64 && ((po.pm >> tid) & 1ull))
         pivcols[r_before + popc64(po.pm & ((1ull << tid) - 1ull))] = g * 64 + tid;
     if (tid == 0) {
+        meta->pm_prev = pm_old;
+        meta->r0_prev = r0_old;
         meta->r0 = r_before;
         meta->rho = po.rho;
         meta->pm = po.pm;
@@ -145,8 +155,9 @@ __device__ __forceinline__ void lds_barrier() {
 // g+1 .. wc-1 (64 / TILE rows per wave instruction): M4RI XORs 16 entries of
 // the pivot rows' 4-bit tables (the four pivot rows 4t..4t+3 a thread holds
 // give table t; slot c = pivot bit c, zero where c is not a pivot), GE XORs
-// the pivot row of every set bit of its panel word.  Word g is left to
-// k_update_panelword so that every tile reads an unmodified multiplier.
+// the pivot row of every set bit of its panel word.  Word g of the earlier
+// pivot rows is left to the next panel's sweep (or, for the last panel, to
+// k_update_panelword) so that every tile reads an unmodified multiplier.
 template <int TILE, bool GE>
 __global__ void __launch_bounds__(UP_WAVES * 64, 64 / TILE)
 k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
@@ -159,11 +170,40 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
     constexpr int RPW = 64 / TILE;          // rows per wave instruction
     constexpr int R = UP_ROWS / RPW;        // row batch per lane
     constexpr int TPT = 16 * TILE / NT;     // (table, word) pairs per thread
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // RREF: word g-1 of the pivot rows found before panel g-1 still lacks
+    // panel g-1's pivot rows (that sweep left word g-1 alone so that every
+    // tile read an unmodified multiplier).  The tile-0 column applies them
+    // here, where nothing else reads or writes that word; the loads go out
+    // first and the update is done on the way out.
+    __shared__ uint64_t PWd[64];
+    const uint64_t pmp = rref && blockIdx.x == 0 && g > 0 ? meta->pm_prev : 0ull;
+    const int rp0 = meta->r0_prev;
+    const int dj0 = (int)blockIdx.y * NT + tid, dstride = (int)gridDim.y * NT;
+    uint64_t dpw = 0, dx = 0;
+    if (pmp) {
+        if (tid < 64 && ((pmp >> tid) & 1ull))
+            dpw = Q[(size_t)(rp0 + popc64(pmp & ((1ull << tid) - 1ull))) * ldq + (g - 1)];
+        if (dj0 < rp0) dx = Q[(size_t)dj0 * ldq + (g - 1)];
+    }
+    auto finish_deferred = [&]() {  // every thread of the workgroup
+        if (!pmp) return;
+        if (tid < 64) PWd[tid] = dpw;
+        __syncthreads();
+        for (int j = dj0, it = 0; j < rp0; j += dstride, ++it) {
+            const uint64_t xv = it == 0 ? dx : Q[(size_t)j * ldq + (g - 1)];
+            uint64_t av = xv;
+            for (uint64_t h = xv & pmp; h; h &= h - 1) av ^= PWd[ffs64(h)];
+            Q[(size_t)j * ldq + (g - 1)] = av;
+        }
+    };
     const uint64_t pm = meta->pm;
-    if (pm == 0) return;
+    if (pm == 0) {
+        finish_deferred();
+        return;
+    }
     const int n_act = meta->n_act, r0 = meta->r0, rho = meta->rho;
     const int total = n_act + (rref ? r0 : 0);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wl = lane % TILE, sub = lane / TILE;
     const int w = g + blockIdx.x * TILE + wl;
     const bool win = w < wc;            // a word of the matrix
@@ -173,7 +213,10 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
     const int stride = gridDim.y * UP_WAVES * per_wave;
     int q0 = (blockIdx.y * UP_WAVES + wave) * per_wave + sub;
     const bool rows_here = (int)blockIdx.y * UP_WAVES * per_wave < total;
-    if (!rows_here && blockIdx.y != 0) return;
+    if (!rows_here && blockIdx.y != 0) {
+        finish_deferred();
+        return;
+    }
     // (table, word) pairs of this thread: table e / TILE, word e % TILE
     auto pair_t = [&](int k) { return (tid + k * NT) / TILE; };
     auto pair_w = [&](int k) { return g + blockIdx.x * TILE + (tid + k * NT) % TILE; };
@@ -249,7 +292,10 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
         }
     }
     lds_barrier();
-    if (!rows_here) return;
+    if (!rows_here) {
+        finish_deferred();
+        return;
+    }
     // GE: the pivot rows themselves, by pivot bit, over the tables' LDS
     uint64_t(*P)[TILE] = reinterpret_cast<uint64_t(*)[TILE]>(&T[0][0][0]);
     if (GE) {
@@ -291,6 +337,7 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
         if (q0 + stride < total) load_rows(q0 + stride);
     }
     mark_rest_nonzero(swept, nz, g);
+    finish_deferred();
 }
 
 // RREF only: word g of the earlier pivot rows (active rows' word g is dead),
@@ -327,6 +374,8 @@ __global__ void k_elim_init(ElimMeta *meta, uint32_t *act, int m) {
         meta->pm = 0;
         meta->finished = 0;
         meta->last_swept = -1;
+        meta->pm_prev = 0;
+        meta->r0_prev = 0;
     }
 }
 
@@ -472,10 +521,12 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
                                        dim3(UP_WAVES * 64), 0, s, Mw, ld, act, Q, ld, meta, swept, panelw, (int)g,
                                        (int)wc, rref ? 1 : 0);
             }
-            if (rref)
-                hipLaunchKernelGGL(k_update_panelword, dim3((unsigned)((rmax + 255) / 256)),
-                                   dim3(256), 0, s, Q, ld, meta, (int)g);
         }
+        // RREF: word g of the earlier pivot rows is brought up to date by the
+        // next panel's sweep; the last panel's word here
+        if (rref)
+            hipLaunchKernelGGL(k_update_panelword, dim3((unsigned)((rmax + 255) / 256)), dim3(256),
+                               0, s, Q, ld, meta, (int)(wc - 1));
         MELA_HIP(hipGetLastError());
         return MELA_OK;
     };
