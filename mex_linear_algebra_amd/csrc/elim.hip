@@ -146,7 +146,8 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 // Sweep of panel g, pivot rows included: the workgroup's TILE-word tile starts
-// at word g (TILE 64: 128 KiB of tables, one workgroup per CU; TILE 32: two).
+// at word g rounded down to 16 words (TILE 64: 128 KiB of tables, one
+// workgroup per CU; TILE 32: two).
 // Each thread owns TILE/32 (table t, word) pairs: it loads the selected rows
 // 4t..4t+3 of its word and builds their 4-bit table t; pivot row c = XOR of
 // the selected rows in comb[c] is then 16 lookups.  Row-stripe 0 of each
@@ -205,7 +206,11 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
     const int n_act = meta->n_act, r0 = meta->r0, rho = meta->rho;
     const int total = n_act + (rref ? r0 : 0);
     const int wl = lane % TILE, sub = lane / TILE;
-    const int w = g + blockIdx.x * TILE + wl;
+    // tiles start at word g rounded down to a 128-byte line (16 words), so
+    // every row segment a tile reads or writes covers whole lines; words
+    // below g+1 ride along unchanged
+    const int g0 = g & ~15;
+    const int w = g0 + blockIdx.x * TILE + wl;
     const bool win = w < wc;            // a word of the matrix
     const bool wok = win && w > g;      // a word the sweep updates
     const int wsafe = win ? w : g;      // loads of words past the end re-read word g
@@ -219,7 +224,7 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
     }
     // (table, word) pairs of this thread: table e / TILE, word e % TILE
     auto pair_t = [&](int k) { return (tid + k * NT) / TILE; };
-    auto pair_w = [&](int k) { return g + blockIdx.x * TILE + (tid + k * NT) % TILE; };
+    auto pair_w = [&](int k) { return g0 + blockIdx.x * TILE + (tid + k * NT) % TILE; };
     // selected rows' words first, then the first row batch, whose loads stay
     // in flight while the tables are built
     // the pivot-row combinations are loaded with the selected rows, so their
@@ -287,7 +292,7 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
 #pragma unroll
             for (int u = 0; u < 16; ++u) v ^= T[(cb >> (4 * u)) & 15u][u][ww];
             pv[k][b] = v;
-            if (blockIdx.y == 0 && pair_w(k) < wc && ((pm >> c) & 1ull))
+            if (blockIdx.y == 0 && pair_w(k) >= g && pair_w(k) < wc && ((pm >> c) & 1ull))
                 Q[(size_t)(r0 + popc64(pm & ((1ull << c) - 1ull))) * ldq + pair_w(k)] = v;
         }
     }
@@ -328,7 +333,7 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
             if (q0 + j * RPW < n_act && wok) nz |= a[j];
         }
         // the next selection's panel words, in active-list order
-        if (blockIdx.x == 0 && wl == 1 && wok)
+        if (blockIdx.x == 0 && w == g + 1 && wok)
 #pragma unroll
             for (int j = 0; j < R; ++j) {
                 const int q = q0 + j * RPW;
@@ -457,12 +462,14 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
     if (Q_out) *Q_out = nullptr;
     const size_t wc = words64(n);
     const size_t rmax = m < n ? m : n;
-    if (ldq_out) *ldq_out = wc ? wc : 1;
+    // working rows (and the pivot rows Q) start on 128-byte lines, so the
+    // sweep's line-aligned tiles stay aligned in every row
+    const size_t ld = (wc + 15) & ~(size_t)15;
+    if (ldq_out) *ldq_out = ld ? ld : 1;
     if (m == 0 || n == 0) return MELA_OK;
     if (m > (size_t)INT32_MAX || wc > (size_t)INT32_MAX)
         return set_error(MELA_ERR_INVALID, "matrix dimension exceeds 2^31-1");
 
-    const size_t ld = wc;
     void *pM = nullptr, *pQ = nullptr, *pMeta = nullptr, *pAct = nullptr;
     mela_status st;
     if ((st = scratch_get(ctx, SLOT_ELIM_M, m * ld * 8, &pM)) != MELA_OK) return st;
@@ -502,9 +509,10 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
                                pivcols, swept, panelw, (int)g, (int)rmax);
             {
                 // the sweep forms the panel's pivot rows of its tile itself;
-                // tiles start at word g, rows strided over gridDim.y, one
-                // round of the resident workgroups (64 / TILE per CU)
-                const size_t rem = wc - g;
+                // tiles start at word g rounded down to 16, rows strided over
+                // gridDim.y, one round of the resident workgroups (64 / TILE
+                // per CU)
+                const size_t rem = wc - (g & ~(size_t)15);
                 const unsigned tx = (unsigned)((rem + SWEEP_TILE - 1) / SWEEP_TILE);
                 const size_t rows_hint = m + (rref ? rmax : 0);
                 unsigned ty = (unsigned)(((64 / SWEEP_TILE) * cu + tx - 1) / tx);
