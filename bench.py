@@ -315,6 +315,14 @@ def bench_batch(args, mela):
     ctx.sync()
     plain_s = (time.perf_counter() - t0) / len(sidx)
     plain_rate = 1.0 / plain_s
+    # sweep bytes of rank + null basis (RREF rows = m) with each matrix's
+    # pivots taken in its leading columns (the most words a sweep can cover),
+    # plus the packed null basis written
+    bytes_alg = 0
+    for (P, m, n, _), r in zip(items, ranks):
+        wc = (n + 63) // 64
+        bytes_alg += sum(16 * m * (wc - j) for j in range((r + 63) // 64))
+        bytes_alg += n * ((n - r + 63) // 64) * 8
     print(json.dumps({
         "workload": f"batched rank+null, {len(items)} uneven matrices (m log-uniform 64..2048, "
                     f"n=m+U[0,m], 90% column-weight 3/4/6, 10% dense)",
@@ -322,7 +330,10 @@ def bench_batch(args, mela):
         "matrices_per_s": round(rate, 1), "autotune_best_ms": round(tuned_ms, 3),
         "plain_one_per_launch_matrices_per_s": round(plain_rate, 1),
         "plain_sample": len(sidx), "speedup_vs_plain": round(rate / plain_rate, 2),
-        "generate_s": round(gen_s, 2), "mean_rank": sum(ranks) / len(ranks)}), flush=True)
+        "generate_s": round(gen_s, 2), "mean_rank": sum(ranks) / len(ranks),
+        "bytes_alg_GB": round(bytes_alg / 1e9, 3),
+        "GBps_alg": round(bytes_alg / (dev_ms * 1e-3) / 1e9, 1),
+        "hbm_frac": round(bytes_alg / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}), flush=True)
     b.close()
     for M in dev:
         ctx.free(M)
