@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "common.h"
+#include "hostpack.h"
 
 using namespace mela;
 
@@ -37,29 +38,78 @@ struct Guard {
     explicit Guard(mela_ctx *c) : lk(c->mu) { (void)hipSetDevice(c->device); }
 };
 
-// Copies a host column-major array to the device and packs it into `slot`.
+// Bytes of the packed transpose (one row of words64(rows) words per column)
+// of a rows x cols host array; 0 when the array has no element.
+size_t staged_bytes(size_t rows, size_t cols) {
+    return rows && cols ? cols * words64(rows) * 8 : 0;
+}
+
+// Packs a host column-major array into `packed_slot` (row-major, ld words).
+// The host packs its columns into the pinned `stage` (staged_bytes(rows,
+// cols) long) in chunks, each chunk's copy to `raw_slot` overlapping the
+// packing of the next; one bit-matrix transpose on the device then gives the
+// row-major layout.  PCIe carries one bit per element instead of the
+// element's 8 (logical) or 64 (double).
 mela_status upload_pack(mela_ctx *ctx, const void *host, int cls, size_t rows, size_t cols,
-                        int raw_slot, int packed_slot, uint64_t **out, size_t *ld_out) {
+                        uint8_t *stage, int raw_slot, int packed_slot, uint64_t **out,
+                        size_t *ld_out) {
     const size_t ld = words64(cols) ? words64(cols) : 1;
     size_t n_el, bytes;
     if (mul_overflows(rows, cols, &n_el) || mul_overflows(n_el, elem_size(cls), &bytes))
         return set_error(MELA_ERR_INVALID, "matrix too large");
-    void *raw = nullptr, *packed = nullptr;
+    void *packed = nullptr;
     mela_status st = scratch_get(ctx, packed_slot, (rows ? rows : 1) * ld * 8, &packed);
     if (st != MELA_OK) return st;
-    if (bytes) {
-        if ((st = scratch_get(ctx, raw_slot, bytes, &raw)) != MELA_OK) return st;
-        MELA_HIP(hipMemcpyAsync(raw, host, bytes, hipMemcpyHostToDevice, ctx->stream));
-    }
-    if (rows) {
-        if (cols)
-            st = launch_pack(ctx->stream, raw, cls & 0xff, rows, cols, (uint64_t *)packed, ld);
-        else
-            st = launch_fill_zero(ctx->stream, packed, rows * ld * 8);
+    if (rows && cols) {
+        const size_t ldt = words64(rows);
+        void *raw = nullptr;
+        if ((st = scratch_get(ctx, raw_slot, staged_bytes(rows, cols), &raw)) != MELA_OK)
+            return st;
+        // about eight chunks, each at least 8 MiB of the host array
+        const size_t col_bytes = rows * elem_size(cls);
+        size_t chunk = (cols + 7) / 8;
+        const size_t min_cols = (((size_t)8 << 20) + col_bytes - 1) / col_bytes;
+        if (chunk < min_cols) chunk = min_cols;
+        for (size_t j0 = 0; j0 < cols; j0 += chunk) {
+            const size_t nc = cols - j0 < chunk ? cols - j0 : chunk;
+            uint64_t *dst = (uint64_t *)stage + j0 * ldt;
+            host_pack_columns((const uint8_t *)host + j0 * col_bytes, cls & 0xff, rows, nc, dst);
+            MELA_HIP(hipMemcpyAsync((uint64_t *)raw + j0 * ldt, dst, nc * ldt * 8,
+                                    hipMemcpyHostToDevice, ctx->stream));
+        }
+        st = launch_transpose(ctx->stream, (const uint64_t *)raw, ldt, cols, rows,
+                              (uint64_t *)packed, ld);
         if (st != MELA_OK) return st;
+    } else if (rows) {
+        if ((st = launch_fill_zero(ctx->stream, packed, rows * ld * 8)) != MELA_OK) return st;
     }
     *out = (uint64_t *)packed;
     *ld_out = ld;
+    return MELA_OK;
+}
+
+// The rows x cols packed matrix P (row-major, ldp words) to the host as a
+// column-major logical array: the device transposes it into `slot`, the
+// packed columns come back through the pinned buffer and the host spreads
+// them into bytes.  Returns with the stream idle.
+mela_status download_unpack(mela_ctx *ctx, const uint64_t *P, size_t ldp, size_t rows,
+                            size_t cols, int slot, uint8_t *host) {
+    if (rows && cols) {
+        const size_t ldt = words64(rows), tb = staged_bytes(rows, cols);
+        void *t = nullptr;
+        uint8_t *pin = nullptr;
+        mela_status st = scratch_get(ctx, slot, tb, &t);
+        if (st != MELA_OK) return st;
+        if ((st = launch_transpose(ctx->stream, P, ldp, rows, cols, (uint64_t *)t, ldt)) !=
+            MELA_OK)
+            return st;
+        if ((st = pinned_get(ctx, tb, &pin)) != MELA_OK) return st;
+        MELA_HIP(hipMemcpyAsync(pin, t, tb, hipMemcpyDeviceToHost, ctx->stream));
+        MELA_HIP(hipStreamSynchronize(ctx->stream));
+        host_unpack_columns((const uint64_t *)pin, rows, cols, host);
+        return MELA_OK;
+    }
+    MELA_HIP(hipStreamSynchronize(ctx->stream));
     return MELA_OK;
 }
 
@@ -91,21 +141,20 @@ mela_status matmul_entry(mela_ctx *ctx, const void *A, int ac, size_t m, size_t 
     Guard g(ctx);
     uint64_t *Ap, *Bp;
     size_t lda, ldb;
-    mela_status st = upload_pack(ctx, A, ac, m, k, SLOT_IN_A, SLOT_A, &Ap, &lda);
+    const size_t sa = staged_bytes(m, k);
+    uint8_t *pin = nullptr;
+    mela_status st = pinned_get(ctx, sa + staged_bytes(k, n), &pin);
     if (st != MELA_OK) return st;
-    st = upload_pack(ctx, B, bc, k, n, SLOT_IN_B, SLOT_B, &Bp, &ldb);
-    if (st != MELA_OK) return st;
+    if ((st = upload_pack(ctx, A, ac, m, k, pin, SLOT_IN_A, SLOT_A, &Ap, &lda)) != MELA_OK)
+        return st;
+    if ((st = upload_pack(ctx, B, bc, k, n, pin + sa, SLOT_IN_B, SLOT_B, &Bp, &ldb)) != MELA_OK)
+        return st;
     const size_t ldc = words64(n);
-    void *Cp = nullptr, *out = nullptr;
+    void *Cp = nullptr;
     if ((st = scratch_get(ctx, SLOT_C, m * ldc * 8, &Cp)) != MELA_OK) return st;
-    if ((st = scratch_get(ctx, SLOT_OUT, mn, &out)) != MELA_OK) return st;
     st = matmul_device(ctx, Ap, lda, Bp, ldb, (uint64_t *)Cp, ldc, m, k, n, algo);
     if (st != MELA_OK) return st;
-    if ((st = launch_unpack(ctx->stream, (uint64_t *)Cp, ldc, m, n, (uint8_t *)out)) != MELA_OK)
-        return st;
-    MELA_HIP(hipMemcpyAsync(C, out, mn, hipMemcpyDeviceToHost, ctx->stream));
-    MELA_HIP(hipStreamSynchronize(ctx->stream));
-    return MELA_OK;
+    return download_unpack(ctx, (const uint64_t *)Cp, ldc, m, n, SLOT_OUT, C);
 }
 
 mela_status rank_entry(mela_ctx *ctx, const void *A, int ac, size_t m, size_t n, double *rank,
@@ -128,8 +177,11 @@ mela_status rank_entry(mela_ctx *ctx, const void *A, int ac, size_t m, size_t n,
     Guard g(ctx);
     uint64_t *Ap;
     size_t lda;
-    mela_status st = upload_pack(ctx, A, ac, m, n, SLOT_IN_A, SLOT_A, &Ap, &lda);
+    uint8_t *pin = nullptr;
+    mela_status st = pinned_get(ctx, staged_bytes(m, n), &pin);
     if (st != MELA_OK) return st;
+    if ((st = upload_pack(ctx, A, ac, m, n, pin, SLOT_IN_A, SLOT_A, &Ap, &lda)) != MELA_OK)
+        return st;
     ElimResult res;
     st = eliminate(ctx, Ap, lda, m, n, variant, false, false, &res, nullptr, nullptr);
     if (st != MELA_OK) return st;
@@ -191,7 +243,9 @@ mela_status null_entry(mela_ctx *ctx, const void *H, int hc, size_t m, size_t n,
     if (m && n) {
         uint64_t *Ap;
         size_t lda;
-        st = upload_pack(ctx, H, hc, m, n, SLOT_IN_A, SLOT_A, &Ap, &lda);
+        uint8_t *pin = nullptr;
+        if ((st = pinned_get(ctx, staged_bytes(m, n), &pin)) != MELA_OK) return st;
+        st = upload_pack(ctx, H, hc, m, n, pin, SLOT_IN_A, SLOT_A, &Ap, &lda);
         if (st != MELA_OK) return st;
         st = eliminate(ctx, Ap, lda, m, n, variant, true, true, &res, &Q, &ldq);
         if (st != MELA_OK) return st;
@@ -206,19 +260,19 @@ mela_status null_entry(mela_ctx *ctx, const void *H, int hc, size_t m, size_t n,
     build_null_maps(n, piv, rowmap, freecols);
     int *d_rowmap, *d_free;
     if ((st = upload_maps(ctx, rowmap, freecols, &d_rowmap, &d_free)) != MELA_OK) return st;
+    // the basis packed (n x f, row-major), then to the host as columns
+    const size_t ldz = words64(f);
     void *dz = nullptr;
-    if ((st = scratch_get(ctx, SLOT_NULL_Z, n * f, &dz)) != MELA_OK) return st;
+    if ((st = scratch_get(ctx, SLOT_NULL_Z, n * ldz * 8, &dz)) != MELA_OK) return st;
     if (!Q) {
         void *dummy;
         if ((st = scratch_get(ctx, SLOT_ELIM_Q, 8, &dummy)) != MELA_OK) return st;
         Q = (uint64_t *)dummy;
     }
-    if ((st = launch_null_logical(ctx->stream, Q, ldq, d_rowmap, d_free, n, f, (uint8_t *)dz)) !=
+    if ((st = launch_null_packed(ctx, Q, ldq, d_rowmap, piv, n, f, (uint64_t *)dz, ldz)) !=
         MELA_OK)
         return st;
-    MELA_HIP(hipMemcpyAsync(Z, dz, n * f, hipMemcpyDeviceToHost, ctx->stream));
-    MELA_HIP(hipStreamSynchronize(ctx->stream));
-    return MELA_OK;
+    return download_unpack(ctx, (const uint64_t *)dz, ldz, n, f, SLOT_OUT, Z);
 }
 
 mela_status check_bitmat(const mela_bitmat *M, const char *name) {
@@ -281,7 +335,10 @@ mela_status mela_rref_gf2(mela_ctx *ctx, const void *A, int a_class, size_t m, s
     Guard g(ctx);
     uint64_t *Ap;
     size_t lda;
-    if ((st = upload_pack(ctx, A, a_class, m, n, SLOT_IN_A, SLOT_A, &Ap, &lda)) != MELA_OK) return st;
+    uint8_t *pin = nullptr;
+    if ((st = pinned_get(ctx, staged_bytes(m, n), &pin)) != MELA_OK) return st;
+    if ((st = upload_pack(ctx, A, a_class, m, n, pin, SLOT_IN_A, SLOT_A, &Ap, &lda)) != MELA_OK)
+        return st;
     ElimResult res;
     uint64_t *Q = nullptr;
     size_t ldq = 1;
@@ -289,17 +346,14 @@ mela_status mela_rref_gf2(mela_ctx *ctx, const void *A, int a_class, size_t m, s
         return st;
     // R: the rank pivot rows on top, zero rows below, unpacked column-major
     const size_t wc = words64(n);
-    void *Rp = nullptr, *out = nullptr;
+    void *Rp = nullptr;
     if ((st = scratch_get(ctx, SLOT_C, m * wc * 8, &Rp)) != MELA_OK) return st;
-    if ((st = scratch_get(ctx, SLOT_OUT, mn, &out)) != MELA_OK) return st;
     if ((st = launch_fill_zero(ctx->stream, Rp, m * wc * 8)) != MELA_OK) return st;
     if (res.rank)
         MELA_HIP(hipMemcpy2DAsync(Rp, wc * 8, Q, ldq * 8, wc * 8, res.rank, hipMemcpyDeviceToDevice,
                                   ctx->stream));
-    if ((st = launch_unpack(ctx->stream, (uint64_t *)Rp, wc, m, n, (uint8_t *)out)) != MELA_OK)
+    if ((st = download_unpack(ctx, (const uint64_t *)Rp, wc, m, n, SLOT_OUT, R)) != MELA_OK)
         return st;
-    MELA_HIP(hipMemcpyAsync(R, out, mn, hipMemcpyDeviceToHost, ctx->stream));
-    MELA_HIP(hipStreamSynchronize(ctx->stream));
     if (jb)
         for (size_t i = 0; i < res.rank; ++i) jb[i] = (double)res.pivots[i] + 1.0;
     *rank = res.rank;

@@ -63,6 +63,9 @@ struct mela_ctx {
     // event pairs around the dominant kernel of each product while profiling
     bool prof_on = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof;
+    // pinned host staging of the host-array entry points (grow-only)
+    void *pin = nullptr;
+    size_t pin_bytes = 0;
 };
 
 namespace mela {
@@ -89,6 +92,10 @@ inline void prof_mark(mela_ctx *ctx, bool start) {
 
 // Returns a device buffer of at least `bytes` for `slot` (kept across calls).
 mela_status scratch_get(mela_ctx *ctx, int slot, size_t bytes, void **out);
+
+// Returns the context's pinned host staging buffer, at least `bytes` long,
+// once the context stream is idle (no earlier copy still reads or writes it).
+mela_status pinned_get(mela_ctx *ctx, size_t bytes, uint8_t **out);
 
 enum ScratchSlot {
     SLOT_A = 0, SLOT_B, SLOT_C, SLOT_BT, SLOT_IN_A, SLOT_IN_B, SLOT_OUT,
@@ -133,10 +140,8 @@ struct ElimResult {
 mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m,
                       size_t n, int variant, bool rref, bool want_pivots,
                       ElimResult *res, uint64_t **Q_out, size_t *ldq_out);
-// logical column-major null basis (n x f bytes, device) from RREF rows
-mela_status launch_null_logical(hipStream_t s, const uint64_t *Q, size_t ldq,
-                                const int *rowmap_dev, const int *free_dev,
-                                size_t n, size_t f, uint8_t *Z);
+// packed null basis (n x f, row-major, ldz words, device) from the RREF rows;
+// returns with the stream idle
 mela_status launch_null_packed(mela_ctx *ctx, const uint64_t *Q, size_t ldq, const int *rowmap,
                                const std::vector<int> &piv, size_t n, size_t f, uint64_t *Z,
                                size_t ldz);

@@ -58,6 +58,28 @@ mela_status scratch_get(mela_ctx *ctx, int slot, size_t bytes, void **out) {
     return MELA_OK;
 }
 
+mela_status pinned_get(mela_ctx *ctx, size_t bytes, uint8_t **out) {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    if (bytes == 0) bytes = 16;
+    if (ctx->pin_bytes < bytes) {
+        if (ctx->pin) (void)hipHostFree(ctx->pin);
+        ctx->pin = nullptr;
+        ctx->pin_bytes = 0;
+        const size_t want = (bytes + ((size_t)1 << 20) - 1) & ~(((size_t)1 << 20) - 1);
+        void *p = nullptr;
+        e = hipHostMalloc(&p, want, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return set_error(MELA_ERR_NOMEM, "Out of memory.", "mela:memory");
+        }
+        ctx->pin = p;
+        ctx->pin_bytes = want;
+    }
+    *out = (uint8_t *)ctx->pin;
+    return MELA_OK;
+}
+
 }  // namespace mela
 
 using namespace mela;
@@ -125,6 +147,7 @@ void mela_ctx_destroy(mela_ctx *ctx) {
     }
     for (auto &s : ctx->scratch)
         if (s.ptr) (void)hipFree(s.ptr);
+    if (ctx->pin) (void)hipHostFree(ctx->pin);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -393,30 +393,11 @@ k_copy_rows(const uint64_t *__restrict__ src, size_t lds, uint64_t *__restrict__
     dst[i * ldd + w] = src[i * lds + w];
 }
 
-// Z (n x f logical, column-major): row p of column j is 1 if p is the j-th
-// free column, bit free[j] of pivot row rowmap[p] if p is a pivot column, 0
-// otherwise (gf2null.m:53-55; null_gf2.c:223-273 computes the same entries by
-// back-substitution).
-__global__ void __launch_bounds__(256)
-k_null_logical(const uint64_t *__restrict__ Q, size_t ldq, const int *__restrict__ rowmap,
-               const int *__restrict__ freecols, size_t n, size_t f, uint8_t *__restrict__ Z) {
-    const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (p >= n) return;
-    const int rm = rowmap[p];
-    for (size_t j = blockIdx.y; j < f; j += gridDim.y) {
-        uint8_t v;
-        if (rm >= 0) {
-            const int fc = freecols[j];
-            v = (uint8_t)((Q[(size_t)rm * ldq + (fc >> 6)] >> (fc & 63)) & 1ull);
-        } else {
-            v = (size_t)(-rm - 1) == j;
-        }
-        Z[j * n + p] = v;
-    }
-}
-
-// Packed Z (n x f, row-major, ldz words), one thread per word (p, jw) so
-// the stores are coalesced: a free column's row is its unit vector; a pivot
+// Packed Z (n x f, row-major, ldz words): row p of column j is 1 if p is the
+// j-th free column, bit free[j] of pivot row rowmap[p] if p is a pivot column
+// (gf2null.m:53-55; null_gf2.c:223-273 computes the same entries by
+// back-substitution).  One thread per word (p, jw), so the stores are
+// coalesced: a free column's row is its unit vector; a pivot
 // column's row gathers the free columns of its RREF row rowmap[p]: output
 // word jw takes the compressed column words tab[jw].ws .. tab[jw+1].ws, each
 // word's free bits packed by the six-step compress and shifted to their place
@@ -588,16 +569,6 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
         MELA_HIP(hipStreamSynchronize(s));
     }
     if (Q_out) *Q_out = Q;
-    return MELA_OK;
-}
-
-mela_status launch_null_logical(hipStream_t s, const uint64_t *Q, size_t ldq, const int *rowmap,
-                                const int *freecols, size_t n, size_t f, uint8_t *Z) {
-    if (n == 0 || f == 0) return MELA_OK;
-    const unsigned gy = (unsigned)(f < 65535 ? f : 65535);
-    hipLaunchKernelGGL(k_null_logical, dim3((unsigned)((n + 255) / 256), gy), dim3(256), 0, s, Q,
-                       ldq, rowmap, freecols, n, f, Z);
-    MELA_HIP(hipGetLastError());
     return MELA_OK;
 }
 

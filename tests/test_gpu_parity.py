@@ -262,6 +262,32 @@ def test_null_double_input(mela, orc):
     assert np.array_equal(mela.mela_null_m4ri(H), orc.null(H, "gf2"))
 
 
+def test_host_arrays_in_several_staging_chunks(mela, orc):
+    """Host arrays above 8 MiB are packed and copied in column chunks (a
+    1500-row double column is 12000 bytes, so these go in 3-4 chunks):
+    every entry point still matches the oracle bit for bit, double specials
+    included, and the results come back through the packed transpose."""
+    rng = np.random.default_rng(17)
+    H = rng.integers(-7, 8, (1500, 2100)).astype(np.float64)
+    H[::97, ::89] = np.nan
+    H[5::101, 3::83] = 2147483649.0
+    H[7::103, 1::79] = -0.75
+    B = rng.integers(0, 2, (2100, 700)).astype(bool)
+    assert mela.mela_rank_m4ri(H) == orc.rank(H, "m4ri")
+    assert mela.mela_rank_gf2(H) == orc.rank(H, "gf2")
+    assert np.array_equal(mela.mela_null_gf2(H), orc.null(H, "gf2"))
+    assert np.array_equal(mela.mela_matmul_gf2(H, B), orc.matmul(H, B, "gf2"))
+    # logical: 4096-byte columns, 8 MiB per chunk -> two chunks of 2048 columns
+    L = np.asfortranarray(rng.integers(0, 2, (4096, 2600)).astype(bool))
+    B3 = rng.integers(0, 2, (2600, 3)).astype(bool)
+    assert np.array_equal(mela.mela_matmul_m4ri(L, B3), orc.matmul(L, B3, "gf2"))
+    assert mela.mela_rank_gf2(L) == orc.rank(L, "gf2")
+    R, jb = mela.mela_rref_gf2(L[:, :2300])
+    M, piv = orc.rref_packed(orc.pack(L[:, :2300]), 4096, 2300, "gf2")
+    assert np.array_equal(R, orc.unpack(M, 4096, 2300))
+    assert jb.ravel().tolist() == [float(c + 1) for c in piv]
+
+
 def test_rref_packed_matches_oracle(ctx, orc):
     m, n = 500, 900
     P = orc.gen_bernoulli_packed(3, m, n)
