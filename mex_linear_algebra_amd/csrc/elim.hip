@@ -139,6 +139,15 @@ constexpr int UP_WAVES = 8;   // 512 threads
 constexpr int UP_ROWS = 16;       // rows in flight per wave in the sweep
 constexpr int SWEEP_TILE = 32;    // words per sweep workgroup (32 or 64)
 
+// Row words that the next launch reads: written through the L2 (agent-scope
+// relaxed stores, `sc1`).  The eight XCDs' L2s are not coherent with each
+// other, so the end of a launch writes back every dirty L2 line for the
+// workgroups of the next launch; stores that went through leave none, and
+// the per-panel launches end sooner (3-4% over a 16384 x 16384 elimination).
+__device__ __forceinline__ void store_through(uint64_t *p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // LDS-only workgroup barrier: row loads already in flight stay in flight.
 __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_s_waitcnt((0xF) | (3 << 14) | (7 << 4) | (0 << 8));  // lgkmcnt(0)
@@ -195,7 +204,7 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
             const uint64_t xv = it == 0 ? dx : Q[(size_t)j * ldq + (g - 1)];
             uint64_t av = xv;
             for (uint64_t h = xv & pmp; h; h &= h - 1) av ^= PWd[ffs64(h)];
-            Q[(size_t)j * ldq + (g - 1)] = av;
+            store_through(&Q[(size_t)j * ldq + (g - 1)], av);
         }
     };
     const uint64_t pm = meta->pm;
@@ -293,7 +302,7 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
             for (int u = 0; u < 16; ++u) v ^= T[(cb >> (4 * u)) & 15u][u][ww];
             pv[k][b] = v;
             if (blockIdx.y == 0 && pair_w(k) >= g && pair_w(k) < wc && ((pm >> c) & 1ull))
-                Q[(size_t)(r0 + popc64(pm & ((1ull << c) - 1ull))) * ldq + pair_w(k)] = v;
+                store_through(&Q[(size_t)(r0 + popc64(pm & ((1ull << c) - 1ull))) * ldq + pair_w(k)], v);
         }
     }
     lds_barrier();
@@ -329,7 +338,7 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
         }
 #pragma unroll
         for (int j = 0; j < R; ++j) {
-            if (x[j]) rp[j][w] = a[j];
+            if (x[j]) store_through(&rp[j][w], a[j]);
             if (q0 + j * RPW < n_act && wok) nz |= a[j];
         }
         // the next selection's panel words, in active-list order
@@ -337,7 +346,7 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
 #pragma unroll
             for (int j = 0; j < R; ++j) {
                 const int q = q0 + j * RPW;
-                if (q < n_act && q < PS_THREADS) panelw[q] = a[j];
+                if (q < n_act && q < PS_THREADS) store_through(&panelw[q], a[j]);
             }
         if (q0 + stride < total) load_rows(q0 + stride);
     }
