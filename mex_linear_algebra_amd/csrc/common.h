@@ -43,6 +43,10 @@ struct ElimGraph {
     int variant = 0, rref = 0;
     const void *bufs[4] = {nullptr, nullptr, nullptr, nullptr};
     int seen = 0;
+    // panels the last uncaptured run needed (the first panel whose selection
+    // found the elimination complete, plus one; the word count if none did)
+    size_t panels_needed = 0;
+    size_t g_end = 0;  // panels in the captured sequence
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
 };

@@ -47,6 +47,7 @@ struct ElimMeta {
     int last_swept;     // last panel with pivots (its sweep ran), -1 before the first
     uint64_t pm_prev;   // pm and r0 of the previous panel (RREF: its word update is
     int r0_prev;        // applied by the next panel's sweep)
+    int done_at;        // first panel whose selection found the elimination complete
     int sel_row[64];           // selected row ids (selection slot -> row)
     uint64_t comb[64];  // by pivot bit c: selection slots XORed into pivot row c
 };
@@ -99,6 +100,7 @@ k_panel_select(const uint64_t *__restrict__ M, size_t ld, uint32_t *__restrict__
             meta->rho = 0;
             meta->pm = 0;
             meta->finished = finished ? 1 : 0;
+            if (meta->done_at > g) meta->done_at = g;
         }
         return;
     }
@@ -390,6 +392,7 @@ __global__ void k_elim_init(ElimMeta *meta, uint32_t *act, int m) {
         meta->last_swept = -1;
         meta->pm_prev = 0;
         meta->r0_prev = 0;
+        meta->done_at = INT32_MAX;
     }
 }
 
@@ -483,18 +486,23 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
     MELA_HIP(hipGetLastError());
 
     const int cu = ctx->cu_count > 0 ? ctx->cu_count : 256;
-    auto enqueue_panels = [&]() -> mela_status {
-        hipLaunchKernelGGL(k_elim_init, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, meta,
-                           act, (int)m);
-        // pivot rows are zero left of their panel (only their words >= g are
-        // written); the RREF output and the null extraction read them whole
-        if (rref) {
-            mela_status zs = launch_fill_zero(s, Q, rmax * ld * sizeof(uint64_t));
+    // panels g_begin .. g_end-1 (the first range also initialises the state);
+    // `last`: for RREF, the final panel's word of the earlier pivot rows too
+    auto enqueue_panels = [&](size_t g_begin, size_t g_end, bool last) -> mela_status {
+        if (g_begin == 0) {
+            hipLaunchKernelGGL(k_elim_init, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s,
+                               meta, act, (int)m);
+            // pivot rows are zero left of their panel (only their words >= g
+            // are written); the RREF output and the null extraction read them
+            // whole
+            if (rref) {
+                mela_status zs = launch_fill_zero(s, Q, rmax * ld * sizeof(uint64_t));
+                if (zs != MELA_OK) return zs;
+            }
+            mela_status zs = launch_fill_zero(s, swept, NSWEPT * sizeof(unsigned));
             if (zs != MELA_OK) return zs;
         }
-        mela_status zs = launch_fill_zero(s, swept, NSWEPT * sizeof(unsigned));
-        if (zs != MELA_OK) return zs;
-        for (size_t g = 0; g < wc; ++g) {
+        for (size_t g = g_begin; g < g_end; ++g) {
             hipLaunchKernelGGL(k_panel_select, dim3(1), dim3(PS_THREADS), 0, s, Mw, ld, act, meta,
                                pivcols, swept, panelw, (int)g, (int)rmax);
             {
@@ -522,9 +530,9 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
         }
         // RREF: word g of the earlier pivot rows is brought up to date by the
         // next panel's sweep; the last panel's word here
-        if (rref)
+        if (rref && last)
             hipLaunchKernelGGL(k_update_panelword, dim3((unsigned)((rmax + 255) / 256)), dim3(256),
-                               0, s, Q, ld, meta, (int)(wc - 1));
+                               0, s, Q, ld, meta, (int)(g_end - 1));
         MELA_HIP(hipGetLastError());
         return MELA_OK;
     };
@@ -552,24 +560,47 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
     }
     eg->seen++;
     if (!eg->exec && eg->seen >= 2 && wc <= 4096 && ctx->capture_ok) {
+        // the captured sequence stops after the panels the last run needed
+        // (panels past the one whose selection finds the elimination complete
+        // are empty launches); a matrix that needs more continues below
+        const size_t g_end = eg->panels_needed ? eg->panels_needed : wc;
         MELA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        mela_status st = enqueue_panels();
+        mela_status st = enqueue_panels(0, g_end, g_end == wc);
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(s, &graph);
         if (st != MELA_OK) return st;
         if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
         eg->graph = graph;
+        eg->g_end = g_end;
         MELA_HIP(hipGraphInstantiate(&eg->exec, graph, nullptr, nullptr, 0));
     }
+    ElimMeta hm;
+    auto read_meta = [&]() -> mela_status {
+        MELA_HIP(hipMemcpyAsync(&hm, meta, sizeof(ElimMeta), hipMemcpyDeviceToHost, s));
+        MELA_HIP(hipStreamSynchronize(s));
+        return MELA_OK;
+    };
     if (eg->exec) {
         MELA_HIP(hipGraphLaunch(eg->exec, s));
+        if ((st = read_meta()) != MELA_OK) return st;
+        if (eg->g_end < wc && (size_t)hm.done_at >= eg->g_end) {
+            // not complete within the captured panels: the rest uncaptured,
+            // and the next call captures the whole sequence
+            if ((st = enqueue_panels(eg->g_end, wc, true)) != MELA_OK) return st;
+            if ((st = read_meta()) != MELA_OK) return st;
+            (void)hipGraphExecDestroy(eg->exec);
+            (void)hipGraphDestroy(eg->graph);
+            eg->exec = nullptr;
+            eg->graph = nullptr;
+            eg->panels_needed = wc;
+            eg->seen = 1;
+        }
     } else {
-        mela_status st = enqueue_panels();
-        if (st != MELA_OK) return st;
+        if ((st = enqueue_panels(0, wc, true)) != MELA_OK) return st;
+        if ((st = read_meta()) != MELA_OK) return st;
+        eg->panels_needed = (size_t)hm.done_at < wc ? (size_t)hm.done_at + 1 : wc;
     }
-    int r_dev = 0;
-    MELA_HIP(hipMemcpyAsync(&r_dev, &meta->r, sizeof(int), hipMemcpyDeviceToHost, s));
-    MELA_HIP(hipStreamSynchronize(s));
+    const int r_dev = hm.r;
     res->rank = (size_t)r_dev;
     if (want_pivots && r_dev > 0) {
         res->pivots.resize((size_t)r_dev);

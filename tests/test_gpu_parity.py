@@ -262,6 +262,29 @@ def test_null_double_input(mela, orc):
     assert np.array_equal(mela.mela_null_m4ri(H), orc.null(H, "gf2"))
 
 
+@pytest.mark.parametrize("variant", ["gf2", "m4ri"])
+def test_captured_panels_continue_for_a_higher_rank(mela, orc, variant):
+    """Repeated eliminations of one shape replay a captured panel sequence
+    that stops where the earlier run found the elimination complete.  A later
+    matrix of the same shape that needs more panels continues past it, and
+    the sequence after that is captured whole; every result matches the
+    oracle (rank, and the null basis through the RREF path)."""
+    rng = np.random.default_rng(23)
+    m, n = 600, 1300
+    low = orc.matmul(rng.integers(0, 2, (m, 100)).astype(bool),
+                     rng.integers(0, 2, (100, n)).astype(bool), "gf2")
+    full = rng.integers(0, 2, (m, n)).astype(bool)
+    rank_fn = getattr(mela, f"mela_rank_{variant}")
+    null_fn = getattr(mela, f"mela_null_{variant}")
+    want_rank = {id(low): orc.rank(low, "gf2"), id(full): orc.rank(full, "gf2")}
+    want_null = {id(low): orc.null(low, "gf2"), id(full): orc.null(full, "gf2")}
+    assert want_rank[id(low)] == 100 and want_rank[id(full)] == 600
+    for A in (low, low, low, full, full, full, low, full):
+        assert rank_fn(A) == want_rank[id(A)]
+    for A in (low, low, low, full, full, full, low, full):
+        assert np.array_equal(null_fn(A), want_null[id(A)])
+
+
 def test_host_arrays_in_several_staging_chunks(mela, orc):
     """Host arrays above 8 MiB are packed and copied in column chunks (a
     1500-row double column is 12000 bytes, so these go in 3-4 chunks):
