@@ -129,7 +129,8 @@ struct SelectLDS {
     uint32_t mv[64];
     unsigned long long pm;
     int nsel;
-    int fast;  // the fast path selected 64 rows (combinations already final)
+    int fast;   // the fast path selected 64 rows (combinations already final)
+    int added;  // the last candidate group added pivots
 };
 
 // Result of a panel selection (valid on every thread after the call).
@@ -264,14 +265,24 @@ __device__ PanelOut wg_panel_select(SelectLDS<NT> &L, ACT *act, int n_act, int g
             L.cpos[my] = pos;
         }
         __syncthreads();
-        if (wave == 0) {
-            uint64_t pmw = L.pm;
-            int nsel = L.nsel;
-            for (int gb = 0; gb < ncand && pmw != ~0ull; gb += 64) {
+        // groups of 64 candidates through wave 0's Gauss-Jordan; after a group
+        // that added pivots every thread reduces the remaining candidates by
+        // the basis as it is now and compacts them (once the panel's pivots
+        // are found the rest of the chunk usually vanishes here, instead of
+        // being reduced 64 at a time by wave 0)
+        for (int gb = 0;;) {
+            // every thread reads the loop state before the barrier; wave 0
+            // writes it only after
+            const bool more = gb < ncand && L.pm != ~0ull;
+            __syncthreads();
+            if (!more) break;
+            if (wave == 0) {
+                uint64_t pmw = L.pm;
+                int nsel = L.nsel;
                 const int ci = gb + lane;
                 const bool valid = ci < ncand;
                 uint64_t v = valid ? L.cy[ci] : 0;
-                uint64_t h2 = v & pmw;  // pivots added since the chunk filter
+                uint64_t h2 = v & pmw;  // pivots added since the last filter
                 while (h2) {
                     const int c = ffs64(h2);
                     h2 &= h2 - 1;
@@ -303,13 +314,42 @@ __device__ PanelOut wg_panel_select(SelectLDS<NT> &L, ACT *act, int n_act, int g
                 __threadfence_block();
                 nsel += popc64(pl);
                 pmw |= newmask;
+                if (lane == 0) {
+                    L.pm = pmw;
+                    L.nsel = nsel;
+                    L.added = pl != 0;
+                }
             }
-            if (lane == 0) {
-                L.pm = pmw;
-                L.nsel = nsel;
+            __syncthreads();
+            gb += 64;
+            if (L.added && gb < ncand && L.pm != ~0ull) {
+                const int ci = gb + tid;
+                uint64_t yy = 0;
+                int cp = 0;
+                if (ci < ncand) {
+                    yy = L.cy[ci];
+                    cp = L.cpos[ci];
+                    const uint64_t pmn = L.pm;
+                    for (uint64_t h = yy & pmn; h; h &= h - 1) yy ^= L.bas[ffs64(h)];
+                }
+                const uint64_t b2 = __ballot(yy != 0);
+                if (lane == 0) L.wcnt[wave] = popc64(b2);
+                __syncthreads();
+                int off2 = 0, n2 = 0;
+                for (int q = 0; q < NT / 64; ++q) {
+                    const int cq = L.wcnt[q];
+                    if (q < wave) off2 += cq;
+                    n2 += cq;
+                }
+                if (yy != 0) {
+                    const int my = gb + off2 + popc64(b2 & ((1ull << lane) - 1ull));
+                    L.cy[my] = yy;
+                    L.cpos[my] = cp;
+                }
+                ncand = gb + n2;
+                __syncthreads();
             }
         }
-        __syncthreads();
         if (L.pm == ~0ull) break;
     }
 
