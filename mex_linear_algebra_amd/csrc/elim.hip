@@ -108,6 +108,8 @@ k_panel_select(const uint64_t *__restrict__ M, size_t ld, uint32_t *__restrict__
     const dev::PanelOut po = dev::wg_panel_select<PS_THREADS>(L, act, n_act, g, rowfn, meta->comb,
                                                               meta->sel_row, have_pre, pre, actc);
     const int tid = threadIdx.x;
+    // slots past the selection keep a valid row id (the sweep loads every slot)
+    if (tid < 64 && tid >= po.rho) meta->sel_row[tid] = 0;
     if (tid < 64 && ((po.pm >> tid) & 1ull))
         pivcols[r_before + popc64(po.pm & ((1ull << tid) - 1ull))] = g * 64 + tid;
     if (tid == 0) {
@@ -174,7 +176,8 @@ template <int TILE, bool GE>
 __global__ void __launch_bounds__(UP_WAVES * 64, 64 / TILE)
 k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
         uint64_t *__restrict__ Q, size_t ldq, const ElimMeta *__restrict__ meta,
-        unsigned *__restrict__ swept, uint64_t *__restrict__ panelw, int g, int wc, int rref) {
+        unsigned *__restrict__ swept, uint64_t *__restrict__ panelw, int g, int wc, int rref,
+        int mrows) {
     // T[entry][table][word]: the table index lands in the ds_read immediate
     // offset, the entry index is the only per-lookup address
     __shared__ uint64_t T[16][16][TILE];
@@ -209,6 +212,26 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
             store_through(&Q[(size_t)j * ldq + (g - 1)], av);
         }
     };
+    const int wl = lane % TILE, sub = lane / TILE;
+    const int per_wave = R * RPW;
+    int q0 = (blockIdx.y * UP_WAVES + wave) * per_wave + sub;
+    // (table, word) pairs of this thread: table e / TILE, word e % TILE
+    auto pair_t = [&](int k) { return (tid + k * NT) / TILE; };
+    // loads that need no metadata go out with it: the selected rows' ids
+    // (every slot holds a valid row id; slots from rho on are discarded) and
+    // the first row batch's active-list entries (every position below mrows
+    // holds a row id)
+    int srow[TPT][4];
+#pragma unroll
+    for (int k = 0; k < TPT; ++k)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) srow[k][b] = meta->sel_row[4 * pair_t(k) + b];
+    uint32_t pa[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int q = q0 + j * RPW;
+        pa[j] = act[q < mrows ? q : mrows - 1];
+    }
     const uint64_t pm = meta->pm;
     if (pm == 0) {
         finish_deferred();
@@ -216,7 +239,6 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
     }
     const int n_act = meta->n_act, r0 = meta->r0, rho = meta->rho;
     const int total = n_act + (rref ? r0 : 0);
-    const int wl = lane % TILE, sub = lane / TILE;
     // tiles start at word g rounded down to a 128-byte line (16 words), so
     // every row segment a tile reads or writes covers whole lines; words
     // below g+1 ride along unchanged
@@ -225,16 +247,12 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
     const bool win = w < wc;            // a word of the matrix
     const bool wok = win && w > g;      // a word the sweep updates
     const int wsafe = win ? w : g;      // loads of words past the end re-read word g
-    const int per_wave = R * RPW;
     const int stride = gridDim.y * UP_WAVES * per_wave;
-    int q0 = (blockIdx.y * UP_WAVES + wave) * per_wave + sub;
     const bool rows_here = (int)blockIdx.y * UP_WAVES * per_wave < total;
     if (!rows_here && blockIdx.y != 0) {
         finish_deferred();
         return;
     }
-    // (table, word) pairs of this thread: table e / TILE, word e % TILE
-    auto pair_t = [&](int k) { return (tid + k * NT) / TILE; };
     auto pair_w = [&](int k) { return g0 + blockIdx.x * TILE + (tid + k * NT) % TILE; };
     // selected rows' words first, then the first row batch, whose loads stay
     // in flight while the tables are built
@@ -251,17 +269,22 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
         const int pw = pair_w(k) < wc ? pair_w(k) : g;
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const int slot = 4 * pair_t(k) + b;
-            sq[k][b] = M[(size_t)meta->sel_row[slot < rho ? slot : 0] * ld + pw];
+            sq[k][b] = M[(size_t)srow[k][b] * ld + pw];
         }
     }
     uint64_t *rp[R];
     uint64_t x[R], a[R];
-    auto load_rows = [&](int qb) {
+    // first: the batch at q0, whose active-list entries are in pa (a row past
+    // the end reads any valid row and is discarded)
+    auto load_rows = [&](int qb, bool first) {
 #pragma unroll
         for (int j = 0; j < R; ++j) {
             const int q = qb + j * RPW;
-            rp[j] = row_ptr(M, ld, act, Q, ldq, n_act, q < total ? q : 0);
+            if (first)
+                rp[j] = q < n_act || q >= total ? M + (size_t)pa[j] * ld
+                                                : Q + (size_t)(q - n_act) * ldq;
+            else
+                rp[j] = row_ptr(M, ld, act, Q, ldq, n_act, q < total ? q : 0);
         }
 #pragma unroll
         for (int j = 0; j < R; ++j) x[j] = rp[j][g];
@@ -270,7 +293,7 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
 #pragma unroll
         for (int j = 0; j < R; ++j) x[j] = (qb + j * RPW < total && wok) ? x[j] & pm : 0ull;
     };
-    if (rows_here) load_rows(q0);
+    if (rows_here) load_rows(q0, true);
     auto build = [&](const uint64_t (&v)[TPT][4]) {
 #pragma unroll
         for (int k = 0; k < TPT; ++k) {
@@ -350,7 +373,7 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
                 const int q = q0 + j * RPW;
                 if (q < n_act && q < PS_THREADS) store_through(&panelw[q], a[j]);
             }
-        if (q0 + stride < total) load_rows(q0 + stride);
+        if (q0 + stride < total) load_rows(q0 + stride, false);
     }
     mark_rest_nonzero(swept, nz, g);
     finish_deferred();
@@ -382,6 +405,7 @@ k_update_panelword(uint64_t *__restrict__ Q, size_t ldq, const ElimMeta *__restr
 __global__ void k_elim_init(ElimMeta *meta, uint32_t *act, int m) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < m) act[i] = (uint32_t)i;
+    if (i < 64) meta->sel_row[i] = 0;  // every slot holds a valid row id
     if (i == 0) {
         meta->n_act = m;
         meta->r = 0;
@@ -521,11 +545,11 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
                 if (variant == MELA_ELIM_GE)
                     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sweep<SWEEP_TILE, true>), dim3(tx, ty),
                                        dim3(UP_WAVES * 64), 0, s, Mw, ld, act, Q, ld, meta, swept, panelw, (int)g,
-                                       (int)wc, rref ? 1 : 0);
+                                       (int)wc, rref ? 1 : 0, (int)m);
                 else
                     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sweep<SWEEP_TILE, false>), dim3(tx, ty),
                                        dim3(UP_WAVES * 64), 0, s, Mw, ld, act, Q, ld, meta, swept, panelw, (int)g,
-                                       (int)wc, rref ? 1 : 0);
+                                       (int)wc, rref ? 1 : 0, (int)m);
             }
         }
         // RREF: word g of the earlier pivot rows is brought up to date by the
