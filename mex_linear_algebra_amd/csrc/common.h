@@ -70,6 +70,7 @@ struct mela_ctx {
     // pinned host staging of the host-array entry points (grow-only)
     void *pin = nullptr;
     size_t pin_bytes = 0;
+    bool pin_is_malloc = false;  // hipHostMalloc failed: ordinary host memory
 };
 
 namespace mela {
@@ -97,9 +98,11 @@ inline void prof_mark(mela_ctx *ctx, bool start) {
 // Returns a device buffer of at least `bytes` for `slot` (kept across calls).
 mela_status scratch_get(mela_ctx *ctx, int slot, size_t bytes, void **out);
 
-// Returns the context's pinned host staging buffer, at least `bytes` long,
-// once the context stream is idle (no earlier copy still reads or writes it).
+// Returns the context's host staging buffer (pinned, or ordinary host memory
+// when no pinned memory can be had), at least `bytes` long, once the context
+// stream is idle (no earlier copy still reads or writes it).
 mela_status pinned_get(mela_ctx *ctx, size_t bytes, uint8_t **out);
+void pinned_release(mela_ctx *ctx);
 
 enum ScratchSlot {
     SLOT_A = 0, SLOT_B, SLOT_C, SLOT_BT, SLOT_IN_A, SLOT_IN_B, SLOT_OUT,

@@ -1,4 +1,5 @@
 // Context, errors and device memory of libmela.
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 
@@ -58,20 +59,34 @@ mela_status scratch_get(mela_ctx *ctx, int slot, size_t bytes, void **out) {
     return MELA_OK;
 }
 
+void pinned_release(mela_ctx *ctx) {
+    if (ctx->pin) {
+        if (ctx->pin_is_malloc)
+            std::free(ctx->pin);
+        else
+            (void)hipHostFree(ctx->pin);
+    }
+    ctx->pin = nullptr;
+    ctx->pin_bytes = 0;
+    ctx->pin_is_malloc = false;
+}
+
 mela_status pinned_get(mela_ctx *ctx, size_t bytes, uint8_t **out) {
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     if (bytes == 0) bytes = 16;
     if (ctx->pin_bytes < bytes) {
-        if (ctx->pin) (void)hipHostFree(ctx->pin);
-        ctx->pin = nullptr;
-        ctx->pin_bytes = 0;
+        pinned_release(ctx);
         const size_t want = (bytes + ((size_t)1 << 20) - 1) & ~(((size_t)1 << 20) - 1);
         void *p = nullptr;
         e = hipHostMalloc(&p, want, hipHostMallocDefault);
         if (e != hipSuccess) {
+            // no pinned memory to be had: ordinary host memory still works
+            // (the copies then go through the runtime's own staging)
             (void)hipGetLastError();
-            return set_error(MELA_ERR_NOMEM, "Out of memory.", "mela:memory");
+            p = std::malloc(want);
+            if (!p) return set_error(MELA_ERR_NOMEM, "Out of memory.", "mela:memory");
+            ctx->pin_is_malloc = true;
         }
         ctx->pin = p;
         ctx->pin_bytes = want;
@@ -147,7 +162,7 @@ void mela_ctx_destroy(mela_ctx *ctx) {
     }
     for (auto &s : ctx->scratch)
         if (s.ptr) (void)hipFree(s.ptr);
-    if (ctx->pin) (void)hipHostFree(ctx->pin);
+    pinned_release(ctx);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
