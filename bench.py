@@ -446,21 +446,23 @@ def bench_serve(args, mela):
 
 
 def bench_c1(args, mela):
-    """2048 x 2048 logical A, B (column-major host arrays, as MATLAB passes
-    them): matmul (both entry points), rank(A) and null(A) end to end through
-    the C ABI -- host copies, packing, kernels and unpacking included -- next to
-    the oracle's restatement of the reference on the same inputs, both timed
-    with the reference benchmark's policy."""
+    """n x n logical A, B (n = 2048 by default; column-major host arrays, as
+    MATLAB passes them): matmul (both entry points), rank(A) and null(A) end to
+    end through the C ABI -- host packing, copies, kernels and unpacking
+    included -- next to the oracle's restatement of the reference on the same
+    inputs (skipped with --no-cpu), both timed with the reference benchmark's
+    policy."""
     import numpy as np
-    from oracle import oracle as orc
     rng = np.random.default_rng(1)
-    n = 2048
+    n = args.c1_n
+    cpu = not args.no_cpu
+    if cpu:
+        from oracle import oracle as orc
     # column-major like a MATLAB array, so no layout conversion is timed
     A = np.asfortranarray(rng.integers(0, 2, (n, n)).astype(bool))
     B = np.asfortranarray(rng.integers(0, 2, (n, n)).astype(bool))
     ctx = mela.Context(0)
 
-    rows = []
     for name, gfn, cfn, work in (
             ("matmul_gf2", lambda: mela.mela_matmul_gf2(A, B, ctx=ctx),
              lambda: orc.matmul(A, B, "gf2"), 2.0 * n ** 3),
@@ -471,10 +473,12 @@ def bench_c1(args, mela):
             ("null_gf2", lambda: mela.mela_null_gf2(A, ctx=ctx), lambda: orc.null(A, "gf2"), None),
             ("null_m4ri", lambda: mela.mela_null_m4ri(A, ctx=ctx), lambda: orc.null(A, "m4ri"), None)):
         g = reference_timing(gfn)[0] * 1e3
-        c = reference_timing(cfn)[0] * 1e3
-        row = {"workload": f"C1 {name} 2048x2048 logical, end to end (host in/out)",
-               "gpu_ms": round(g, 3), "cpu_oracle_ms": round(c, 3), "speedup": round(c / g, 1),
-               "cpu_threads": orc.num_threads()}
+        row = {"workload": f"C1 {name} {n}x{n} logical, end to end (host in/out)",
+               "gpu_ms": round(g, 3)}
+        if cpu:
+            c = reference_timing(cfn)[0] * 1e3
+            row.update({"cpu_oracle_ms": round(c, 3), "speedup": round(c / g, 1),
+                        "cpu_threads": orc.num_threads()})
         if work:
             row["gpu_Tbitops_incl_pcie"] = round(work / (g * 1e-3) / 1e12, 2)
         print(json.dumps(row), flush=True)
@@ -510,6 +514,8 @@ def main():
     ap.add_argument("--traffic", type=float, default=None,
                     help="HBM bytes per launch measured by rocprofv3 --pmc (scaled)")
     ap.add_argument("--elim-reps", type=int, default=2)
+    ap.add_argument("--c1-n", type=int, default=2048,
+                    help="side of the square host arrays of the c1 workload")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--plain-sample", type=int, default=256)
     ap.add_argument("--bytes", type=int, default=1 << 30)
