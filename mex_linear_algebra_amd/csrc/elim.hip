@@ -347,7 +347,10 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
     }
     lds_barrier();
     uint64_t nz = 0;  // active rows' words past the panel, ORed
-    for (; q0 < total; q0 += stride) {
+    // GE: the loop runs while any lane has rows (the wave's row halves end one
+    // iteration apart at most), so the walk's wave reductions see every lane
+    // active; a half past the end has zero multipliers
+    for (; GE ? __ballot(q0 < total) != 0ull : q0 < total; q0 += stride) {
         // rows whose multiplier is zero on the whole wave are skipped (a
         // wave-uniform branch); stores come after all lookups, so no row
         // waits for another row's store
@@ -355,7 +358,33 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
         for (int j = 0; j < R; ++j) {
             if (__ballot(x[j] != 0ull) == 0ull) continue;
             if (GE) {
-                for (uint64_t xx = x[j]; xx; xx &= xx - 1) a[j] ^= P[ffs64(xx)][wl];
+                // one walk over the union of the wave's multipliers (its
+                // TILE-lane rows), in scalar registers; a lane XORs pivot row
+                // c only where its own multiplier has bit c (the sign-extended
+                // bit is the mask), so every row still takes one XOR per set
+                // multiplier bit, without per-lane loop control
+                const uint32_t xl = (uint32_t)x[j], xh = (uint32_t)(x[j] >> 32);
+                const uint32_t ul = dev::wave_or32(xl), uh = dev::wave_or32(xh);
+                uint32_t al = (uint32_t)a[j], ah = (uint32_t)(a[j] >> 32);
+                // two bits per step, so two pivot-row reads are in flight
+                auto walk = [&](uint32_t h, uint32_t xs, int base) {
+                    while (h) {
+                        const int c1 = __builtin_ctz(h);
+                        h &= h - 1;
+                        const bool two = h != 0;
+                        const int c2 = two ? __builtin_ctz(h) : c1;
+                        if (two) h &= h - 1;
+                        const uint64_t p1 = P[base + c1][wl], p2 = P[base + c2][wl];
+                        const uint32_t m1 = (uint32_t)__builtin_amdgcn_sbfe((int)xs, c1, 1);
+                        const uint32_t m2 =
+                            two ? (uint32_t)__builtin_amdgcn_sbfe((int)xs, c2, 1) : 0u;
+                        al ^= ((uint32_t)p1 & m1) ^ ((uint32_t)p2 & m2);
+                        ah ^= ((uint32_t)(p1 >> 32) & m1) ^ ((uint32_t)(p2 >> 32) & m2);
+                    }
+                };
+                walk(ul, xl, 0);
+                walk(uh, xh, 32);
+                a[j] = ((uint64_t)ah << 32) | al;
             } else {
 #pragma unroll
                 for (int t = 0; t < 16; ++t) a[j] ^= T[(x[j] >> (4 * t)) & 15u][t][wl];
@@ -373,7 +402,12 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
                 const int q = q0 + j * RPW;
                 if (q < n_act && q < PS_THREADS) store_through(&panelw[q], a[j]);
             }
-        if (q0 + stride < total) load_rows(q0 + stride, false);
+        if (q0 + stride < total) {
+            load_rows(q0 + stride, false);
+        } else if (GE) {
+#pragma unroll
+            for (int j = 0; j < R; ++j) x[j] = 0ull;
+        }
     }
     mark_rest_nonzero(swept, nz, g);
     finish_deferred();

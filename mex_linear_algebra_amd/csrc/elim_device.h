@@ -112,6 +112,18 @@ __device__ __forceinline__ uint64_t wave_xor64(uint64_t v) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// OR of v over the 64 lanes of the wave, the same DPP pattern as wave_xor32
+// (all 64 lanes must be active).
+__device__ __forceinline__ uint32_t wave_or32(uint32_t v) {
+    v |= __builtin_amdgcn_update_dpp(0u, v, 0x111, 0xF, 0xF, true);   // row_shr:1
+    v |= __builtin_amdgcn_update_dpp(0u, v, 0x112, 0xF, 0xF, true);   // row_shr:2
+    v |= __builtin_amdgcn_update_dpp(0u, v, 0x114, 0xF, 0xF, true);   // row_shr:4
+    v |= __builtin_amdgcn_update_dpp(0u, v, 0x118, 0xF, 0xF, true);   // row_shr:8
+    v |= __builtin_amdgcn_update_dpp(0u, v, 0x142, 0xA, 0xF, false);  // row_bcast:15 -> rows 1, 3
+    v |= __builtin_amdgcn_update_dpp(0u, v, 0x143, 0xC, 0xF, false);  // row_bcast:31 -> rows 2, 3
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
 // Pivots the first 64 active rows must give for the selection to complete
 // them by single insertions instead of the general chunked path.
 constexpr int FAST_MIN_PIVOTS = 48;
