@@ -167,9 +167,13 @@ mela_status mela_unpack_dev(mela_ctx *ctx, const mela_bitmat *M, uint8_t *dst_de
 mela_status mela_matmul_packed(mela_ctx *ctx, const mela_bitmat *A,
                                const mela_bitmat *B, mela_bitmat *C, int algo);
 
-/* Elimination variants: GE (pivot rows applied one by one) and M4RI (pivot
- * rows applied through 2^k-entry tables).  Outputs are identical. */
-enum { MELA_ELIM_GE = 0, MELA_ELIM_M4RI = 1 };
+/* Elimination variants: GE (pivot rows applied one by one), M4RI (pivot
+ * rows applied through 2^k-entry tables) and AUTO (per row batch: the walk
+ * for multipliers with few set bits, the tables otherwise).  Outputs are
+ * identical -- rank, the reduced row echelon form and the canonical null
+ * basis are unique -- so the six host-array entry points use AUTO whichever
+ * reference algorithm they stand for; GE and M4RI force one sweep. */
+enum { MELA_ELIM_GE = 0, MELA_ELIM_M4RI = 1, MELA_ELIM_AUTO = 2 };
 
 /* rank of A (A is left unchanged; work memory comes from the context) */
 mela_status mela_rank_packed(mela_ctx *ctx, const mela_bitmat *A, int variant,

@@ -24,6 +24,7 @@ __all__ = [
     "mela_matmul_gf2", "mela_matmul_m4ri", "mela_rank_gf2", "mela_rank_m4ri",
     "mela_null_gf2", "mela_null_m4ri", "mela_rref_gf2",
     "ALGO_AUTO", "ALGO_ANDPOP", "ALGO_M4RM", "ALGO_MFMA_I8", "ALGO_MFMA_FP4", "ELIM_GE", "ELIM_M4RI",
+    "ELIM_AUTO",
     "Batch", "BATCH_NULL", "BATCH_GE", "BATCH_GRAPH",
     "Server", "Request", "Result", "OP_RANK", "OP_NULL", "OP_MATMUL",
 ]
@@ -32,7 +33,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 
 MELA_LOGICAL, MELA_DOUBLE, MELA_OTHER, MELA_SPARSE = 0, 1, 2, 0x100
 ALGO_AUTO, ALGO_ANDPOP, ALGO_M4RM, ALGO_MFMA_I8, ALGO_MFMA_FP4 = 0, 1, 2, 3, 4
-ELIM_GE, ELIM_M4RI = 0, 1
+ELIM_GE, ELIM_M4RI, ELIM_AUTO = 0, 1, 2
 
 STATUS = {0: "OK", 1: "SPARSE", 2: "DIM", 3: "TYPE", 4: "NOMEM", 5: "INVALID",
           6: "DEVICE", 7: "ADMISSION", 8: "CANCELLED", 9: "TIMEOUT", 10: "ARGS"}
@@ -264,18 +265,51 @@ class Context:
             _check(self.lib.mela_copy_d2h(self.handle, out.ctypes.data, M.data, out.nbytes))
         return out
 
+    def pack_dev(self, A) -> BitMatrix:
+        """Packed device matrix from a logical or double array that is first
+        copied to device memory as MATLAB's column-major bytes
+        (mela_pack_dev: the device-side parity packer)."""
+        buf, cls, m, n = _classify(A)
+        if buf is None:
+            raise MelaError("Inputs must be logical or double matrices.")
+        M = self.bitmat(m, n)
+        src = C.c_void_p()
+        _check(self.lib.mela_dev_alloc(self.handle, max(buf.nbytes, 1), C.byref(src)))
+        try:
+            if buf.nbytes:
+                _check(self.lib.mela_copy_h2d(self.handle, src, buf.ctypes.data, buf.nbytes))
+            _check(self.lib.mela_pack_dev(self.handle, src, cls, C.byref(M)))
+            self.sync()
+        finally:
+            self.lib.mela_dev_free(self.handle, src)
+        return M
+
+    def unpack_dev(self, M: BitMatrix) -> np.ndarray:
+        """M as a column-major logical array unpacked on the device
+        (mela_unpack_dev), copied back to the host."""
+        out = np.zeros((M.rows, M.cols), dtype=np.uint8, order="F")
+        dst = C.c_void_p()
+        _check(self.lib.mela_dev_alloc(self.handle, max(out.nbytes, 1), C.byref(dst)))
+        try:
+            _check(self.lib.mela_unpack_dev(self.handle, C.byref(M), dst))
+            if out.nbytes:
+                _check(self.lib.mela_copy_d2h(self.handle, out.ctypes.data, dst, out.nbytes))
+        finally:
+            self.lib.mela_dev_free(self.handle, dst)
+        return out.view(np.bool_)
+
     def gen_bernoulli(self, seed: int, M: BitMatrix):
         _check(self.lib.mela_gen_bernoulli(self.handle, seed, C.byref(M)))
 
     def matmul_packed(self, A: BitMatrix, B: BitMatrix, Cm: BitMatrix, algo: int = ALGO_AUTO):
         _check(self.lib.mela_matmul_packed(self.handle, C.byref(A), C.byref(B), C.byref(Cm), algo))
 
-    def rank_packed(self, A: BitMatrix, variant: int = ELIM_M4RI) -> int:
+    def rank_packed(self, A: BitMatrix, variant: int = ELIM_AUTO) -> int:
         r = C.c_size_t()
         _check(self.lib.mela_rank_packed(self.handle, C.byref(A), variant, C.byref(r)))
         return r.value
 
-    def rref_packed(self, A: BitMatrix, variant: int = ELIM_M4RI):
+    def rref_packed(self, A: BitMatrix, variant: int = ELIM_AUTO):
         rmax = min(A.rows, A.cols)
         R = self.bitmat(rmax, A.cols)
         piv = (C.c_int * max(rmax, 1))()
@@ -321,7 +355,7 @@ class Context:
         data = S.data if S.dtype in (np.float64, np.bool_) else S.data.astype(np.float64)
         return self.pack_csc(S.indptr, S.indices, data, S.shape[0], S.shape[1])
 
-    def null_packed(self, A: BitMatrix, variant: int = ELIM_M4RI) -> BitMatrix:
+    def null_packed(self, A: BitMatrix, variant: int = ELIM_AUTO) -> BitMatrix:
         Z = BitMatrix()
         f = C.c_size_t()
         _check(self.lib.mela_null_packed(self.handle, C.byref(A), variant, C.byref(Z), C.byref(f)))

@@ -157,10 +157,13 @@ mela_status matmul_entry(mela_ctx *ctx, const void *A, int ac, size_t m, size_t 
     return download_unpack(ctx, (const uint64_t *)Cp, ldc, m, n, SLOT_OUT, C);
 }
 
+// ge: the messages of mela_rank_gf2 (else mela_rank_m4ri's); the sweep is
+// MELA_ELIM_AUTO for both (the rank does not depend on it)
 mela_status rank_entry(mela_ctx *ctx, const void *A, int ac, size_t m, size_t n, double *rank,
-                       int variant) {
+                       bool ge) {
+    const int variant = MELA_ELIM_AUTO;
     if (!is_supported(ac)) {
-        if (variant == MELA_ELIM_GE)  // rank_gf2.c:131-132
+        if (ge)  // rank_gf2.c:131-132
             return set_error(MELA_ERR_TYPE, "Input must be a logical or double matrix.",
                              "bitrank_mex:inputNotLogical");
         return set_error(MELA_ERR_TYPE, "Input must be double or logical.");  // rank_m4ri.c:331-334
@@ -226,7 +229,8 @@ mela_status upload_maps(mela_ctx *ctx, const std::vector<int> &rowmap,
 }
 
 mela_status null_entry(mela_ctx *ctx, const void *H, int hc, size_t m, size_t n,
-                       mela_alloc_logical_fn alloc, void *user, size_t *nullity, int variant) {
+                       mela_alloc_logical_fn alloc, void *user, size_t *nullity) {
+    const int variant = MELA_ELIM_AUTO;
     // null_gf2.c:151-156: sparse, then class
     if (hc & MELA_SPARSE) return set_error(MELA_ERR_SPARSE, kSparseMsg);
     if (!is_supported(hc)) return set_error(MELA_ERR_TYPE, "H must be double or logical.");
@@ -275,6 +279,12 @@ mela_status null_entry(mela_ctx *ctx, const void *H, int hc, size_t m, size_t n,
     return download_unpack(ctx, (const uint64_t *)dz, ldz, n, f, SLOT_OUT, Z);
 }
 
+mela_status check_variant(int variant) {
+    if (variant != MELA_ELIM_GE && variant != MELA_ELIM_M4RI && variant != MELA_ELIM_AUTO)
+        return set_error(MELA_ERR_INVALID, "unknown elimination variant");
+    return MELA_OK;
+}
+
 mela_status check_bitmat(const mela_bitmat *M, const char *name) {
     if (!M) return set_error(MELA_ERR_INVALID, std::string("null descriptor ") + name);
     if (M->rows && M->cols && !M->data)
@@ -300,22 +310,22 @@ mela_status mela_matmul_m4ri(mela_ctx *ctx, const void *A, int a_class, size_t m
 
 mela_status mela_rank_gf2(mela_ctx *ctx, const void *A, int a_class, size_t m, size_t n,
                           double *rank) {
-    return rank_entry(ctx, A, a_class, m, n, rank, MELA_ELIM_GE);
+    return rank_entry(ctx, A, a_class, m, n, rank, true);
 }
 
 mela_status mela_rank_m4ri(mela_ctx *ctx, const void *A, int a_class, size_t m, size_t n,
                            double *rank) {
-    return rank_entry(ctx, A, a_class, m, n, rank, MELA_ELIM_M4RI);
+    return rank_entry(ctx, A, a_class, m, n, rank, false);
 }
 
 mela_status mela_null_gf2(mela_ctx *ctx, const void *H, int h_class, size_t m, size_t n,
                           mela_alloc_logical_fn alloc, void *user, size_t *nullity) {
-    return null_entry(ctx, H, h_class, m, n, alloc, user, nullity, MELA_ELIM_GE);
+    return null_entry(ctx, H, h_class, m, n, alloc, user, nullity);
 }
 
 mela_status mela_null_m4ri(mela_ctx *ctx, const void *H, int h_class, size_t m, size_t n,
                            mela_alloc_logical_fn alloc, void *user, size_t *nullity) {
-    return null_entry(ctx, H, h_class, m, n, alloc, user, nullity, MELA_ELIM_M4RI);
+    return null_entry(ctx, H, h_class, m, n, alloc, user, nullity);
 }
 
 mela_status mela_rref_gf2(mela_ctx *ctx, const void *A, int a_class, size_t m, size_t n,
@@ -342,7 +352,7 @@ mela_status mela_rref_gf2(mela_ctx *ctx, const void *A, int a_class, size_t m, s
     ElimResult res;
     uint64_t *Q = nullptr;
     size_t ldq = 1;
-    if ((st = eliminate(ctx, Ap, lda, m, n, MELA_ELIM_M4RI, true, true, &res, &Q, &ldq)) != MELA_OK)
+    if ((st = eliminate(ctx, Ap, lda, m, n, MELA_ELIM_AUTO, true, true, &res, &Q, &ldq)) != MELA_OK)
         return st;
     // R: the rank pivot rows on top, zero rows below, unpacked column-major
     const size_t wc = words64(n);
@@ -440,7 +450,7 @@ mela_status mela_matmul_packed(mela_ctx *ctx, const mela_bitmat *A, const mela_b
 mela_status mela_rank_packed(mela_ctx *ctx, const mela_bitmat *A, int variant, size_t *rank) {
     if (!ctx || !rank) return set_error(MELA_ERR_INVALID, "null argument");
     mela_status st = check_bitmat(A, "A");
-    if (st != MELA_OK) return st;
+    if (st != MELA_OK || (st = check_variant(variant)) != MELA_OK) return st;
     Guard g(ctx);
     ElimResult res;
     st = eliminate(ctx, A->data, A->ld, A->rows, A->cols, variant, false, false, &res, nullptr,
@@ -454,7 +464,7 @@ mela_status mela_rref_packed(mela_ctx *ctx, const mela_bitmat *A, int variant, m
                              int *pivots, size_t *rank) {
     if (!ctx) return set_error(MELA_ERR_INVALID, "null context");
     mela_status st = check_bitmat(A, "A");
-    if (st != MELA_OK) return st;
+    if (st != MELA_OK || (st = check_variant(variant)) != MELA_OK) return st;
     const size_t rmax = A->rows < A->cols ? A->rows : A->cols;
     if (R) {
         if ((st = check_bitmat(R, "R")) != MELA_OK) return st;
@@ -483,25 +493,25 @@ mela_status mela_null_packed(mela_ctx *ctx, const mela_bitmat *A, int variant, m
                              size_t *nullity) {
     if (!ctx || !Z) return set_error(MELA_ERR_INVALID, "null argument");
     mela_status st = check_bitmat(A, "A");
-    if (st != MELA_OK) return st;
+    if (st != MELA_OK || (st = check_variant(variant)) != MELA_OK) return st;
     const size_t n = A->cols;
     std::vector<int> piv;
     uint64_t *Q = nullptr;
     size_t ldq = 1;
-    {
-        Guard g(ctx);
-        if (A->rows && n) {
-            ElimResult res;
-            st = eliminate(ctx, A->data, A->ld, A->rows, n, variant, true, true, &res, &Q, &ldq);
-            if (st != MELA_OK) return st;
-            piv = res.pivots;
-        }
+    // one hold of the context over elimination and extraction: the pivot
+    // rows Q and the maps are context scratch that another call on this
+    // context would overwrite (mela_bitmat_alloc takes no context lock)
+    Guard g(ctx);
+    if (A->rows && n) {
+        ElimResult res;
+        st = eliminate(ctx, A->data, A->ld, A->rows, n, variant, true, true, &res, &Q, &ldq);
+        if (st != MELA_OK) return st;
+        piv = res.pivots;
     }
     const size_t f = n - piv.size();
     if ((st = mela_bitmat_alloc(ctx, n, f, Z)) != MELA_OK) return st;
     if (nullity) *nullity = f;
     if (n == 0 || f == 0) return MELA_OK;
-    Guard g(ctx);
     std::vector<int> rowmap, freecols;
     build_null_maps(n, piv, rowmap, freecols);
     int *d_rowmap, *d_free;

@@ -660,8 +660,12 @@ mela_status mela_batch_create(mela_ctx *ctx, const mela_bitmat *mats, size_t cou
                               mela_batch **out) {
     if (!ctx || !out || (count && !mats)) return set_error(MELA_ERR_INVALID, "null argument");
     *out = nullptr;
+    // the context's stream and cached arenas are used below
+    std::lock_guard<std::mutex> lk(ctx->mu);
     (void)hipSetDevice(ctx->device);
+    static std::mutex attr_mu;
     static bool attr_set = false;
+    std::lock_guard<std::mutex> alk(attr_mu);
     if (!attr_set) {
         MELA_HIP(hipFuncSetAttribute((const void *)k_batch_rref<16>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -776,8 +780,10 @@ mela_status mela_batch_create(mela_ctx *ctx, const mela_bitmat *mats, size_t cou
     return MELA_OK;
 }
 
-mela_status mela_batch_set_groups(mela_batch *b, const int *groups) {
-    if (!b || (b->count && !groups)) return set_error(MELA_ERR_INVALID, "null argument");
+namespace {
+
+// Bodies of the setters, for callers that already hold the context lock.
+mela_status set_groups_locked(mela_batch *b, const int *groups) {
     for (size_t i = 0; i < b->count; ++i)
         if (groups[i] < 1 || groups[i] > MAX_GROUP)
             return set_error(MELA_ERR_INVALID, "workgroups per matrix must be 1..8");
@@ -785,8 +791,7 @@ mela_status mela_batch_set_groups(mela_batch *b, const int *groups) {
     return batch_upload_plan(b);
 }
 
-mela_status mela_batch_set_modes(mela_batch *b, const int *modes) {
-    if (!b || !modes) return set_error(MELA_ERR_INVALID, "null argument");
+mela_status set_modes_locked(mela_batch *b, const int *modes) {
     for (size_t i = 0; i < b->count; ++i) b->mats[i].mode = modes[i] ? 1u : 0u;
     MELA_HIP(hipMemcpyAsync(b->d_mats, b->mats.data(), b->mats.size() * sizeof(BatchMat),
                             hipMemcpyHostToDevice, b->ctx->stream));
@@ -794,8 +799,33 @@ mela_status mela_batch_set_modes(mela_batch *b, const int *modes) {
     return MELA_OK;
 }
 
+mela_status autotune_locked(mela_batch *b, float *best_ms);
+
+}  // namespace
+
+mela_status mela_batch_set_groups(mela_batch *b, const int *groups) {
+    if (!b || (b->count && !groups)) return set_error(MELA_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(b->ctx->mu);
+    (void)hipSetDevice(b->ctx->device);
+    return set_groups_locked(b, groups);
+}
+
+mela_status mela_batch_set_modes(mela_batch *b, const int *modes) {
+    if (!b || !modes) return set_error(MELA_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(b->ctx->mu);
+    (void)hipSetDevice(b->ctx->device);
+    return set_modes_locked(b, modes);
+}
+
 mela_status mela_batch_autotune(mela_batch *b, float *best_ms) {
     if (!b) return set_error(MELA_ERR_INVALID, "null batch");
+    std::lock_guard<std::mutex> lk(b->ctx->mu);
+    (void)hipSetDevice(b->ctx->device);
+    return autotune_locked(b, best_ms);
+}
+
+namespace {
+mela_status autotune_locked(mela_batch *b, float *best_ms) {
     const size_t nb = b->mats.size();
     if (nb == 0) {
         if (best_ms) *best_ms = 0.f;
@@ -853,7 +883,7 @@ mela_status mela_batch_autotune(mela_batch *b, float *best_ms) {
     };
     mela_status st;
     std::vector<int> ones(nb, 1);
-    if ((st = mela_batch_set_groups(b, ones.data())) != MELA_OK) return st;
+    if ((st = set_groups_locked(b, ones.data())) != MELA_OK) return st;
     // search 1: tile width (16 / 32 words) x per-matrix mode assignment
     float best = 1e30f;
     size_t bi = 0;
@@ -861,7 +891,7 @@ mela_status mela_batch_autotune(mela_batch *b, float *best_ms) {
     for (int tw : {16, 32}) {
         b->tw = tw;
         for (size_t c = 0; c < cands.size(); ++c) {
-            if ((st = mela_batch_set_modes(b, cands[c].data())) != MELA_OK) return st;
+            if ((st = set_modes_locked(b, cands[c].data())) != MELA_OK) return st;
             float ms = 0.f;
             if ((st = timed(&ms)) != MELA_OK) return st;
             if (ms < best) {
@@ -872,7 +902,7 @@ mela_status mela_batch_autotune(mela_batch *b, float *best_ms) {
         }
     }
     b->tw = best_tw;
-    if ((st = mela_batch_set_modes(b, cands[bi].data())) != MELA_OK) return st;
+    if ((st = set_modes_locked(b, cands[bi].data())) != MELA_OK) return st;
     b->tuned = (int)bi;
     // search 2: workgroups per matrix for the head of the queue (the largest
     // matrices, whose single-workgroup time bounds the launch)
@@ -882,7 +912,7 @@ mela_status mela_batch_autotune(mela_batch *b, float *best_ms) {
             std::vector<int> gr(nb, 1);
             const size_t top = std::max<size_t>(1, (size_t)(frac * (double)nb));
             for (size_t q = 0; q < top && q < nb; ++q) gr[b->order[q]] = G;
-            if ((st = mela_batch_set_groups(b, gr.data())) != MELA_OK) return st;
+            if ((st = set_groups_locked(b, gr.data())) != MELA_OK) return st;
             float ms = 0.f;
             if ((st = timed(&ms)) != MELA_OK) return st;
             if (ms < best) {
@@ -894,10 +924,11 @@ mela_status mela_batch_autotune(mela_batch *b, float *best_ms) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     // also drops a graph captured before tuning (old tile width and grid)
-    if ((st = mela_batch_set_groups(b, gbest.data())) != MELA_OK) return st;
+    if ((st = set_groups_locked(b, gbest.data())) != MELA_OK) return st;
     if (best_ms) *best_ms = best;
     return MELA_OK;
 }
+}  // namespace
 
 mela_status mela_batch_run(mela_batch *b) {
     if (!b) return set_error(MELA_ERR_INVALID, "null batch");
@@ -962,10 +993,13 @@ mela_status mela_batch_results(mela_batch *b, uint32_t *ranks, mela_bitmat *null
 void mela_batch_destroy(mela_batch *b) {
     if (!b) return;
     if (b->ctx) {
+        std::lock_guard<std::mutex> lk(b->ctx->mu);
         (void)hipSetDevice(b->ctx->device);
         (void)hipStreamSynchronize(b->ctx->stream);
+        free_batch_device(b);
+    } else {
+        free_batch_device(b);
     }
-    free_batch_device(b);
     delete b;
 }
 

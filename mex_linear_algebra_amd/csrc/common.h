@@ -71,6 +71,9 @@ struct mela_ctx {
     void *pin = nullptr;
     size_t pin_bytes = 0;
     bool pin_is_malloc = false;  // hipHostMalloc failed: ordinary host memory
+    // staging in ordinary host memory from the start (MELA_FORCE_PAGEABLE=1 at
+    // context creation): the path taken when no pinned memory can be had
+    bool force_pageable = false;
 };
 
 namespace mela {

@@ -79,7 +79,8 @@ mela_status pinned_get(mela_ctx *ctx, size_t bytes, uint8_t **out) {
         pinned_release(ctx);
         const size_t want = (bytes + ((size_t)1 << 20) - 1) & ~(((size_t)1 << 20) - 1);
         void *p = nullptr;
-        e = hipHostMalloc(&p, want, hipHostMallocDefault);
+        e = ctx->force_pageable ? hipErrorOutOfMemory
+                                : hipHostMalloc(&p, want, hipHostMallocDefault);
         if (e != hipSuccess) {
             // no pinned memory to be had: ordinary host memory still works
             // (the copies then go through the runtime's own staging)
@@ -120,6 +121,10 @@ mela_status mela_ctx_create(int device, mela_ctx **out) {
     mela_ctx *ctx = new mela_ctx();
     ctx->device = device;
     ctx->cu_count = prop.multiProcessorCount;
+    {
+        const char *fp = std::getenv("MELA_FORCE_PAGEABLE");
+        ctx->force_pageable = fp && *fp && std::strcmp(fp, "0") != 0;
+    }
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete ctx;

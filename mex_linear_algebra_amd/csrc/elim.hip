@@ -172,7 +172,14 @@ __device__ __forceinline__ void lds_barrier() {
 // the pivot row of every set bit of its panel word.  Word g of the earlier
 // pivot rows is left to the next panel's sweep (or, for the last panel, to
 // k_update_panelword) so that every tile reads an unmodified multiplier.
-template <int TILE, bool GE>
+// MODE: SW_GE walks the pivot rows, SW_M4RI looks up the tables, SW_AUTO
+// decides per row batch of a wave: a multiplier with more than
+// AUTO_DENSE_BITS set bits on any of the wave's rows takes the 16 lookups,
+// otherwise the walk (reading the pivot rows out of the tables, where entry
+// 1 << b of table t is pivot row 4t + b).
+enum { SW_GE = 0, SW_M4RI = 1, SW_AUTO = 2 };
+constexpr int AUTO_DENSE_BITS = 8;
+template <int TILE, int MODE>
 __global__ void __launch_bounds__(UP_WAVES * 64, 64 / TILE)
 k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
         uint64_t *__restrict__ Q, size_t ldq, const ElimMeta *__restrict__ meta,
@@ -337,7 +344,11 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
     }
     // GE: the pivot rows themselves, by pivot bit, over the tables' LDS
     uint64_t(*P)[TILE] = reinterpret_cast<uint64_t(*)[TILE]>(&T[0][0][0]);
-    if (GE) {
+    // pivot row c, word wl: its own slot (GE) or its table entry (AUTO)
+    auto prow = [&](int c) -> uint64_t {
+        return MODE == SW_GE ? P[c][wl] : T[1 << (c & 3)][c >> 2][wl];
+    };
+    if (MODE == SW_GE) {
 #pragma unroll
         for (int k = 0; k < TPT; ++k)
 #pragma unroll
@@ -350,14 +361,17 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
     // GE: the loop runs while any lane has rows (the wave's row halves end one
     // iteration apart at most), so the walk's wave reductions see every lane
     // active; a half past the end has zero multipliers
-    for (; GE ? __ballot(q0 < total) != 0ull : q0 < total; q0 += stride) {
+    for (; MODE != SW_M4RI ? __ballot(q0 < total) != 0ull : q0 < total; q0 += stride) {
         // rows whose multiplier is zero on the whole wave are skipped (a
         // wave-uniform branch); stores come after all lookups, so no row
         // waits for another row's store
 #pragma unroll
         for (int j = 0; j < R; ++j) {
             if (__ballot(x[j] != 0ull) == 0ull) continue;
-            if (GE) {
+            const bool walk =
+                MODE == SW_GE ||
+                (MODE == SW_AUTO && __ballot(popc64(x[j]) > AUTO_DENSE_BITS) == 0ull);
+            if (walk) {
                 // one walk over the union of the wave's multipliers (its
                 // TILE-lane rows), in scalar registers; a lane XORs pivot row
                 // c only where its own multiplier has bit c (the sign-extended
@@ -374,7 +388,7 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
                         const bool two = h != 0;
                         const int c2 = two ? __builtin_ctz(h) : c1;
                         if (two) h &= h - 1;
-                        const uint64_t p1 = P[base + c1][wl], p2 = P[base + c2][wl];
+                        const uint64_t p1 = prow(base + c1), p2 = prow(base + c2);
                         const uint32_t m1 = (uint32_t)__builtin_amdgcn_sbfe((int)xs, c1, 1);
                         const uint32_t m2 =
                             two ? (uint32_t)__builtin_amdgcn_sbfe((int)xs, c2, 1) : 0u;
@@ -404,7 +418,7 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
             }
         if (q0 + stride < total) {
             load_rows(q0 + stride, false);
-        } else if (GE) {
+        } else if (MODE != SW_M4RI) {
 #pragma unroll
             for (int j = 0; j < R; ++j) x[j] = 0ull;
         }
@@ -577,11 +591,15 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
                 if (ty > ty_max) ty = ty_max;
                 if (ty < 1) ty = 1;
                 if (variant == MELA_ELIM_GE)
-                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sweep<SWEEP_TILE, true>), dim3(tx, ty),
+                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sweep<SWEEP_TILE, SW_GE>), dim3(tx, ty),
+                                       dim3(UP_WAVES * 64), 0, s, Mw, ld, act, Q, ld, meta, swept, panelw, (int)g,
+                                       (int)wc, rref ? 1 : 0, (int)m);
+                else if (variant == MELA_ELIM_M4RI)
+                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sweep<SWEEP_TILE, SW_M4RI>), dim3(tx, ty),
                                        dim3(UP_WAVES * 64), 0, s, Mw, ld, act, Q, ld, meta, swept, panelw, (int)g,
                                        (int)wc, rref ? 1 : 0, (int)m);
                 else
-                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sweep<SWEEP_TILE, false>), dim3(tx, ty),
+                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sweep<SWEEP_TILE, SW_AUTO>), dim3(tx, ty),
                                        dim3(UP_WAVES * 64), 0, s, Mw, ld, act, Q, ld, meta, swept, panelw, (int)g,
                                        (int)wc, rref ? 1 : 0, (int)m);
             }

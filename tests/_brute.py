@@ -93,14 +93,16 @@ def matrix_from_index_4x4(i):
     return np.array([[(i >> (4 * r + c)) & 1 for c in range(4)] for r in range(4)], dtype=bool)
 
 
-def parity_rows(P, x):
-    """y_i = parity(popcount(P_i AND x)) for packed rows P (uint64) and packed x."""
-    v = np.bitwise_and(P, x[None, : P.shape[1]])
-    # fold 64 -> 1 bit parity with XOR shifts
-    for s in (32, 16, 8, 4, 2, 1):
-        v = v ^ (v >> np.uint64(s))
-    v = v & np.uint64(1)
-    return np.bitwise_xor.reduce(v, axis=1).astype(np.uint8)
+def parity_rows(P, x, chunk=4096):
+    """y_i = parity(popcount(P_i AND x)) for packed rows P (uint64) and packed
+    x, in row chunks (bounded temporaries for 128 MiB operands)."""
+    P = np.asarray(P)
+    xw = np.asarray(x, dtype=np.uint64)[None, : P.shape[1]]
+    out = np.zeros(P.shape[0], dtype=np.uint8)
+    for r0 in range(0, P.shape[0], chunk):
+        v = np.bitwise_count(np.bitwise_and(P[r0:r0 + chunk], xw))
+        out[r0:r0 + chunk] = (v.sum(axis=1, dtype=np.int64) & 1).astype(np.uint8)
+    return out
 
 
 def pack_bits(bits):
@@ -110,6 +112,22 @@ def pack_bits(bits):
     pad = np.zeros(w * 64, dtype=np.uint8)
     pad[: len(bits)] = bits
     return np.packbits(pad, bitorder="little").view("<u8").astype(np.uint64)
+
+
+def freivalds_vectors(Ap, Bp, n, trials, seed):
+    """[(packed x, A (B x))] for `trials` random x: shared by the checks of
+    several products of the same A and B."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(trials):
+        xp = pack_bits(rng.integers(0, 2, n).astype(np.uint8))
+        out.append((xp, parity_rows(Ap, pack_bits(parity_rows(Bp, xp)))))
+    return out
+
+
+def freivalds_check(Cp, vectors):
+    """C x == A (B x) for every precomputed (x, A (B x)) pair."""
+    return all(np.array_equal(parity_rows(Cp, xp), abx) for xp, abx in vectors)
 
 
 def freivalds(Ap, Bp, Cp, k, n, trials=4, seed=0):

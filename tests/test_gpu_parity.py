@@ -360,60 +360,3 @@ def test_error_messages(mela):
         mela.mela_null_gf2(A.astype(np.uint16))
     with pytest.raises(mela.MelaError, match="Need two input matrices A,B."):
         mela.mela_matmul_gf2(A, A, A)
-
-
-def test_matmul_c3_rectangular_freivalds(ctx, mela):
-    # 65536 x 32768 * 32768 x 4096 (the rectangular large-product shape)
-    m, k, n = 65536, 32768, 4096
-    print(f"[matmul C3 rect] {m}x{k}x{n}", flush=True)
-    dA, dB, dC = ctx.bitmat(m, k), ctx.bitmat(k, n), ctx.bitmat(m, n)
-    ctx.gen_bernoulli(31, dA)
-    ctx.gen_bernoulli(32, dB)
-    Ap, Bp = ctx.download(dA), ctx.download(dB)
-    for algo in (mela.ALGO_AUTO, mela.ALGO_M4RM, mela.ALGO_MFMA_FP4):
-        ctx.matmul_packed(dA, dB, dC, algo)
-        assert bf.freivalds(Ap, Bp, ctx.download(dC), k, n, trials=3, seed=algo), algo
-    for M in (dA, dB, dC):
-        ctx.free(M)
-
-
-def test_c2_full_size_rank_and_rref(ctx, orc, mela):
-    # 16384 x 16384 = L * R (inner 12000): rank and pivot columns vs the oracle
-    L, R = ctx.bitmat(16384, 12000), ctx.bitmat(12000, 16384)
-    ctx.gen_bernoulli(11, L)
-    ctx.gen_bernoulli(12, R)
-    H = ctx.bitmat(16384, 16384)
-    ctx.matmul_packed(L, R, H)
-    ctx.free(L)
-    ctx.free(R)
-    Hp = ctx.download(H)
-    print("[C2a] oracle rref", flush=True)
-    M, opiv = orc.rref_packed(Hp, 16384, 16384, "gf2")
-    assert len(opiv) == 12000
-    for variant in (mela.ELIM_GE, mela.ELIM_M4RI):
-        assert ctx.rank_packed(H, variant) == 12000
-        Rm, piv = ctx.rref_packed(H, variant)
-        assert list(piv) == list(opiv)
-        ctx.free(Rm)
-    # canonical null basis: A * Z = 0 on the device, Z(free,:) = I
-    Z = ctx.null_packed(H)
-    assert Z.cols == 16384 - 12000
-    Cm = ctx.bitmat(16384, Z.cols)
-    ctx.matmul_packed(H, Z, Cm)
-    assert not ctx.download(Cm).any()
-    Zp = ctx.download(Z)
-    free = np.setdiff1d(np.arange(16384), np.array(opiv))
-    Zb = np.unpackbits(Zp[free].view(np.uint8), axis=1, bitorder="little")[:, : Z.cols]
-    assert np.array_equal(Zb.astype(bool), np.eye(Z.cols, dtype=bool))
-    for X in (H, Z, Cm):
-        ctx.free(X)
-
-
-def test_c2b_full_size_rank(ctx, orc, mela):
-    D = ctx.bitmat(16384, 20000)
-    ctx.gen_bernoulli(13, D)
-    want = orc.rank_packed(ctx.download(D), 16384, 20000, "m4ri")
-    assert want == 16384
-    for variant in (mela.ELIM_GE, mela.ELIM_M4RI):
-        assert ctx.rank_packed(D, variant) == want
-    ctx.free(D)
