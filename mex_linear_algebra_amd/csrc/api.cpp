@@ -34,7 +34,7 @@ mela_status resolve(mela_ctx **ctx) {
 }
 
 struct Guard {
-    std::lock_guard<std::mutex> lk;
+    CtxLock lk;
     explicit Guard(mela_ctx *c) : lk(c->mu) { (void)hipSetDevice(c->device); }
 };
 

@@ -661,7 +661,7 @@ mela_status mela_batch_create(mela_ctx *ctx, const mela_bitmat *mats, size_t cou
     if (!ctx || !out || (count && !mats)) return set_error(MELA_ERR_INVALID, "null argument");
     *out = nullptr;
     // the context's stream and cached arenas are used below
-    std::lock_guard<std::mutex> lk(ctx->mu);
+    CtxLock lk(ctx->mu);
     (void)hipSetDevice(ctx->device);
     static std::mutex attr_mu;
     static bool attr_set = false;
@@ -805,21 +805,21 @@ mela_status autotune_locked(mela_batch *b, float *best_ms);
 
 mela_status mela_batch_set_groups(mela_batch *b, const int *groups) {
     if (!b || (b->count && !groups)) return set_error(MELA_ERR_INVALID, "null argument");
-    std::lock_guard<std::mutex> lk(b->ctx->mu);
+    CtxLock lk(b->ctx->mu);
     (void)hipSetDevice(b->ctx->device);
     return set_groups_locked(b, groups);
 }
 
 mela_status mela_batch_set_modes(mela_batch *b, const int *modes) {
     if (!b || !modes) return set_error(MELA_ERR_INVALID, "null argument");
-    std::lock_guard<std::mutex> lk(b->ctx->mu);
+    CtxLock lk(b->ctx->mu);
     (void)hipSetDevice(b->ctx->device);
     return set_modes_locked(b, modes);
 }
 
 mela_status mela_batch_autotune(mela_batch *b, float *best_ms) {
     if (!b) return set_error(MELA_ERR_INVALID, "null batch");
-    std::lock_guard<std::mutex> lk(b->ctx->mu);
+    CtxLock lk(b->ctx->mu);
     (void)hipSetDevice(b->ctx->device);
     return autotune_locked(b, best_ms);
 }
@@ -933,7 +933,7 @@ mela_status autotune_locked(mela_batch *b, float *best_ms) {
 mela_status mela_batch_run(mela_batch *b) {
     if (!b) return set_error(MELA_ERR_INVALID, "null batch");
     mela_ctx *ctx = b->ctx;
-    std::lock_guard<std::mutex> lk(ctx->mu);
+    CtxLock lk(ctx->mu);
     (void)hipSetDevice(ctx->device);
     hipStream_t s = ctx->stream;
     if (!b->mats.empty()) {
@@ -961,12 +961,12 @@ mela_status mela_batch_results(mela_batch *b, uint32_t *ranks, mela_bitmat *null
     if (!b) return set_error(MELA_ERR_INVALID, "null batch");
     mela_ctx *ctx = b->ctx;
     {
-        std::lock_guard<std::mutex> lk(ctx->mu);
+        CtxLock lk(ctx->mu);
         MELA_HIP(hipStreamSynchronize(ctx->stream));
     }
     std::vector<uint32_t> r(b->mats.size());
     if (!r.empty()) {
-        std::lock_guard<std::mutex> lk(ctx->mu);
+        CtxLock lk(ctx->mu);
         int qerr[2] = {0, 0};
         MELA_HIP(hipMemcpyAsync(r.data(), b->d_ranks, r.size() * sizeof(uint32_t),
                                 hipMemcpyDeviceToHost, ctx->stream));
@@ -993,7 +993,7 @@ mela_status mela_batch_results(mela_batch *b, uint32_t *ranks, mela_bitmat *null
 void mela_batch_destroy(mela_batch *b) {
     if (!b) return;
     if (b->ctx) {
-        std::lock_guard<std::mutex> lk(b->ctx->mu);
+        CtxLock lk(b->ctx->mu);
         (void)hipSetDevice(b->ctx->device);
         (void)hipStreamSynchronize(b->ctx->stream);
         free_batch_device(b);

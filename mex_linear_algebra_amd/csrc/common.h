@@ -56,7 +56,9 @@ struct ElimGraph {
 struct mela_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    std::mutex mu;  // serialises calls on one context
+    // serialises calls on one context: every call that uses the context's
+    // stream or scratch holds it (recursive: entry points call each other)
+    std::recursive_mutex mu;
     std::vector<mela::Scratch> scratch;  // indexed by slot
     int cu_count = 256;
     hipEvent_t ev = nullptr;
@@ -77,6 +79,9 @@ struct mela_ctx {
 };
 
 namespace mela {
+
+// The context lock (mela_ctx::mu), held for the whole of a call.
+using CtxLock = std::lock_guard<std::recursive_mutex>;
 
 // Records an event on the context stream before (start) or after the
 // dominant kernel of a product when profiling is on; never used inside a

@@ -176,13 +176,15 @@ void *mela_ctx_stream(mela_ctx *ctx) { return ctx ? (void *)ctx->stream : nullpt
 
 mela_status mela_ctx_sync(mela_ctx *ctx) {
     if (!ctx) return set_error(MELA_ERR_INVALID, "null context");
+    CtxLock lk(ctx->mu);
+    MELA_HIP(hipSetDevice(ctx->device));
     MELA_HIP(hipStreamSynchronize(ctx->stream));
     return MELA_OK;
 }
 
 mela_status mela_ctx_profile_begin(mela_ctx *ctx) {
     if (!ctx) return set_error(MELA_ERR_INVALID, "null context");
-    std::lock_guard<std::mutex> lk(ctx->mu);
+    CtxLock lk(ctx->mu);
     MELA_HIP(hipSetDevice(ctx->device));
     prof_clear(ctx);
     ctx->prof_on = true;
@@ -191,7 +193,7 @@ mela_status mela_ctx_profile_begin(mela_ctx *ctx) {
 
 mela_status mela_ctx_profile_end(mela_ctx *ctx, double *total_ms, int *launches) {
     if (!ctx || !total_ms || !launches) return set_error(MELA_ERR_INVALID, "null argument");
-    std::lock_guard<std::mutex> lk(ctx->mu);
+    CtxLock lk(ctx->mu);
     MELA_HIP(hipSetDevice(ctx->device));
     ctx->prof_on = false;
     MELA_HIP(hipStreamSynchronize(ctx->stream));
@@ -215,6 +217,7 @@ mela_status mela_ctx_profile_end(mela_ctx *ctx, double *total_ms, int *launches)
 
 mela_status mela_dev_alloc(mela_ctx *ctx, size_t bytes, void **dptr) {
     if (!ctx || !dptr) return set_error(MELA_ERR_INVALID, "null argument");
+    CtxLock lk(ctx->mu);
     MELA_HIP(hipSetDevice(ctx->device));
     hipError_t e = hipMalloc(dptr, bytes ? bytes : 16);
     if (e != hipSuccess) {
@@ -228,6 +231,8 @@ mela_status mela_dev_alloc(mela_ctx *ctx, size_t bytes, void **dptr) {
 mela_status mela_dev_free(mela_ctx *ctx, void *dptr) {
     if (!ctx) return set_error(MELA_ERR_INVALID, "null context");
     if (dptr) {
+        CtxLock lk(ctx->mu);
+        MELA_HIP(hipSetDevice(ctx->device));
         MELA_HIP(hipStreamSynchronize(ctx->stream));
         MELA_HIP(hipFree(dptr));
     }
@@ -237,6 +242,8 @@ mela_status mela_dev_free(mela_ctx *ctx, void *dptr) {
 mela_status mela_copy_h2d(mela_ctx *ctx, void *dst, const void *src, size_t bytes) {
     if (!ctx) return set_error(MELA_ERR_INVALID, "null context");
     if (bytes == 0) return MELA_OK;
+    CtxLock lk(ctx->mu);
+    MELA_HIP(hipSetDevice(ctx->device));
     MELA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
     MELA_HIP(hipStreamSynchronize(ctx->stream));
     return MELA_OK;
@@ -245,6 +252,8 @@ mela_status mela_copy_h2d(mela_ctx *ctx, void *dst, const void *src, size_t byte
 mela_status mela_copy_d2h(mela_ctx *ctx, void *dst, const void *src, size_t bytes) {
     if (!ctx) return set_error(MELA_ERR_INVALID, "null context");
     if (bytes == 0) return MELA_OK;
+    CtxLock lk(ctx->mu);
+    MELA_HIP(hipSetDevice(ctx->device));
     MELA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
     MELA_HIP(hipStreamSynchronize(ctx->stream));
     return MELA_OK;
@@ -252,11 +261,14 @@ mela_status mela_copy_d2h(mela_ctx *ctx, void *dst, const void *src, size_t byte
 
 mela_status mela_copy_d2d(mela_ctx *ctx, void *dst, const void *src, size_t bytes) {
     if (!ctx) return set_error(MELA_ERR_INVALID, "null context");
+    CtxLock lk(ctx->mu);
+    MELA_HIP(hipSetDevice(ctx->device));
     return launch_copy(ctx->stream, dst, src, bytes);
 }
 
 mela_status mela_bitmat_alloc(mela_ctx *ctx, size_t rows, size_t cols, mela_bitmat *out) {
     if (!ctx || !out) return set_error(MELA_ERR_INVALID, "null argument");
+    CtxLock lk(ctx->mu);
     const size_t ld = words64(cols) ? words64(cols) : 1;
     void *p = nullptr;
     mela_status st = mela_dev_alloc(ctx, (rows ? rows : 1) * ld * 8, &p);
@@ -279,6 +291,8 @@ void mela_bitmat_free(mela_ctx *ctx, mela_bitmat *m) {
 mela_status mela_gen_bernoulli(mela_ctx *ctx, uint64_t seed, mela_bitmat *M) {
     if (!ctx || !M || (!M->data && M->rows)) return set_error(MELA_ERR_INVALID, "null argument");
     if (M->ld < words64(M->cols)) return set_error(MELA_ERR_INVALID, "ld smaller than the row width");
+    CtxLock lk(ctx->mu);
+    MELA_HIP(hipSetDevice(ctx->device));
     return launch_gen_bernoulli(ctx->stream, seed, M->data, M->rows, M->cols, M->ld);
 }
 
