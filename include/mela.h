@@ -178,6 +178,20 @@ enum { MELA_ELIM_GE = 0, MELA_ELIM_M4RI = 1, MELA_ELIM_AUTO = 2 };
 /* rank of A (A is left unchanged; work memory comes from the context) */
 mela_status mela_rank_packed(mela_ctx *ctx, const mela_bitmat *A, int variant,
                              size_t *rank);
+/* Allocation-free rank.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same result as mela_rank_packed,
+ * but the work memory (working copy, pivot rows, metadata, active list) is a
+ * fixed 32 MiB device array of the library, one per GPU, held until the
+ * process ends; the call makes no hipMalloc / hipFree and no other device
+ * allocation.  A matrix whose work memory exceeds 32 MiB (about
+ * 8 * ceil16(ceil(cols/64)) * (rows + min(rows, cols)) bytes plus
+ * 4 * rows + 25 KiB: 16384 x 8192 fits, 16384 x 16384 does not) is
+ * rejected with MELA_ERR_INVALID.  Calls on one GPU share that array: the library runs
+ * them one at a time within a process (they synchronise their stream), and
+ * the array keeps the last call's contents. */
+mela_status mela_rank_packed_noalloc(mela_ctx *ctx, const mela_bitmat *A, int variant,
+                                     size_t *rank);
 /* reduced row echelon form: R (min(m,n) x cols, device, may be NULL) gets the
  * rank nonzero rows in pivot order; pivots (host, min(m,n) ints, may be NULL)
  * the pivot columns ascending. */

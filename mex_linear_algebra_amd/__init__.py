@@ -151,6 +151,8 @@ def load():
         L.mela_matmul_packed.restype = st
         L.mela_rank_packed.argtypes = [ctxp, bm, C.c_int, C.POINTER(sz)]
         L.mela_rank_packed.restype = st
+        L.mela_rank_packed_noalloc.argtypes = [ctxp, bm, C.c_int, C.POINTER(sz)]
+        L.mela_rank_packed_noalloc.restype = st
         L.mela_rref_packed.argtypes = [ctxp, bm, C.c_int, bm, C.POINTER(C.c_int), C.POINTER(sz)]
         L.mela_rref_packed.restype = st
         L.mela_null_packed.argtypes = [ctxp, bm, C.c_int, bm, C.POINTER(sz)]
@@ -307,6 +309,13 @@ class Context:
     def rank_packed(self, A: BitMatrix, variant: int = ELIM_AUTO) -> int:
         r = C.c_size_t()
         _check(self.lib.mela_rank_packed(self.handle, C.byref(A), variant, C.byref(r)))
+        return r.value
+
+    def rank_packed_noalloc(self, A: BitMatrix, variant: int = ELIM_AUTO) -> int:
+        """rank through the allocation-free entry point (fixed 32 MiB device
+        work array, no device allocation in the call)."""
+        r = C.c_size_t()
+        _check(self.lib.mela_rank_packed_noalloc(self.handle, C.byref(A), variant, C.byref(r)))
         return r.value
 
     def rref_packed(self, A: BitMatrix, variant: int = ELIM_AUTO):

@@ -76,6 +76,10 @@ struct mela_ctx {
     // staging in ordinary host memory from the start (MELA_FORCE_PAGEABLE=1 at
     // context creation): the path taken when no pinned memory can be had
     bool force_pageable = false;
+    // this device's allocation-free scratch (mela_rank_packed_noalloc), or
+    // nullptr with the reason in noalloc_err
+    void *noalloc_scratch = nullptr;
+    std::string noalloc_err;
 };
 
 namespace mela {
@@ -150,11 +154,32 @@ struct ElimResult {
     size_t rank = 0;
     std::vector<int> pivots;  // filled when want_pivots
 };
+// Caller-provided work buffers of one elimination (the layout of
+// elim_layout); with them eliminate() takes nothing from the context's
+// scratch and neither captures nor replays a graph.
+struct ElimBufs {
+    uint64_t *M = nullptr;   // working copy, m x ld words
+    uint64_t *Q = nullptr;   // pivot rows, min(m, n) x ld words
+    void *meta = nullptr;    // metadata, pivot columns, sweep tags, panel words
+    uint32_t *act = nullptr; // active-row list, m entries
+};
+// Byte offsets of the four buffers in one region, and its size.
+struct ElimLayout {
+    size_t ld = 0;  // words per working row (a multiple of 16)
+    size_t meta_bytes = 0, act_off = 0, m_off = 0, q_off = 0, total = 0;
+};
+ElimLayout elim_layout(size_t m, size_t n);
 // Runs elimination on a packed copy owned by the call.  mode_rref: full RREF
 // (also reduces earlier pivot rows); Q receives pivot rows in order.
+// fixed: work buffers given by the caller (see ElimBufs), else the context's.
 mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m,
                       size_t n, int variant, bool rref, bool want_pivots,
-                      ElimResult *res, uint64_t **Q_out, size_t *ldq_out);
+                      ElimResult *res, uint64_t **Q_out, size_t *ldq_out,
+                      const ElimBufs *fixed = nullptr);
+// Address of this device's allocation-free scratch array (elim.hip), looked
+// up once per device; the device must be current.  nullptr and a message on
+// failure.
+void *noalloc_scratch_for_device(int device, std::string *err);
 // packed null basis (n x f, row-major, ldz words, device) from the RREF rows;
 // returns with the stream idle
 mela_status launch_null_packed(mela_ctx *ctx, const uint64_t *Q, size_t ldq, const int *rowmap,

@@ -131,6 +131,8 @@ mela_status mela_ctx_create(int device, mela_ctx **out) {
         return hip_fail(e, "hipStreamCreateWithFlags");
     }
     ctx->scratch.resize(SLOT_COUNT);
+    // the allocation-free entry point's device array, once per device
+    ctx->noalloc_scratch = noalloc_scratch_for_device(device, &ctx->noalloc_err);
     *out = ctx;
     return MELA_OK;
 }

@@ -24,7 +24,17 @@
 //                   also brings word g-1 of the earlier pivot rows up to date
 //                   with panel g-1's pivot rows;
 //   k_update_panelword (RREF) the same for the last panel's word.
+#include <cstdio>
+#include <mutex>
+
 #include "elim_device.h"
+
+// Work memory of the allocation-free rank entry point
+// (mela_rank_packed_noalloc): one copy per device, held until the process
+// ends.  Host code never uses this name as an address; it takes the device
+// address from hipGetSymbolAddress (noalloc_scratch_for_device).
+constexpr size_t kNoAllocBytes = (size_t)32 << 20;
+static __device__ __align__(256) uint64_t scratch[kNoAllocBytes / 8];
 
 namespace mela {
 
@@ -518,9 +528,31 @@ k_null_packed(const uint64_t *__restrict__ Q, size_t ldq, const int *__restrict_
 
 }  // namespace
 
+namespace {
+// Offsets inside the metadata buffer: ElimMeta, the pivot columns, the
+// sweep tags, the next selection's panel words.
+size_t meta_swept_off(size_t rmax) {
+    return (sizeof(ElimMeta) + (rmax + 1) * sizeof(int) + 15) & ~(size_t)15;
+}
+size_t meta_panelw_off(size_t rmax) { return meta_swept_off(rmax) + NSWEPT * sizeof(unsigned); }
+size_t round256(size_t x) { return (x + 255) & ~(size_t)255; }
+}  // namespace
+
+ElimLayout elim_layout(size_t m, size_t n) {
+    ElimLayout L;
+    const size_t wc = words64(n), rmax = m < n ? m : n;
+    L.ld = (wc + 15) & ~(size_t)15;
+    L.meta_bytes = round256(meta_panelw_off(rmax) + PS_THREADS * sizeof(uint64_t));
+    L.act_off = L.meta_bytes;
+    L.m_off = round256(L.act_off + m * sizeof(uint32_t));
+    L.q_off = round256(L.m_off + m * L.ld * 8);
+    L.total = round256(L.q_off + rmax * L.ld * 8);
+    return L;
+}
+
 mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, size_t n,
                       int variant, bool rref, bool want_pivots, ElimResult *res,
-                      uint64_t **Q_out, size_t *ldq_out) {
+                      uint64_t **Q_out, size_t *ldq_out, const ElimBufs *fixed) {
     hipStream_t s = ctx->stream;
     res->rank = 0;
     res->pivots.clear();
@@ -537,14 +569,22 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
 
     void *pM = nullptr, *pQ = nullptr, *pMeta = nullptr, *pAct = nullptr;
     mela_status st;
-    if ((st = scratch_get(ctx, SLOT_ELIM_M, m * ld * 8, &pM)) != MELA_OK) return st;
-    if ((st = scratch_get(ctx, SLOT_ELIM_Q, rmax * ld * 8, &pQ)) != MELA_OK) return st;
-    const size_t swept_off = (sizeof(ElimMeta) + (rmax + 1) * sizeof(int) + 15) & ~(size_t)15;
-    const size_t panelw_off = swept_off + NSWEPT * sizeof(unsigned);
-    if ((st = scratch_get(ctx, SLOT_ELIM_META, panelw_off + PS_THREADS * sizeof(uint64_t), &pMeta)) !=
-        MELA_OK)
-        return st;
-    if ((st = scratch_get(ctx, SLOT_ELIM_ACT, m * sizeof(uint32_t), &pAct)) != MELA_OK) return st;
+    const size_t swept_off = meta_swept_off(rmax);
+    const size_t panelw_off = meta_panelw_off(rmax);
+    if (fixed) {
+        pM = fixed->M;
+        pQ = fixed->Q;
+        pMeta = fixed->meta;
+        pAct = fixed->act;
+    } else {
+        if ((st = scratch_get(ctx, SLOT_ELIM_M, m * ld * 8, &pM)) != MELA_OK) return st;
+        if ((st = scratch_get(ctx, SLOT_ELIM_Q, rmax * ld * 8, &pQ)) != MELA_OK) return st;
+        if ((st = scratch_get(ctx, SLOT_ELIM_META, panelw_off + PS_THREADS * sizeof(uint64_t),
+                              &pMeta)) != MELA_OK)
+            return st;
+        if ((st = scratch_get(ctx, SLOT_ELIM_ACT, m * sizeof(uint32_t), &pAct)) != MELA_OK)
+            return st;
+    }
     uint64_t *Mw = (uint64_t *)pM, *Q = (uint64_t *)pQ;
     ElimMeta *meta = (ElimMeta *)pMeta;
     int *pivcols = (int *)((char *)pMeta + sizeof(ElimMeta));
@@ -612,6 +652,26 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
         MELA_HIP(hipGetLastError());
         return MELA_OK;
     };
+    ElimMeta hm;
+    auto read_meta = [&]() -> mela_status {
+        MELA_HIP(hipMemcpyAsync(&hm, meta, sizeof(ElimMeta), hipMemcpyDeviceToHost, s));
+        MELA_HIP(hipStreamSynchronize(s));
+        return MELA_OK;
+    };
+    if (fixed) {
+        // caller's buffers: every panel enqueued directly, no graph
+        if ((st = enqueue_panels(0, wc, true)) != MELA_OK) return st;
+        if ((st = read_meta()) != MELA_OK) return st;
+        res->rank = (size_t)hm.r;
+        if (want_pivots && hm.r > 0) {
+            res->pivots.resize((size_t)hm.r);
+            MELA_HIP(hipMemcpyAsync(res->pivots.data(), pivcols, (size_t)hm.r * sizeof(int),
+                                    hipMemcpyDeviceToHost, s));
+            MELA_HIP(hipStreamSynchronize(s));
+        }
+        if (Q_out) *Q_out = Q;
+        return MELA_OK;
+    }
     // replay a captured sequence for a shape seen before (the work buffers are
     // the context's cached scratch, so their addresses repeat)
     ElimGraph *eg = nullptr;
@@ -650,12 +710,6 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
         eg->g_end = g_end;
         MELA_HIP(hipGraphInstantiate(&eg->exec, graph, nullptr, nullptr, 0));
     }
-    ElimMeta hm;
-    auto read_meta = [&]() -> mela_status {
-        MELA_HIP(hipMemcpyAsync(&hm, meta, sizeof(ElimMeta), hipMemcpyDeviceToHost, s));
-        MELA_HIP(hipStreamSynchronize(s));
-        return MELA_OK;
-    };
     if (eg->exec) {
         MELA_HIP(hipGraphLaunch(eg->exec, s));
         if ((st = read_meta()) != MELA_OK) return st;
@@ -726,4 +780,77 @@ mela_status launch_null_packed(mela_ctx *ctx, const uint64_t *Q, size_t ldq, con
     return MELA_OK;
 }
 
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+bool g_noalloc_done[kMaxDevices];
+void *g_noalloc_ptr[kMaxDevices];
+std::string g_noalloc_err[kMaxDevices];
+// one allocation-free call at a time per device: they share its array
+std::mutex g_noalloc_call_mu[kMaxDevices];
+}  // namespace
+
+void *noalloc_scratch_for_device(int device, std::string *err) {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (device < 0 || device >= kMaxDevices) {
+        if (err) *err = "device index beyond the allocation-free table";
+        return nullptr;
+    }
+    if (!g_noalloc_done[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            g_noalloc_err[device] = std::string("hipGetSymbolAddress of the allocation-free "
+                                                "scratch failed: ") + hipGetErrorString(e);
+            ptr = nullptr;
+        }
+        g_noalloc_ptr[device] = ptr;
+        g_noalloc_done[device] = true;
+    }
+    if (!g_noalloc_ptr[device] && err) *err = g_noalloc_err[device];
+    return g_noalloc_ptr[device];
+}
+
 }  // namespace mela
+
+using namespace mela;
+
+extern "C" mela_status mela_rank_packed_noalloc(mela_ctx *ctx, const mela_bitmat *A, int variant,
+                                                size_t *rank) {
+    if (!ctx || !rank || !A) return set_error(MELA_ERR_INVALID, "null argument");
+    if (A->rows && A->cols && !A->data) return set_error(MELA_ERR_INVALID, "null data in A");
+    if (A->cols && A->ld < words64(A->cols)) return set_error(MELA_ERR_INVALID, "ld too small in A");
+    if (variant != MELA_ELIM_GE && variant != MELA_ELIM_M4RI && variant != MELA_ELIM_AUTO)
+        return set_error(MELA_ERR_INVALID, "unknown elimination variant");
+    *rank = 0;
+    const size_t m = A->rows, n = A->cols;
+    if (m == 0 || n == 0) return MELA_OK;
+    if (m > (size_t)INT32_MAX || n > (size_t)INT32_MAX)
+        return set_error(MELA_ERR_INVALID, "matrix dimension exceeds 2^31-1");
+    const ElimLayout L = elim_layout(m, n);
+    if (L.total > kNoAllocBytes) {
+        char buf[256];
+        std::snprintf(buf, sizeof buf,
+                      "matrix %zu x %zu needs %zu bytes of work memory; the allocation-free rank "
+                      "holds at most %zu (use mela_rank_packed)",
+                      m, n, L.total, kNoAllocBytes);
+        return set_error(MELA_ERR_INVALID, buf);
+    }
+    CtxLock lk(ctx->mu);
+    MELA_HIP(hipSetDevice(ctx->device));
+    char *base = (char *)ctx->noalloc_scratch;
+    if (!base) return set_error(MELA_ERR_DEVICE, ctx->noalloc_err, "mela:device");
+    std::lock_guard<std::mutex> lk2(g_noalloc_call_mu[ctx->device]);
+    ElimBufs b;
+    b.meta = base;
+    b.act = (uint32_t *)(base + L.act_off);
+    b.M = (uint64_t *)(base + L.m_off);
+    b.Q = (uint64_t *)(base + L.q_off);
+    ElimResult res;
+    const mela_status st = eliminate(ctx, A->data, A->ld, m, n, variant, false, false, &res,
+                                     nullptr, nullptr, &b);
+    if (st != MELA_OK) return st;
+    *rank = res.rank;
+    return MELA_OK;
+}
