@@ -32,8 +32,8 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per launch of the dominant product kernel, measured with two
 # rocprofv3 --pmc passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a
-# 1 GiB copy kernel) on this workload; profiles/matmul_mfma_fp4_32768.md.
-MEASURED_TRAFFIC = {("mfma_fp4", 32768): 2.590e10}
+# 1 GiB copy kernel) on this workload; profiles/k_mfma_direct_fp4_32768.md.
+MEASURED_TRAFFIC = {("mfma_fp4", 32768): 6.588e9}
 
 
 class HipEvents:
@@ -126,12 +126,15 @@ def roofline_for(algo, achieved, traffic, big_tiles=True):
         # dense MFMA: fp4 10.1 PFLOP/s, int8 5.0 POP/s; one AND+XOR pair of W
         # is one multiply-add (2 ops), so W/s compares directly with ops/s
         peak = 10100.0 if algo == "mfma_fp4" else 5000.0
-        kern = "k_mfma_ring" if big_tiles else "k_mfma_small"
+        if algo == "mfma_fp4":
+            kern = "k_mfma_direct (packed fp4 planes)" if big_tiles else "k_mfma_small<fp4>"
+        else:
+            kern = "k_mfma_ring<int8>" if big_tiles else "k_mfma_small<int8>"
         return {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak,
                 "unit": "Tbitop/s", "frac": round(achieved / peak, 4), "traffic": traffic,
-                "kernel": "%s<%s>" % (kern, "fp4" if algo == "mfma_fp4" else "int8"),
-                "note": "achieved = 2mnk / HIP-event duration of the MFMA kernel launch (operand "
-                        "expansion excluded; it is in ms_per_step)"}
+                "kernel": kern,
+                "note": "achieved = 2mnk / HIP-event duration of the MFMA kernel launch (the "
+                        "operand re-layout kernels are excluded; they are in ms_per_step)"}
     return {"bound": "valu", "achieved": round(achieved, 2), "peak": VALU_BITOP3_PEAK,
             "unit": "Tbitop/s", "frac": round(achieved / VALU_BITOP3_PEAK, 4), "traffic": traffic,
             "kernel": "k_andpop" if algo == "andpop" else "k_m4rm",

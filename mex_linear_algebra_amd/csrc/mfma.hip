@@ -16,6 +16,7 @@
 // Expanded operands are padded with zero rows to a multiple of 256 and with
 // zero bytes to a row pitch that is a whole number of k stages, so the inner
 // loops carry no bounds checks.
+#include <cstdlib>
 #include <type_traits>
 
 #include "common.h"
@@ -519,13 +520,240 @@ k_mfma_ring(const uint8_t *__restrict__ A, const uint8_t *__restrict__ B, size_t
 #endif
 }
 
+// ---------------------------------------------------------------------------
+// Packed operands, expanded in registers.
+//
+// The kernels above stream operands that were expanded to 4 bits per k in
+// HBM; here the matrix cores are fed from packed bits (1 bit per k), so HBM,
+// the L2 and the L1 carry 4x fewer bytes and no expanded copy exists.
+// The expansion is nearly free because of how e2m1 encodes small values:
+// the nibble bits 0, 1 and 2 alone are 0.5, 1.0 and 2.0.  A 32-bit word x
+// of packed bits therefore gives four fp4 operand dwords by masking:
+//   plane 0: x & 0x11111111  (bit 4t of x   -> nibble t, value 0.5)
+//   plane 1: x & 0x22222222  (bit 4t+1      -> nibble t, value 1.0)
+//   plane 2: x & 0x44444444  (bit 4t+2      -> nibble t, value 2.0)
+//   plane 3: (x >> 1) & 0x44444444 (bit 4t+3 -> nibble t, value 2.0)
+// Each MFMA takes one plane of both operands, so every product in it is
+// w * w for the plane's weight w, and the instruction's E8M0 block scales
+// (2^1, 2^0, 2^-1, 2^-1 on each operand) make every product exactly 1.0:
+// the fp32 accumulator counts the k with A(i,k) = B(k,j) = 1 exactly (< 2^24)
+// and C(i,j) = acc & 1.  A and B use the same map from (lane half, dword,
+// nibble, plane) to k, so which k lands where inside a fragment does not
+// matter.
+//
+// Stage-tiled packed layout (PK): rows padded to 256, k padded to a whole
+// number of loop trips (PK_DIRECT stages of 256 bits).  Block b of 256 rows,
+// stage s: 8 KiB at (b * nst + s) * 8 KiB, row r's 32 bytes (k bits 256 s ..
+// 256 s + 255) at r * 32; so each wave-wide fragment load reads 1 KiB of
+// consecutive bytes.
+constexpr int PK_STAGE = 32;                 // packed bytes of k per row per stage (256 k)
+constexpr int PK_SEG = XROWS * PK_STAGE;     // one stage of one 256-row block: 8 KiB
+#ifndef MELA_PK_DIRECT
+#define MELA_PK_DIRECT 4
+#endif
+constexpr int PK_DIRECT = MELA_PK_DIRECT;    // stages in flight in k_mfma_direct
+
+// A (rows x k bits, lds words per row) -> PK layout.  One thread per (row,
+// stage) 32-byte segment; four consecutive lanes take four consecutive
+// stages of one row, so a wave reads 16 runs of 128 contiguous source bytes
+// and writes 4 runs of 16 rows x 32 bytes.  grid.x: groups of 4 stages (the
+// last one partial when nst is not a multiple of 4),
+// grid.y: 64-row groups.  Rows >= rows and words >= kw are written as zeros.
+__global__ void __launch_bounds__(256)
+k_tile_packed(const uint64_t *__restrict__ src, size_t lds, size_t rows, size_t kw, size_t nst,
+              uint8_t *__restrict__ dst) {
+    const int l = threadIdx.x;
+    const size_t s = (size_t)blockIdx.x * 4 + (l & 3);
+    const size_t i = (size_t)blockIdx.y * 64 + (l >> 2);
+    if (s >= nst) return;
+    uint64_t v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const size_t w = 4 * s + q;
+        v[q] = (i < rows && w < kw) ? src[i * lds + w] : 0ull;
+    }
+    uint4 *p = reinterpret_cast<uint4 *>(dst + ((i / XROWS) * nst + s) * PK_SEG + (i % XROWS) * PK_STAGE);
+    p[0] = make_uint4((uint32_t)v[0], (uint32_t)(v[0] >> 32), (uint32_t)v[1], (uint32_t)(v[1] >> 32));
+    p[1] = make_uint4((uint32_t)v[2], (uint32_t)(v[2] >> 32), (uint32_t)v[3], (uint32_t)(v[3] >> 32));
+}
+
+// B (k x n bits) -> B^T in PK layout.  Each wave transposes the four 64 x 64
+// bit blocks of one stage (256 k rows) for 64 columns; lane c then holds
+// column 64 jb + c's 256 k bits and writes them as its row's 32 bytes (a
+// wave writes 2 KiB of consecutive output).  grid.x: groups of 4 stages
+// (one per wave), grid.y: 64-column blocks up to the padded width.  Columns
+// >= n and k rows >= k come out as zeros.
+__global__ void __launch_bounds__(256)
+k_transpose_tile_packed(const uint64_t *__restrict__ B, size_t ldb, size_t k, size_t n, size_t nst,
+                        uint8_t *__restrict__ dst) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const size_t s = (size_t)blockIdx.x * 4 + wave;
+    const size_t jb = blockIdx.y;
+    if (s >= nst) return;
+    const size_t nw = (n + 63) >> 6;
+    uint64_t o[4];
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        const size_t i = (4 * s + h) * 64 + lane;
+        uint64_t v = 0;
+        if (i < k && jb < nw) {
+            v = B[i * ldb + jb];
+            if (jb == nw - 1 && (n & 63)) v &= (1ull << (n & 63)) - 1ull;
+        }
+        o[h] = dev::transpose64(v, lane);
+    }
+    const size_t j = jb * 64 + lane;
+    uint4 *p = reinterpret_cast<uint4 *>(dst + ((j / XROWS) * nst + s) * PK_SEG + (j % XROWS) * PK_STAGE);
+    p[0] = make_uint4((uint32_t)o[0], (uint32_t)(o[0] >> 32), (uint32_t)o[1], (uint32_t)(o[1] >> 32));
+    p[1] = make_uint4((uint32_t)o[2], (uint32_t)(o[2] >> 32), (uint32_t)o[3], (uint32_t)(o[3] >> 32));
+}
+
+// Plane j of four packed dwords as an fp4 operand (the upper four dwords of
+// the 8-dword operand are unused by fp4).
+template <int J>
+__device__ __forceinline__ v8i pk_plane(const uint4 &x) {
+    constexpr uint32_t M = J == 3 ? 0x44444444u : (0x11111111u << J);
+    constexpr int SH = J == 3 ? 1 : 0;
+    return v8i{(int)((x.x >> SH) & M), (int)((x.y >> SH) & M), (int)((x.z >> SH) & M),
+               (int)((x.w >> SH) & M), 0, 0, 0, 0};
+}
+
+// E8M0 scale of plane j on each operand: w_j * w_j * 2^(2 e_j) = 1.
+template <int J>
+constexpr int pk_scale() { return J == 0 ? 128 : J == 1 ? 127 : 126; }
+
+// Plane J of one stage for the wave's 4 x 4 MFMA tiles: expand the 8
+// fragments (5 VALU per 4 dwords over the four planes), 16 MFMAs.
+template <int J>
+__device__ __forceinline__ void pk_mm(const uint4 (&ra)[4], const uint4 (&rb)[4], v16f (&acc)[4][4]) {
+    v8i fa[4], fb[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        fa[t] = pk_plane<J>(ra[t]);
+        fb[t] = pk_plane<J>(rb[t]);
+    }
+#pragma unroll
+    for (int tr = 0; tr < 4; ++tr)
+#pragma unroll
+        for (int tc = 0; tc < 4; ++tc)
+            acc[tr][tc] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+                fa[tr], fb[tc], acc[tr][tc], 4, 4, 0, pk_scale<J>(), 0, pk_scale<J>());
+}
+
+// k_mfma_direct: 256 x 256 tiles, 4 waves in a 2 x 2 grid, each 128 x 128
+// (4 x 4 MFMA tiles, 256 accumulators), one workgroup per CU, the fragments
+// loaded straight into registers.  In the PK layout lane l's
+// fragment of row tile t (row r = 128 w + 32 t + l % 32, k half l / 32) is 16
+// consecutive bytes at r * 32 + (l / 32) * 16 of the stage segment, so one
+// global_load_dwordx4 per fragment moves 1 KiB of consecutive bytes per wave;
+// the two waves that share a row block meet in the L1.  No LDS, no barrier:
+// every wave waits only for its own loads, D stages ahead.  (An LDS ring
+// filled by global_load_lds, one barrier per stage, ran 9.9 ms at 32768^3
+// against 9.0 ms for this form.)
+template <int D>
+__global__ void __launch_bounds__(256, 1)
+k_mfma_direct(const uint8_t *__restrict__ A, const uint8_t *__restrict__ B, int nst,
+              uint64_t *__restrict__ C, size_t ldc, int m, int n, int tx, int ty) {
+    constexpr int TR = 4, WT = 128;
+    constexpr int SEG16 = PK_SEG / 16;  // uint4 per 256-k segment of one 256-row block
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    int bx, by;
+    tile_of<4>(blockIdx.x, tx, ty, bx, by);
+    const int i0 = bx * XROWS, j0 = by * XROWS;
+    const int half = lane >> 5, l32 = lane & 31;
+    const uint4 *pa = reinterpret_cast<const uint4 *>(A + (size_t)bx * nst * PK_SEG) +
+                      (wr * WT + l32) * 2 + half;
+    const uint4 *pb = reinterpret_cast<const uint4 *>(B + (size_t)by * nst * PK_SEG) +
+                      (wc * WT + l32) * 2 + half;
+    v16f acc[TR][TR];
+#pragma unroll
+    for (int a = 0; a < TR; ++a)
+#pragma unroll
+        for (int b = 0; b < TR; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
+    uint4 fa[D][TR], fb[D][TR];
+    // stage st's fragments (past the last stage: the last one again)
+    auto load = [&](int st, uint4 (&ra)[TR], uint4 (&rb)[TR]) __attribute__((always_inline)) {
+        const size_t so = (size_t)(st < nst ? st : nst - 1) * SEG16;
+#pragma unroll
+        for (int t = 0; t < TR; ++t) {
+            ra[t] = pa[so + t * 64];
+            rb[t] = pb[so + t * 64];
+        }
+    };
+#pragma unroll
+    for (int j = 0; j < D; ++j) load(j, fa[j], fb[j]);
+    __builtin_amdgcn_sched_barrier(0);
+    // the host pads k to whole trips of D stages; the scheduling barriers keep
+    // each refill where it is issued (the scheduler would otherwise sink the
+    // loads to their use D steps later and wait for them there)
+    for (int st = 0; st < nst; st += D) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            pk_mm<0>(fa[j], fb[j], acc);
+            pk_mm<1>(fa[j], fb[j], acc);
+            pk_mm<2>(fa[j], fb[j], acc);
+            pk_mm<3>(fa[j], fb[j], acc);
+            __builtin_amdgcn_sched_barrier(0);
+            load(st + j + D, fa[j], fb[j]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    store_tile<TR>(acc, i0 + wr * WT, j0 + wc * WT, C, ldc, m, n);
+}
+
 }  // namespace
+
+// C = A * B on the matrix cores from packed operands (fp4, >= 256 tiles of
+// 256 x 256): A and B^T are re-laid in the stage-tiled packed layout (two
+// passes over 2 x 128 MiB at 32768^3), then k_mfma_direct.
+mela_status matmul_mfma_packed(mela_ctx *ctx, const uint64_t *A, size_t lda, const uint64_t *B,
+                               size_t ldb, uint64_t *C, size_t ldc, size_t m, size_t k, size_t n) {
+    hipStream_t s = ctx->stream;
+    const size_t kw = words64(k);
+    const size_t q = PK_DIRECT;  // the kernel's loop runs whole trips of PK_DIRECT stages
+    const size_t nst = ((k + 255) / 256 + q - 1) / q * q;
+    const size_t mp = (m + XROWS - 1) / XROWS * XROWS, np = (n + XROWS - 1) / XROWS * XROWS;
+    if (nst > (size_t)INT32_MAX || mp / 64 > 65535 || np / 64 > 65535)
+        return set_error(MELA_ERR_INVALID, "matrix too large for the matrix-core product");
+    void *pa = nullptr, *pb = nullptr;
+    mela_status st;
+    if ((st = scratch_get(ctx, SLOT_XA, mp * nst * PK_STAGE, &pa)) != MELA_OK) return st;
+    if ((st = scratch_get(ctx, SLOT_XB, np * nst * PK_STAGE, &pb)) != MELA_OK) return st;
+    hipLaunchKernelGGL(k_tile_packed, dim3((unsigned)((nst + 3) / 4), (unsigned)(mp / 64)), dim3(256), 0, s,
+                       A, lda, m, kw, nst, (uint8_t *)pa);
+    hipLaunchKernelGGL(k_transpose_tile_packed, dim3((unsigned)((nst + 3) / 4), (unsigned)(np / 64)), dim3(256), 0,
+                       s, B, ldb, k, n, nst, (uint8_t *)pb);
+    MELA_HIP(hipGetLastError());
+    const size_t tx = mp / XROWS, ty = np / XROWS;
+    if (tx * ty > (size_t)INT32_MAX) return set_error(MELA_ERR_INVALID, "too many tiles for one launch");
+    prof_mark(ctx, true);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mfma_direct<PK_DIRECT>), dim3((unsigned)(tx * ty)), dim3(256), 0, s,
+                       (const uint8_t *)pa, (const uint8_t *)pb, (int)nst, C, ldc, (int)m, (int)n, (int)tx,
+                       (int)ty);
+    prof_mark(ctx, false);
+    MELA_HIP(hipGetLastError());
+    return MELA_OK;
+}
 
 // C = A * B via the matrix cores; fp4 or int8 expansion.
 mela_status matmul_mfma(mela_ctx *ctx, const uint64_t *A, size_t lda, const uint64_t *B,
                         size_t ldb, uint64_t *C, size_t ldc, size_t m, size_t k, size_t n,
                         bool fp4) {
     hipStream_t s = ctx->stream;
+    {
+        // fp4 grids of >= 256 big tiles: packed operands expanded in
+        // registers (MELA_MFMA_EXPANDED=1 keeps the expanded-operand kernel)
+        const size_t t256 = ((m + XROWS - 1) / XROWS) * ((n + XROWS - 1) / XROWS);
+        static const bool expanded = [] {
+            const char *e = std::getenv("MELA_MFMA_EXPANDED");
+            return e && *e && *e != '0';
+        }();
+        if (fp4 && !expanded && t256 >= (size_t)ctx->cu_count)
+            return matmul_mfma_packed(ctx, A, lda, B, ldb, C, ldc, m, k, n);
+    }
     const size_t kw = words64(k);
     const size_t xbytes = kw * (fp4 ? 32 : 64);  // expanded bytes per row
     const size_t pitch = (xbytes + KPAD - 1) / KPAD * KPAD;
