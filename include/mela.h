@@ -135,12 +135,17 @@ typedef struct mela_bitmat {
     size_t rows, cols, ld;
 } mela_bitmat;
 
+/* Product algorithms.  AUTO: fp4 MFMA from 2^33 bit-products on, AND+parity
+ * below; for products that would take over ~40 us it counts the set bits of
+ * A on the device and takes ROWXOR when A is sparse enough (one B-row read
+ * per set bit costs less than the dense kernel). */
 enum {
     MELA_ALGO_AUTO = 0,
     MELA_ALGO_ANDPOP = 1, /* AND + parity over a transposed B (v_bitop3) */
     MELA_ALGO_M4RM = 2,   /* Method of Four Russians, Gray-code tables in LDS */
     MELA_ALGO_MFMA_I8 = 3, /* exact int8 MFMA, i32 accumulate, & 1 */
-    MELA_ALGO_MFMA_FP4 = 4 /* exact fp4 (e2m1) MFMA, f32 accumulate, & 1 */
+    MELA_ALGO_MFMA_FP4 = 4, /* exact fp4 (e2m1) MFMA, f32 accumulate, & 1 */
+    MELA_ALGO_ROWXOR = 5  /* C_i ^= B_k for each set bit A(i,k) (matmul_gf2.c:127-186) */
 };
 
 mela_status mela_dev_alloc(mela_ctx *ctx, size_t bytes, void **dptr);

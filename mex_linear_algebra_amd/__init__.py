@@ -23,7 +23,8 @@ __all__ = [
     "MelaError", "Context", "BitMatrix", "lib_path", "load", "default_context",
     "mela_matmul_gf2", "mela_matmul_m4ri", "mela_rank_gf2", "mela_rank_m4ri",
     "mela_null_gf2", "mela_null_m4ri", "mela_rref_gf2",
-    "ALGO_AUTO", "ALGO_ANDPOP", "ALGO_M4RM", "ALGO_MFMA_I8", "ALGO_MFMA_FP4", "ELIM_GE", "ELIM_M4RI",
+    "ALGO_AUTO", "ALGO_ANDPOP", "ALGO_M4RM", "ALGO_MFMA_I8", "ALGO_MFMA_FP4", "ALGO_ROWXOR",
+    "ELIM_GE", "ELIM_M4RI",
     "ELIM_AUTO",
     "Batch", "BATCH_NULL", "BATCH_GE", "BATCH_GRAPH",
     "Server", "Request", "Result", "OP_RANK", "OP_NULL", "OP_MATMUL",
@@ -32,7 +33,7 @@ __all__ = [
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
 MELA_LOGICAL, MELA_DOUBLE, MELA_OTHER, MELA_SPARSE = 0, 1, 2, 0x100
-ALGO_AUTO, ALGO_ANDPOP, ALGO_M4RM, ALGO_MFMA_I8, ALGO_MFMA_FP4 = 0, 1, 2, 3, 4
+ALGO_AUTO, ALGO_ANDPOP, ALGO_M4RM, ALGO_MFMA_I8, ALGO_MFMA_FP4, ALGO_ROWXOR = 0, 1, 2, 3, 4, 5
 ELIM_GE, ELIM_M4RI, ELIM_AUTO = 0, 1, 2
 
 STATUS = {0: "OK", 1: "SPARSE", 2: "DIM", 3: "TYPE", 4: "NOMEM", 5: "INVALID",

@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <utility>
@@ -120,8 +121,28 @@ enum ScratchSlot {
     SLOT_A = 0, SLOT_B, SLOT_C, SLOT_BT, SLOT_IN_A, SLOT_IN_B, SLOT_OUT,
     SLOT_ELIM_M, SLOT_ELIM_Q, SLOT_ELIM_META, SLOT_ELIM_ACT, SLOT_ELIM_TAB,
     SLOT_NULL_Z, SLOT_XA, SLOT_XB,
-    SLOT_BATCH_DESC, SLOT_BATCH_WORK, SLOT_BATCH_Z, SLOT_BATCH_RANK, SLOT_NULL_TAB, SLOT_COUNT
+    SLOT_BATCH_DESC, SLOT_BATCH_WORK, SLOT_BATCH_Z, SLOT_BATCH_RANK, SLOT_NULL_TAB, SLOT_DENSITY,
+    SLOT_COUNT
 };
+
+// Device-side choice between the row-XOR product and a dense one (AUTO on
+// large products): a popcount kernel sums the set bits of A into *ones; the
+// row-XOR kernel runs only if *ones < thr, the dense path's kernels only if
+// not.  ones == nullptr: no gate, the kernel always runs.
+// The count is spread over GATE_SLOTS words (workgroup b adds to slot
+// b % GATE_SLOTS: one address for all of them serialised the atomics).
+constexpr int GATE_SLOTS = 16;
+struct Gate {
+    const unsigned long long *ones = nullptr;  // GATE_SLOTS partial counts
+    unsigned long long thr = 0;
+};
+__device__ __forceinline__ bool gate_skip(const Gate &g, bool sparse_path) {
+    if (!g.ones) return false;
+    unsigned long long t = 0;
+#pragma unroll
+    for (int i = 0; i < GATE_SLOTS; ++i) t += g.ones[i];
+    return (t < g.thr) != sparse_path;
+}
 
 // ------------------------------------------------- kernel launchers ----
 // pack.hip
@@ -145,9 +166,17 @@ mela_status matmul_device(mela_ctx *ctx, const uint64_t *A, size_t lda,
                           size_t m, size_t k, size_t n, int algo);
 
 // mfma.hip
+// gate / count / sparse_path: the AUTO product's density choice.  With
+// count set, the call adds the set bits of A to *count (zeroed by the
+// caller) on its way and calls sparse_path(gate) once the count is complete,
+// to enqueue the row-XOR product; the dense kernels run behind the gate.
 mela_status matmul_mfma(mela_ctx *ctx, const uint64_t *A, size_t lda, const uint64_t *B,
                         size_t ldb, uint64_t *C, size_t ldc, size_t m, size_t k, size_t n,
-                        bool fp4);
+                        bool fp4, Gate gate = Gate(), unsigned long long *count = nullptr,
+                        const std::function<mela_status(Gate)> &sparse_path = nullptr);
+// the GATE_SLOTS counts at ones += set bits of the rows x kw words of A (enqueued)
+mela_status launch_popcount(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t rows, size_t kw,
+                            unsigned long long *ones);
 
 // elim.hip
 struct ElimResult {

@@ -11,6 +11,8 @@
 //    workgroup builds the 256 XOR-combinations of those rows (for its 2048
 //    output columns) in LDS; each output row then XORs one table row selected
 //    by the 8 matching bits of A.  Bound: LDS read bandwidth / VALU issue.
+#include <algorithm>
+
 #include "common.h"
 
 namespace mela {
@@ -115,6 +117,82 @@ k_andpop(const uint64_t *__restrict__ A, size_t lda, const uint64_t *__restrict_
             if (lane == r * AP_NR + c && i < m && w < nw) C[(size_t)i * ldc + w] = word;
         }
     }
+}
+
+// --------------------------------------------------------------- row XOR
+// C_i = XOR of the rows B_k over the set bits k of A_i: the reference's own
+// GE product (matmul_gf2.c:127-186 XORs B_k into the row-block accumulator
+// for every odd A(i,k)), whose cost is nnz(A) row reads instead of m*n*k bit
+// operations, so it is the product of choice for sparse A.  One wave per
+// (row i, 64-word chunk of the row of C): lanes are words, the bits of A_i
+// are walked in scalar registers, RX_BATCH row loads of B in flight together
+// (an empty slot re-reads row 64 kq, a valid row, and masks it).
+constexpr int RX_WAVES = 4;
+constexpr int RX_BATCH = 4;
+
+// Row i of C, word w (lane-varying): XOR of B's word w over the set bits of
+// A_i (i is wave-uniform).
+__device__ __forceinline__ void rowxor_unit(const uint64_t *__restrict__ A, size_t lda,
+                                            const uint64_t *__restrict__ B, size_t ldb,
+                                            uint64_t *__restrict__ C, size_t ldc, int i, int w, int kw,
+                                            int nw) {
+    const bool wok = w < nw;
+    const uint64_t *arow = A + (size_t)i * lda;
+    const uint64_t *bcol = B + (wok ? w : 0);
+    uint64_t c = 0;
+    for (int kq = 0; kq < kw; ++kq) {
+        uint64_t x = arow[kq];  // the same word on every lane
+        while (x) {
+            uint64_t v[RX_BATCH], keep[RX_BATCH];
+#pragma unroll
+            for (int q = 0; q < RX_BATCH; ++q) {
+                const bool has = x != 0ull;
+                const int bit = has ? __builtin_ctzll(x) : 0;
+                x &= x - 1ull;
+                keep[q] = has ? ~0ull : 0ull;
+                v[q] = bcol[(size_t)(kq * 64 + bit) * ldb];
+            }
+#pragma unroll
+            for (int q = 0; q < RX_BATCH; ++q) c ^= v[q] & keep[q];
+        }
+    }
+    if (wok) C[(size_t)i * ldc + w] = c;
+}
+
+// The grid is at most a few workgroups per CU, each striding over the (row,
+// chunk) pairs: when the density gate turns the kernel off, thousands of
+// empty workgroups would cost more dispatch time than the product.
+__global__ void __launch_bounds__(RX_WAVES * 64)
+k_rowxor(const uint64_t *__restrict__ A, size_t lda, const uint64_t *__restrict__ B, size_t ldb,
+         uint64_t *__restrict__ C, size_t ldc, int m, int kw, int nw, Gate gate) {
+    if (gate_skip(gate, true)) return;
+    const int lane = threadIdx.x & 63;
+    const int nch = (nw + 63) / 64;
+    const long units = (long)m * nch;  // (row, 64-word chunk) pairs
+    for (long u = __builtin_amdgcn_readfirstlane((int)blockIdx.x * RX_WAVES + (threadIdx.x >> 6)); u < units;
+         u += (long)gridDim.x * RX_WAVES)
+        rowxor_unit(A, lda, B, ldb, C, ldc, (int)(u / nch), (int)(u % nch) * 64 + lane, kw, nw);
+}
+
+// ones[b % GATE_SLOTS] += popcount of workgroup b's share of the rows x kw
+// words of A (the density of the AUTO choice)
+__global__ void __launch_bounds__(256)
+k_popcount_rows(const uint64_t *__restrict__ A, size_t lda, size_t rows, size_t kw,
+                unsigned long long *__restrict__ ones) {
+    unsigned long long acc = 0;
+    const size_t tot = rows * kw;
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < tot; e += (size_t)gridDim.x * 256) {
+        const size_t r = e / kw, q = e - r * kw;
+        acc += (unsigned long long)__popcll(A[r * lda + q]);
+    }
+    // wave sums, then one atomic per workgroup (the grid is a few hundred
+    // workgroups: one atomic per wave on a single address cost ~80 us)
+    __shared__ unsigned long long part[4];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(ones + blockIdx.x % GATE_SLOTS, part[0] + part[1] + part[2] + part[3]);
 }
 
 // ------------------------------------------------------------------ M4RM
@@ -241,6 +319,50 @@ k_m4rm(const uint64_t *__restrict__ A, size_t lda, const uint64_t *__restrict__ 
 
 }  // namespace
 
+mela_status launch_popcount(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t rows, size_t kw,
+                            unsigned long long *ones) {
+    const size_t blocks = std::min<size_t>((rows * kw + 255) / 256, (size_t)ctx->cu_count * 2);
+    hipLaunchKernelGGL(k_popcount_rows, dim3((unsigned)std::max<size_t>(blocks, 1)), dim3(256), 0,
+                       ctx->stream, A, lda, rows, kw, ones);
+    MELA_HIP(hipGetLastError());
+    return MELA_OK;
+}
+
+namespace {
+
+// The AUTO product.  Dense kernels cost m*n*k bit operations whatever the
+// data; the row-XOR product costs one B-row read per set bit of A.  Below
+// 2^33 bit-products AUTO is AND+parity.  From there on it is the fp4 matrix
+// cores, unless A is sparse: the bits of A are counted on the device (by the
+// kernel that re-lays A for the matrix cores, or a separate pass) and both
+// the row-XOR kernel and the remaining fp4 kernels are enqueued behind a
+// gate (each exits at once unless the count picks it), so the choice costs
+// no host round trip.  The threshold compares the
+// measured rates on MI355X: fp4 ~7.5e15 bit-op/s plus ~15 us per call, row
+// XOR ~1.6e13 bytes/s of B rows plus ~5 us.
+Gate auto_gate(size_t m, size_t k, size_t n, unsigned long long *ones) {
+    const double dense_s = 2.0 * (double)m * (double)n * (double)k / 7.5e15 + 15e-6;
+    const double row_bytes = (double)std::max<size_t>(words64(n), 64) * 8.0;
+    const double budget = dense_s - 5e-6 - (double)m * (double)words64(k) * 1e-11;
+    Gate g;
+    g.ones = ones;
+    g.thr = budget > 0 ? (unsigned long long)(budget * 1.6e13 / row_bytes) : 0ull;
+    return g;
+}
+
+mela_status launch_rowxor(hipStream_t s, int cu_count, const uint64_t *A, size_t lda, const uint64_t *B,
+                          size_t ldb, uint64_t *C, size_t ldc, size_t m, size_t k, size_t n, Gate gate) {
+    const int kw = (int)words64(k), nwc = (int)words64(n);
+    const size_t units = m * (size_t)((nwc + 63) / 64);
+    const size_t grid = std::min<size_t>((units + RX_WAVES - 1) / RX_WAVES, (size_t)cu_count * 8);
+    hipLaunchKernelGGL(k_rowxor, dim3((unsigned)std::max<size_t>(grid, 1)), dim3(RX_WAVES * 64), 0, s, A, lda,
+                       B, ldb, C, ldc, (int)m, kw, nwc, gate);
+    MELA_HIP(hipGetLastError());
+    return MELA_OK;
+}
+
+}  // namespace
+
 mela_status matmul_device(mela_ctx *ctx, const uint64_t *A, size_t lda, const uint64_t *B,
                           size_t ldb, uint64_t *C, size_t ldc, size_t m, size_t k, size_t n,
                           int algo) {
@@ -250,14 +372,30 @@ mela_status matmul_device(mela_ctx *ctx, const uint64_t *A, size_t lda, const ui
     if (k == 0) return launch_fill_zero(s, C, m * ldc * sizeof(uint64_t));  // C = 0
     if (m > (size_t)INT32_MAX || n > (size_t)INT32_MAX || k > (size_t)INT32_MAX)
         return set_error(MELA_ERR_INVALID, "matrix dimension exceeds 2^31-1");
-    // AUTO: the fp4 matrix-core path once the product is large enough to
-    // amortise expanding the operands (m*n*k >= 2^33 bit-products, exact
-    // while k < 2^24); the AND+parity kernel below that.
-    if (algo == MELA_ALGO_AUTO)
-        algo = ((double)m * (double)n * (double)k >= 8589934592.0 && k < (1u << 24))
-                   ? MELA_ALGO_MFMA_FP4
-                   : MELA_ALGO_ANDPOP;
-
+    if (algo == MELA_ALGO_AUTO) {
+        const bool fp4 = (double)m * (double)n * (double)k >= 8589934592.0 && k < (1u << 24);
+        if (!fp4) {
+            algo = MELA_ALGO_ANDPOP;
+        } else {
+            void *p = nullptr;
+            const size_t cb = GATE_SLOTS * sizeof(unsigned long long);
+            mela_status st = scratch_get(ctx, SLOT_DENSITY, cb, &p);
+            if (st != MELA_OK) return st;
+            if ((st = launch_fill_zero(s, p, cb)) != MELA_OK) return st;
+            unsigned long long *count = (unsigned long long *)p;
+            const Gate gate = auto_gate(m, k, n, count);
+            auto sparse = [&](Gate g) {
+                return launch_rowxor(s, ctx->cu_count, A, lda, B, ldb, C, ldc, m, k, n, g);
+            };
+            return matmul_mfma(ctx, A, lda, B, ldb, C, ldc, m, k, n, true, gate, count, sparse);
+        }
+    }
+    if (algo == MELA_ALGO_ROWXOR) {
+        prof_mark(ctx, true);
+        mela_status st = launch_rowxor(s, ctx->cu_count, A, lda, B, ldb, C, ldc, m, k, n, Gate());
+        prof_mark(ctx, false);
+        return st;
+    }
     if (algo == MELA_ALGO_ANDPOP) {
         // Bt = B^T (n x ceil(k/64) words)
         const size_t ldbt = words64(k);

@@ -17,6 +17,7 @@
 // zero bytes to a row pitch that is a whole number of k stages, so the inner
 // loops carry no bounds checks.
 #include <cstdlib>
+#include <functional>
 #include <type_traits>
 
 #include "common.h"
@@ -58,7 +59,8 @@ __device__ __forceinline__ size_t xoff(size_t i, size_t b, size_t pitch, bool ti
 template <bool FP4>
 __global__ void __launch_bounds__(256)
 k_expand(const uint64_t *__restrict__ src, size_t lds, size_t rows, size_t kw, size_t rows_pad,
-         size_t pitch, uint8_t *__restrict__ dst) {
+         size_t pitch, uint8_t *__restrict__ dst, Gate gate) {
+    if (gate_skip(gate, false)) return;
     const size_t chunks = pitch / 32;
     const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
     const size_t i = idx / chunks, c = idx % chunks;
@@ -123,7 +125,8 @@ __device__ __forceinline__ void expand_word(uint64_t x, uint32_t *o) {
 template <bool FP4>
 __global__ void __launch_bounds__(256)
 k_expand_tiled(const uint64_t *__restrict__ src, size_t lds, size_t rows, size_t kw, size_t pitch,
-               uint8_t *__restrict__ dst) {
+               uint8_t *__restrict__ dst, Gate gate) {
+    if (gate_skip(gate, false)) return;
     static_assert(XROWS % 64 == 0 && RING_STAGE == 64, "64-row groups inside a row block");
     const int l = threadIdx.x;
     const size_t seg = (size_t)blockIdx.x * 4 + (l & 3);
@@ -151,7 +154,8 @@ k_expand_tiled(const uint64_t *__restrict__ src, size_t lds, size_t rows, size_t
 template <bool FP4, int KB>
 __global__ void __launch_bounds__(256)
 k_transpose_expand(const uint64_t *__restrict__ B, size_t ldb, size_t k, size_t n, size_t kwp,
-                   uint8_t *__restrict__ dst, size_t pitch, bool tiled) {
+                   uint8_t *__restrict__ dst, size_t pitch, bool tiled, Gate gate) {
+    if (gate_skip(gate, false)) return;
     static_assert(KB == 1 || (FP4 && KB == 2), "a lane writes at most 64 bytes");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const size_t kb0 = ((size_t)blockIdx.x * 4 + wave) * KB;  // first 64-row block of B (k)
@@ -256,7 +260,8 @@ __device__ __forceinline__ void store_tile(Acc (&acc)[TR][TR], int row0, int col
 template <bool FP4>
 __global__ void __launch_bounds__(256, 2)
 k_mfma_small(const uint8_t *__restrict__ A, const uint8_t *__restrict__ B, size_t pitch,
-             uint64_t *__restrict__ C, size_t ldc, int m, int n, int tx, int ty) {
+             uint64_t *__restrict__ C, size_t ldc, int m, int n, int tx, int ty, Gate gate) {
+    if (gate_skip(gate, false)) return;
     constexpr int TR = 2, WT = 64, ROWS = 128, U = 8;
     __shared__ uint4 sA[2][U * ROWS];
     __shared__ uint4 sB[2][U * ROWS];
@@ -388,7 +393,8 @@ __device__ __forceinline__ int ring_swz(int r) { return (r >> 2) & 3; }
 template <bool FP4>
 __global__ void __launch_bounds__(256, 1)
 k_mfma_ring(const uint8_t *__restrict__ A, const uint8_t *__restrict__ B, size_t pitch,
-            uint64_t *__restrict__ C, size_t ldc, int m, int n, int tx, int ty) {
+            uint64_t *__restrict__ C, size_t ldc, int m, int n, int tx, int ty, Gate gate) {
+    if (gate_skip(gate, false)) return;
     // The host pass only needs the launch stub; the body uses LDS-DMA
     // builtins that exist for the device target alone.
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -559,22 +565,38 @@ constexpr int PK_DIRECT = MELA_PK_DIRECT;    // stages in flight in k_mfma_direc
 // and writes 4 runs of 16 rows x 32 bytes.  grid.x: groups of 4 stages (the
 // last one partial when nst is not a multiple of 4),
 // grid.y: 64-row groups.  Rows >= rows and words >= kw are written as zeros.
+// count != nullptr: also adds the number of set bits of A to the GATE_SLOTS
+// counts at count (the density gate of the AUTO product; one atomic per
+// workgroup).
 __global__ void __launch_bounds__(256)
 k_tile_packed(const uint64_t *__restrict__ src, size_t lds, size_t rows, size_t kw, size_t nst,
-              uint8_t *__restrict__ dst) {
+              uint8_t *__restrict__ dst, unsigned long long *__restrict__ count) {
     const int l = threadIdx.x;
     const size_t s = (size_t)blockIdx.x * 4 + (l & 3);
     const size_t i = (size_t)blockIdx.y * 64 + (l >> 2);
-    if (s >= nst) return;
-    uint64_t v[4];
+    uint64_t v[4] = {0ull, 0ull, 0ull, 0ull};
+    if (s < nst) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const size_t w = 4 * s + q;
-        v[q] = (i < rows && w < kw) ? src[i * lds + w] : 0ull;
+        for (int q = 0; q < 4; ++q) {
+            const size_t w = 4 * s + q;
+            v[q] = (i < rows && w < kw) ? src[i * lds + w] : 0ull;
+        }
+        uint4 *p = reinterpret_cast<uint4 *>(dst + ((i / XROWS) * nst + s) * PK_SEG + (i % XROWS) * PK_STAGE);
+        p[0] = make_uint4((uint32_t)v[0], (uint32_t)(v[0] >> 32), (uint32_t)v[1], (uint32_t)(v[1] >> 32));
+        p[1] = make_uint4((uint32_t)v[2], (uint32_t)(v[2] >> 32), (uint32_t)v[3], (uint32_t)(v[3] >> 32));
     }
-    uint4 *p = reinterpret_cast<uint4 *>(dst + ((i / XROWS) * nst + s) * PK_SEG + (i % XROWS) * PK_STAGE);
-    p[0] = make_uint4((uint32_t)v[0], (uint32_t)(v[0] >> 32), (uint32_t)v[1], (uint32_t)(v[1] >> 32));
-    p[1] = make_uint4((uint32_t)v[2], (uint32_t)(v[2] >> 32), (uint32_t)v[3], (uint32_t)(v[3] >> 32));
+    if (count) {
+        __shared__ unsigned long long part[4];
+        unsigned long long c = (unsigned long long)(__popcll(v[0]) + __popcll(v[1]) + __popcll(v[2]) +
+                                                    __popcll(v[3]));
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+        if ((l & 63) == 0) part[l >> 6] = c;
+        __syncthreads();
+        if (l == 0)
+            atomicAdd(count + (blockIdx.y * gridDim.x + blockIdx.x) % GATE_SLOTS,
+                      part[0] + part[1] + part[2] + part[3]);
+    }
 }
 
 // B (k x n bits) -> B^T in PK layout.  Each wave transposes the four 64 x 64
@@ -585,7 +607,8 @@ k_tile_packed(const uint64_t *__restrict__ src, size_t lds, size_t rows, size_t 
 // >= n and k rows >= k come out as zeros.
 __global__ void __launch_bounds__(256)
 k_transpose_tile_packed(const uint64_t *__restrict__ B, size_t ldb, size_t k, size_t n, size_t nst,
-                        uint8_t *__restrict__ dst) {
+                        uint8_t *__restrict__ dst, Gate gate) {
+    if (gate_skip(gate, false)) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const size_t s = (size_t)blockIdx.x * 4 + wave;
     const size_t jb = blockIdx.y;
@@ -653,7 +676,8 @@ __device__ __forceinline__ void pk_mm(const uint4 (&ra)[4], const uint4 (&rb)[4]
 template <int D>
 __global__ void __launch_bounds__(256, 1)
 k_mfma_direct(const uint8_t *__restrict__ A, const uint8_t *__restrict__ B, int nst,
-              uint64_t *__restrict__ C, size_t ldc, int m, int n, int tx, int ty) {
+              uint64_t *__restrict__ C, size_t ldc, int m, int n, int tx, int ty, Gate gate) {
+    if (gate_skip(gate, false)) return;
     constexpr int TR = 4, WT = 128;
     constexpr int SEG16 = PK_SEG / 16;  // uint4 per 256-k segment of one 256-row block
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -710,7 +734,9 @@ k_mfma_direct(const uint8_t *__restrict__ A, const uint8_t *__restrict__ B, int 
 // 256 x 256): A and B^T are re-laid in the stage-tiled packed layout (two
 // passes over 2 x 128 MiB at 32768^3), then k_mfma_direct.
 mela_status matmul_mfma_packed(mela_ctx *ctx, const uint64_t *A, size_t lda, const uint64_t *B,
-                               size_t ldb, uint64_t *C, size_t ldc, size_t m, size_t k, size_t n) {
+                               size_t ldb, uint64_t *C, size_t ldc, size_t m, size_t k, size_t n,
+                               Gate gate, unsigned long long *count,
+                               const std::function<mela_status(Gate)> &sparse_path) {
     hipStream_t s = ctx->stream;
     const size_t kw = words64(k);
     const size_t q = PK_DIRECT;  // the kernel's loop runs whole trips of PK_DIRECT stages
@@ -723,16 +749,21 @@ mela_status matmul_mfma_packed(mela_ctx *ctx, const uint64_t *A, size_t lda, con
     if ((st = scratch_get(ctx, SLOT_XA, mp * nst * PK_STAGE, &pa)) != MELA_OK) return st;
     if ((st = scratch_get(ctx, SLOT_XB, np * nst * PK_STAGE, &pb)) != MELA_OK) return st;
     hipLaunchKernelGGL(k_tile_packed, dim3((unsigned)((nst + 3) / 4), (unsigned)(mp / 64)), dim3(256), 0, s,
-                       A, lda, m, kw, nst, (uint8_t *)pa);
+                       A, lda, m, kw, nst, (uint8_t *)pa, count);
+    if (count && sparse_path) {
+        // the gate is complete once A is tiled: the row-XOR product goes here
+        mela_status rs = sparse_path(gate);
+        if (rs != MELA_OK) return rs;
+    }
     hipLaunchKernelGGL(k_transpose_tile_packed, dim3((unsigned)((nst + 3) / 4), (unsigned)(np / 64)), dim3(256), 0,
-                       s, B, ldb, k, n, nst, (uint8_t *)pb);
+                       s, B, ldb, k, n, nst, (uint8_t *)pb, gate);
     MELA_HIP(hipGetLastError());
     const size_t tx = mp / XROWS, ty = np / XROWS;
     if (tx * ty > (size_t)INT32_MAX) return set_error(MELA_ERR_INVALID, "too many tiles for one launch");
     prof_mark(ctx, true);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mfma_direct<PK_DIRECT>), dim3((unsigned)(tx * ty)), dim3(256), 0, s,
                        (const uint8_t *)pa, (const uint8_t *)pb, (int)nst, C, ldc, (int)m, (int)n, (int)tx,
-                       (int)ty);
+                       (int)ty, gate);
     prof_mark(ctx, false);
     MELA_HIP(hipGetLastError());
     return MELA_OK;
@@ -741,7 +772,8 @@ mela_status matmul_mfma_packed(mela_ctx *ctx, const uint64_t *A, size_t lda, con
 // C = A * B via the matrix cores; fp4 or int8 expansion.
 mela_status matmul_mfma(mela_ctx *ctx, const uint64_t *A, size_t lda, const uint64_t *B,
                         size_t ldb, uint64_t *C, size_t ldc, size_t m, size_t k, size_t n,
-                        bool fp4) {
+                        bool fp4, Gate gate, unsigned long long *count,
+                        const std::function<mela_status(Gate)> &sparse_path) {
     hipStream_t s = ctx->stream;
     {
         // fp4 grids of >= 256 big tiles: packed operands expanded in
@@ -752,7 +784,7 @@ mela_status matmul_mfma(mela_ctx *ctx, const uint64_t *A, size_t lda, const uint
             return e && *e && *e != '0';
         }();
         if (fp4 && !expanded && t256 >= (size_t)ctx->cu_count)
-            return matmul_mfma_packed(ctx, A, lda, B, ldb, C, ldc, m, k, n);
+            return matmul_mfma_packed(ctx, A, lda, B, ldb, C, ldc, m, k, n, gate, count, sparse_path);
     }
     const size_t kw = words64(k);
     const size_t xbytes = kw * (fp4 ? 32 : 64);  // expanded bytes per row
@@ -762,6 +794,12 @@ mela_status matmul_mfma(mela_ctx *ctx, const uint64_t *A, size_t lda, const uint
     mela_status st;
     if ((st = scratch_get(ctx, SLOT_XA, mp * pitch, &pa)) != MELA_OK) return st;
     if ((st = scratch_get(ctx, SLOT_XB, np * pitch, &pb)) != MELA_OK) return st;
+    if (count) {
+        // expanded-operand paths: a separate count, then the row-XOR product
+        mela_status cs = launch_popcount(ctx, A, lda, m, kw, count);
+        if (cs == MELA_OK && sparse_path) cs = sparse_path(gate);
+        if (cs != MELA_OK) return cs;
+    }
     const size_t ca = mp * (pitch / 32);
     const size_t kwp = pitch / (fp4 ? 32 : 64);  // 64-bit k words the padded pitch holds
     if (np / 64 > 65535 || (kwp + 3) / 4 > 0x7fffffffu)
@@ -782,25 +820,25 @@ mela_status matmul_mfma(mela_ctx *ctx, const uint64_t *A, size_t lda, const uint
     if (fp4) {
         if (ring)
             hipLaunchKernelGGL(k_expand_tiled<true>, gx, dim3(256), 0, s, A,
-                               lda, m, kw, pitch, (uint8_t *)pa);
+                               lda, m, kw, pitch, (uint8_t *)pa, gate);
         else
             hipLaunchKernelGGL(k_expand<true>, dim3((unsigned)((ca + 255) / 256)), dim3(256), 0, s,
-                               A, lda, m, kw, mp, pitch, (uint8_t *)pa);
+                               A, lda, m, kw, mp, pitch, (uint8_t *)pa, gate);
         if (ring)
             hipLaunchKernelGGL((k_transpose_expand<true, 2>), gt, dim3(256), 0, s, B, ldb, k, n, kwp,
-                               (uint8_t *)pb, pitch, true);
+                               (uint8_t *)pb, pitch, true, gate);
         else
             hipLaunchKernelGGL((k_transpose_expand<true, 1>), gt, dim3(256), 0, s, B, ldb, k, n, kwp,
-                               (uint8_t *)pb, pitch, false);
+                               (uint8_t *)pb, pitch, false, gate);
     } else {
         if (ring)
             hipLaunchKernelGGL(k_expand_tiled<false>, gx, dim3(256), 0, s,
-                               A, lda, m, kw, pitch, (uint8_t *)pa);
+                               A, lda, m, kw, pitch, (uint8_t *)pa, gate);
         else
             hipLaunchKernelGGL(k_expand<false>, dim3((unsigned)((ca + 255) / 256)), dim3(256), 0, s,
-                               A, lda, m, kw, mp, pitch, (uint8_t *)pa);
+                               A, lda, m, kw, mp, pitch, (uint8_t *)pa, gate);
         hipLaunchKernelGGL((k_transpose_expand<false, 1>), gt, dim3(256), 0, s, B, ldb, k, n, kwp,
-                           (uint8_t *)pb, pitch, ring);
+                           (uint8_t *)pb, pitch, ring, gate);
     }
     MELA_HIP(hipGetLastError());
     const uint8_t *xa = (const uint8_t *)pa, *xb = (const uint8_t *)pb;
@@ -809,16 +847,16 @@ mela_status matmul_mfma(mela_ctx *ctx, const uint64_t *A, size_t lda, const uint
         if (tx2 * ty2 > (size_t)INT32_MAX) return set_error(MELA_ERR_INVALID, "too many tiles for one launch");
         const dim3 grid((unsigned)(tx2 * ty2));
         if (fp4)
-            hipLaunchKernelGGL(k_mfma_ring<true>, grid, dim3(256), 0, s, xa, xb, pitch, C, ldc, (int)m, (int)n, (int)tx2, (int)ty2);
+            hipLaunchKernelGGL(k_mfma_ring<true>, grid, dim3(256), 0, s, xa, xb, pitch, C, ldc, (int)m, (int)n, (int)tx2, (int)ty2, gate);
         else
-            hipLaunchKernelGGL(k_mfma_ring<false>, grid, dim3(256), 0, s, xa, xb, pitch, C, ldc, (int)m, (int)n, (int)tx2, (int)ty2);
+            hipLaunchKernelGGL(k_mfma_ring<false>, grid, dim3(256), 0, s, xa, xb, pitch, C, ldc, (int)m, (int)n, (int)tx2, (int)ty2, gate);
     } else {
         const size_t tx1 = (m + 127) / 128, ty1 = (n + 127) / 128;
         const dim3 grid((unsigned)(tx1 * ty1));
         if (fp4)
-            hipLaunchKernelGGL(k_mfma_small<true>, grid, dim3(256), 0, s, xa, xb, pitch, C, ldc, (int)m, (int)n, (int)tx1, (int)ty1);
+            hipLaunchKernelGGL(k_mfma_small<true>, grid, dim3(256), 0, s, xa, xb, pitch, C, ldc, (int)m, (int)n, (int)tx1, (int)ty1, gate);
         else
-            hipLaunchKernelGGL(k_mfma_small<false>, grid, dim3(256), 0, s, xa, xb, pitch, C, ldc, (int)m, (int)n, (int)tx1, (int)ty1);
+            hipLaunchKernelGGL(k_mfma_small<false>, grid, dim3(256), 0, s, xa, xb, pitch, C, ldc, (int)m, (int)n, (int)tx1, (int)ty1, gate);
     }
     prof_mark(ctx, false);
     MELA_HIP(hipGetLastError());

@@ -92,7 +92,7 @@ def test_matmul_double_parity(mela, orc):
     assert np.array_equal(mela.mela_matmul_m4ri(A, B.astype(bool)), orc.matmul(A, B.astype(bool), "gf2"))
 
 
-@pytest.mark.parametrize("algo", [1, 2, 3, 4])
+@pytest.mark.parametrize("algo", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("shape", [(64, 64, 64), (129, 4097, 300), (2100, 520, 4160), (1000, 8, 9),
                                    (33, 31, 95), (257, 2000, 129)])
 def test_matmul_packed_algos(mela, ctx, orc, algo, shape):
@@ -125,6 +125,26 @@ def test_matmul_mfma_large_tiles_exact(mela, ctx, orc, algo, shape):
     ctx.matmul_packed(dA, dB, dC, algo)
     got = ctx.download(dC)
     want = orc.matmul_packed(Ap, m, k, Bp, n, "m4ri")
+    for M in (dA, dB, dC):
+        ctx.free(M)
+    assert np.array_equal(got[:, : want.shape[1]], want)
+
+
+@pytest.mark.parametrize("algo", [0, 5])
+@pytest.mark.parametrize("m,k,n,w", [(6000, 8000, 5000, 3), (4096, 4096, 9000, 6), (300, 20000, 700, 4)])
+def test_matmul_sparse_operand(mela, ctx, orc, algo, m, k, n, w):
+    """A with exactly w ones per column (parity-check-like), B dense: AUTO
+    (which counts the bits of A and takes the row-XOR product when A is
+    sparse) and the row-XOR product itself equal the oracle."""
+    from mex_linear_algebra_amd import workloads as W
+    print(f"[matmul sparse] algo={algo} {m}x{k}x{n} w={w}", flush=True)
+    rng = np.random.default_rng(m + k + n + w)
+    Ap = W.parity_check_matrix(rng, m, k, w)
+    Bp = orc.gen_bernoulli_packed(91, k, n)
+    dA, dB, dC = ctx.upload(Ap, k), ctx.upload(Bp, n), ctx.bitmat(m, n)
+    ctx.matmul_packed(dA, dB, dC, algo)
+    got = ctx.download(dC)
+    want = orc.matmul_packed(Ap, m, k, Bp, n, "gf2")
     for M in (dA, dB, dC):
         ctx.free(M)
     assert np.array_equal(got[:, : want.shape[1]], want)
