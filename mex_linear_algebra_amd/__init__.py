@@ -26,7 +26,7 @@ __all__ = [
     "ALGO_AUTO", "ALGO_ANDPOP", "ALGO_M4RM", "ALGO_MFMA_I8", "ALGO_MFMA_FP4", "ALGO_ROWXOR",
     "ELIM_GE", "ELIM_M4RI",
     "ELIM_AUTO",
-    "Batch", "BATCH_NULL", "BATCH_GE", "BATCH_GRAPH",
+    "Batch", "BATCH_NULL", "BATCH_GE", "BATCH_GRAPH", "MatmulBatch",
     "Server", "Request", "Result", "OP_RANK", "OP_NULL", "OP_MATMUL",
 ]
 
@@ -173,6 +173,16 @@ def load():
         L.mela_batch_results.restype = st
         L.mela_batch_destroy.argtypes = [bp]
         L.mela_batch_destroy.restype = None
+        L.mela_matmul_batch_create.argtypes = [ctxp, bm, bm, bm, sz, C.c_int, C.POINTER(bp)]
+        L.mela_matmul_batch_create.restype = st
+        L.mela_matmul_batch_set_algos.argtypes = [bp, C.POINTER(C.c_int)]
+        L.mela_matmul_batch_set_algos.restype = st
+        L.mela_matmul_batch_get_algos.argtypes = [bp, C.POINTER(C.c_int)]
+        L.mela_matmul_batch_get_algos.restype = st
+        L.mela_matmul_batch_run.argtypes = [bp]
+        L.mela_matmul_batch_run.restype = st
+        L.mela_matmul_batch_destroy.argtypes = [bp]
+        L.mela_matmul_batch_destroy.restype = None
         srv = C.c_void_p
         L.mela_server_create.argtypes = [C.c_int, sz, C.c_int, C.c_int, C.POINTER(srv)]
         L.mela_server_create.restype = st
@@ -412,9 +422,55 @@ class Batch:
         return [ranks[i] for i in range(self.count)], [nulls[i] for i in range(self.count)]
 
     def close(self):
-        if self.handle:
+        # destroying a batch needs its context (stream and lock); after the
+        # context is closed the batch's device memory stays with the process
+        if self.handle and self.ctx.handle:
             self.lib.mela_batch_destroy(self.handle)
-            self.handle = None
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MatmulBatch:
+    """C_i = A_i * B_i for many uneven packed device products in one
+    persistent launch (mela_matmul_batch_* in include/mela.h).  The C_i are
+    caller-allocated device matrices."""
+
+    def __init__(self, ctx: Context, As, Bs, Cs, flags: int = BATCH_GRAPH):
+        if not (len(As) == len(Bs) == len(Cs)):
+            raise MelaError("A, B and C lists must have the same length")
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.count = len(As)
+        cnt = max(self.count, 1)
+        self._A = (BitMatrix * cnt)(*As)
+        self._B = (BitMatrix * cnt)(*Bs)
+        self._C = (BitMatrix * cnt)(*Cs)
+        h = C.c_void_p()
+        _check(self.lib.mela_matmul_batch_create(ctx.handle, self._A, self._B, self._C, self.count,
+                                                 flags, C.byref(h)))
+        self.handle = h
+
+    def algos(self):
+        arr = (C.c_int * max(self.count, 1))()
+        _check(self.lib.mela_matmul_batch_get_algos(self.handle, arr))
+        return [arr[i] for i in range(self.count)]
+
+    def set_algos(self, algos):
+        arr = (C.c_int * max(self.count, 1))(*[int(a) for a in algos])
+        _check(self.lib.mela_matmul_batch_set_algos(self.handle, arr))
+
+    def run(self):
+        _check(self.lib.mela_matmul_batch_run(self.handle))
+
+    def close(self):
+        if self.handle and self.ctx.handle:
+            self.lib.mela_matmul_batch_destroy(self.handle)
+        self.handle = None
 
     def __del__(self):
         try:

@@ -63,3 +63,17 @@ def batch_c4(count: int = 4096, seed: int = 4, mmin: int = 64, mmax: int = 2048)
         else:
             out.append((dense_matrix(rng, m, n), m, n, "dense"))
     return out
+
+
+def batch_matmul_items(count: int = 2048, seed: int = 5, mmin: int = 64, mmax: int = 2048,
+                       bmin: int = 32, bmax: int = 1024):
+    """Uneven product batch C_i = A_i * B_i: A_i as in batch_c4 (m log-uniform,
+    k = m + U{0..m}, 90% column weight 3/4/6, 10% dense), B_i a dense k x b
+    block with b log-uniform in [bmin, bmax] (e.g. syndromes of b error
+    patterns).  Returns a list of (A packed, m, k, B packed, b, kind)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for P, m, k, kind in batch_c4(count, seed=seed, mmin=mmin, mmax=mmax):
+        b = int(np.floor(np.exp(rng.uniform(np.log(bmin), np.log(bmax)))))
+        out.append((P, m, k, dense_matrix(rng, k, b), b, kind))
+    return out
