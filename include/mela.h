@@ -281,11 +281,17 @@ void mela_matmul_batch_destroy(mela_matmul_batch *batch);
 /* ---------------------------------------------------------------- serving
  *
  * No reference counterpart.  A server admits mixed rank / null-space / matmul
- * requests on device-resident packed inputs by an HBM budget, batches the
- * admitted rank and null requests continuously into the persistent batch
- * kernel, runs jobs concurrently on `workers` streams, and reports latency
- * percentiles.  Inputs must stay valid until the request completes; outputs
- * (null basis or product) stay on the device until mela_server_release. */
+ * requests on device-resident packed inputs in arrival order while the
+ * device memory they need fits hbm_budget_bytes (everything the server holds
+ * counts: its buffer pool, its workers' scratch caches, outputs not yet
+ * released, and an upper bound for each running job).  Rank and null
+ * requests that one workgroup finishes quickly go to a device-resident loop,
+ * a persistent kernel that takes each request as soon as it is published
+ * (at most max_batch of them on the device at once, up to 4096); larger ones
+ * and all products run on `workers` streams with the single-matrix entry
+ * points.  Latency percentiles are kept.  Inputs must stay valid until the
+ * request completes; outputs (null basis or product) stay on the device
+ * until mela_server_release. */
 enum { MELA_OP_RANK = 0, MELA_OP_NULL = 1, MELA_OP_MATMUL = 2 };
 typedef struct mela_request {
     int op;          /* MELA_OP_* */
@@ -303,7 +309,9 @@ typedef struct mela_server_stats {
     uint64_t submitted, completed, rejected, cancelled, failed;
     double p50_ms, p99_ms, mean_ms;
     size_t hbm_in_use, hbm_peak;
-    uint64_t jobs, batched_requests;
+    uint64_t jobs;              /* requests run by the workers */
+    uint64_t batched_requests;  /* requests run by the device-resident loop */
+    uint64_t loop_launches;     /* starts of the loop kernel (an instance lives <= 1 ms) */
 } mela_server_stats;
 typedef struct mela_server mela_server;
 mela_status mela_server_create(int device, size_t hbm_budget_bytes, int workers,

@@ -75,7 +75,8 @@ class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
                 ("cancelled", C.c_uint64), ("failed", C.c_uint64), ("p50_ms", C.c_double),
                 ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("hbm_in_use", C.c_size_t),
-                ("hbm_peak", C.c_size_t), ("jobs", C.c_uint64), ("batched_requests", C.c_uint64)]
+                ("hbm_peak", C.c_size_t), ("jobs", C.c_uint64), ("batched_requests", C.c_uint64),
+                ("loop_launches", C.c_uint64)]
 
 _lib = None
 _lib_lock = threading.Lock()

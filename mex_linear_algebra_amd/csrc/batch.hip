@@ -124,6 +124,374 @@ __device__ __forceinline__ bool group_sync(unsigned *bar, int G, unsigned phase,
     return s_ok != 0;
 }
 
+// One matrix of a batch (member mk of its d.groups workgroups): copy-in,
+// panel-by-panel elimination, rank and (want_null) the packed canonical null
+// basis into Z.  Returns false when a group barrier gave up (the batch then
+// fails as a whole).
+template <int TW>
+__device__ bool batch_matrix(BatchLDS<TW> &L, const BatchMat &d, int mk, unsigned *bar, int *err,
+                             const uint64_t *__restrict__ A, uint64_t *__restrict__ work,
+                             uint64_t *__restrict__ zarena, uint32_t *__restrict__ rank_out,
+                             int want_null) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int G = (int)d.groups;
+    unsigned phase = 0;
+    const int m = (int)d.m, n = (int)d.n;
+    const int wc = (n + 63) >> 6;
+    const int rmax = m < n ? m : n;
+    uint64_t *M = work + d.w_off;
+    uint64_t *Q = work + d.q_off;
+
+    {
+        // member mk copies its share of the words
+        constexpr int CU = 8;  // words per thread with loads in flight together
+        const int share = (m * wc + G - 1) / G;
+        const int mw = min(m * wc, (mk + 1) * share);
+        for (int i0 = mk * share + tid; i0 < mw; i0 += BT * CU) {
+            uint64_t v[CU];
+#pragma unroll
+            for (int k = 0; k < CU; ++k) {
+                const int i = i0 + k * BT < mw ? i0 + k * BT : i0;
+                const int r = i / wc, w = i - r * wc;
+                v[k] = A[(size_t)r * d.lda + w];
+            }
+#pragma unroll
+            for (int k = 0; k < CU; ++k)
+                if (i0 + k * BT < mw) M[i0 + k * BT] = v[k];
+        }
+    }
+    for (int i = tid; i < m; i += BT) L.act[i] = (uint16_t)i;
+    for (int i = tid; i < wc; i += BT) L.pmask[i] = 0;
+    if (tid == 0) {
+        L.s_r = 0;
+        L.s_nact = m;
+    }
+    if (!group_sync(bar, G, ++phase, err, L.s_ok)) return false;
+
+    // Members run the same panel selection on the same words (identical
+    // results); the tiles of each panel are dealt out round robin; one
+    // group barrier per panel.  Member 0 owns tile 0 (word g), so it
+    // updates word g of the earlier pivot rows, after the barrier (when
+    // every member has read it).
+    bool dropped = false;
+    for (int g = 0; g < wc; ++g) {
+        const int n_act = L.s_nact, r0 = L.s_r;
+        if (n_act == 0 || r0 >= rmax) break;
+        auto rowfn = [&](int q) { return M + (size_t)q * wc; };
+        const dev::PanelOut po =
+            dev::wg_panel_select<BT>(L.sel, L.act, n_act, g, rowfn, L.comb, L.selrow);
+        const uint64_t pm = po.pm;
+        if (pm == 0) continue;  // no pivot in this panel (uniform)
+        const int rho = po.rho;
+        if (tid == 0) {
+            L.pmask[g] = pm;
+            L.s_r = r0 + rho;
+            L.s_nact = po.n_act_new;
+        }
+        // the new pivot rows are zero left of their panel
+        for (int e = mk == 0 ? tid : rho * g; e < rho * g; e += BT) {
+            const int j = e / g, w = e - j * g;
+            Q[(size_t)(r0 + j) * wc + w] = 0ull;
+        }
+        // active rows and (for the null basis) earlier pivot rows
+        const int nact = po.n_act_new;
+        const int total = nact + (want_null ? r0 : 0);
+        // rows of sweep positions: active rows by row id, earlier pivot
+        // rows (Q) by 0x8000 | index
+        auto sweep_row = [&](uint32_t code) -> uint64_t * {
+            return code & 0x8000u ? Q + (size_t)(code & 0x7fffu) * wc : M + (size_t)code * wc;
+        };
+        // list of the positions whose multiplier (word g & pm) is nonzero
+        int nupd = 0;
+        {
+            constexpr int CK = 4;  // positions per thread per round, loads together
+            for (int b0 = 0; b0 < total; b0 += BT * CK) {
+                uint32_t code[CK];
+                bool f[CK];
+#pragma unroll
+                for (int k = 0; k < CK; ++k) {
+                    const int q = b0 + k * BT + tid;
+                    const int qa = q < nact ? q : 0;
+                    code[k] = q < nact ? (uint32_t)L.act[qa] : 0x8000u | (uint32_t)(q - nact);
+                    if (q >= total) code[k] = 0;
+                }
+                uint64_t xv[CK];
+#pragma unroll
+                for (int k = 0; k < CK; ++k) xv[k] = sweep_row(code[k])[g];
+                // wave-local slots, k-major: entry k of lane l goes to
+                // (entries of rounds < k) + (lanes < l with entry k)
+                int wsum = 0, off[CK];
+                const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+                for (int k = 0; k < CK; ++k) {
+                    f[k] = b0 + k * BT + tid < total && (xv[k] & pm) != 0ull;
+                    const uint64_t bal = __ballot(f[k]);
+                    off[k] = wsum + dev::popc64(bal & below);
+                    wsum += dev::popc64(bal);
+                }
+                if (lane == 0) L.sel.wcnt[wave] = wsum;
+                __syncthreads();
+                int base = nupd, all = 0;
+#pragma unroll
+                for (int v = 0; v < BT_WAVES; ++v) {
+                    const int c = L.sel.wcnt[v];
+                    if (v < wave) base += c;
+                    all += c;
+                }
+#pragma unroll
+                for (int k = 0; k < CK; ++k)
+                    if (f[k]) L.upd[base + off[k]] = (uint16_t)code[k];
+                nupd += all;
+                __syncthreads();
+            }
+        }
+        constexpr int RPP = 64 / TW;  // rows per wave pass, TW lanes (words) per row
+        const int sub = lane / TW, wl = lane % TW;
+        for (int w0 = g + mk * TW; w0 < wc; w0 += G * TW) {
+            // selected rows' words of the tile (loads issued together;
+            // slots past rho and words past wc read row 0 and store 0)
+            {
+                constexpr int SE = 64 * TW / BT;
+                uint64_t sv[SE];
+#pragma unroll
+                for (int k = 0; k < SE; ++k) {
+                    const int e = tid + k * BT, slot = e / TW, ww = e - slot * TW;
+                    const bool ok = slot < rho && w0 + ww < wc;
+                    sv[k] = M[ok ? (size_t)L.selrow[slot] * wc + w0 + ww : 0];
+                }
+#pragma unroll
+                for (int k = 0; k < SE; ++k) {
+                    const int e = tid + k * BT, slot = e / TW, ww = e - slot * TW;
+                    L.u.S[slot][ww] = (slot < rho && w0 + ww < wc) ? sv[k] : 0ull;
+                }
+            }
+            __syncthreads();
+            // Thread (t, ww) owns table t of tile word ww: first the 4-bit
+            // tables of the selected rows 4t..4t+3, from which it forms
+            // the pivot rows 4t..4t+3 (pivot row c = XOR of the selected
+            // rows in comb[c], 16 lookups), then the tables of those.
+            constexpr int TPT = (16 * TW + BT - 1) / BT;  // tables per thread
+            uint64_t q[TPT][4];
+#pragma unroll
+            for (int k = 0; k < TPT; ++k) {
+                const int e = tid + k * BT, t = e / TW, ww = e - t * TW;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) q[k][b] = e < 16 * TW ? L.u.S[4 * t + b][ww] : 0ull;
+            }
+            __syncthreads();
+            auto build_tables = [&]() {
+#pragma unroll
+                for (int k = 0; k < TPT; ++k) {
+                    const int e = tid + k * BT, t = e / TW, ww = e - t * TW;
+                    if (e >= 16 * TW) continue;
+                    uint64_t cur = 0;
+                    L.u.T[t][0][ww] = 0;
+#pragma unroll
+                    for (int v = 1; v < 16; ++v) {
+                        cur ^= q[k][__builtin_ctz(v)];
+                        L.u.T[t][v ^ (v >> 1)][ww] = cur;  // Gray-code order
+                    }
+                }
+            };
+            build_tables();
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < TPT; ++k) {
+                const int e = tid + k * BT, t = e / TW, ww = e - t * TW;
+                const int w = w0 + ww;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int c = 4 * t + b;
+                    const bool pc = e < 16 * TW && ((pm >> c) & 1ull);
+                    const uint64_t cb = pc ? L.comb[c] : 0ull;
+                    uint64_t v = 0;
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) v ^= L.u.T[u][(cb >> (4 * u)) & 15u][e < 16 * TW ? ww : 0];
+                    q[k][b] = v;
+                    if (pc && w < wc) Q[(size_t)(r0 + dev::popc64(pm & ((1ull << c) - 1ull))) * wc + w] = v;
+                    if (pc && w == g) L.pw[c] = v;
+                }
+            }
+            __syncthreads();
+            if (d.mode == 1) {
+                build_tables();
+            } else {
+#pragma unroll
+                for (int k = 0; k < TPT; ++k) {
+                    const int e = tid + k * BT, t = e / TW, ww = e - t * TW;
+                    if (e >= 16 * TW) continue;
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) L.u.P[4 * t + b][ww] = q[k][b];
+                }
+            }
+            __syncthreads();
+            // sweep words > g of the tile; SW_PASSES passes' loads together
+            const int w = w0 + wl;
+            const bool wok = w < wc && w > g;
+            const int wsafe = wok ? w : g;
+#pragma unroll 1
+            for (int i0 = wave * RPP + sub; i0 < nupd; i0 += BT_WAVES * RPP * SW_PASSES) {
+                // every pass loads unconditionally (entries past the end
+                // re-read row 0 of M) so all loads are in flight together
+                uint64_t *rp[SW_PASSES];
+                uint64_t x[SW_PASSES], a[SW_PASSES];
+#pragma unroll
+                for (int j = 0; j < SW_PASSES; ++j) {
+                    const int i = i0 + j * BT_WAVES * RPP;
+                    rp[j] = i < nupd ? sweep_row(L.upd[i]) : M;
+                }
+#pragma unroll
+                for (int j = 0; j < SW_PASSES; ++j) x[j] = rp[j][g];
+#pragma unroll
+                for (int j = 0; j < SW_PASSES; ++j) a[j] = rp[j][wsafe];
+#pragma unroll
+                for (int j = 0; j < SW_PASSES; ++j) {
+                    const int i = i0 + j * BT_WAVES * RPP;
+                    x[j] = (i < nupd && wok) ? x[j] & pm : 0ull;
+                }
+                if (d.mode == 1) {
+                    // table-major: one lookup of every pass per table, so
+                    // the passes' LDS reads overlap (a zero multiplier
+                    // reads the zero entry)
+#pragma unroll
+                    for (int t = 0; t < 16; ++t)
+#pragma unroll
+                        for (int j = 0; j < SW_PASSES; ++j) a[j] ^= L.u.T[t][(x[j] >> (4 * t)) & 15u][wl];
+                } else {
+#pragma unroll
+                    for (int j = 0; j < SW_PASSES; ++j)
+                        for (uint64_t xx = x[j]; xx; xx &= xx - 1) a[j] ^= L.u.P[dev::ffs64(xx)][wl];
+                }
+#pragma unroll
+                for (int j = 0; j < SW_PASSES; ++j)
+                    if (x[j]) rp[j][w] = a[j];
+            }
+            __syncthreads();
+        }
+        if (!group_sync(bar, G, ++phase, err, L.s_ok)) {
+            dropped = true;
+            break;
+        }
+        // word g of the earlier pivot rows (every member has read it)
+        if (want_null && mk == 0) {
+            constexpr int PU = 4;  // rows per thread with loads in flight together
+            for (int q0 = tid; q0 < r0; q0 += BT * PU) {
+                uint64_t xv[PU];
+#pragma unroll
+                for (int k = 0; k < PU; ++k) {
+                    const int q = q0 + k * BT < r0 ? q0 + k * BT : q0;
+                    xv[k] = Q[(size_t)q * wc + g];
+                }
+#pragma unroll
+                for (int k = 0; k < PU; ++k) {
+                    const int q = q0 + k * BT;
+                    if (q >= r0) continue;
+                    uint64_t a = xv[k];
+                    for (uint64_t h = xv[k] & pm; h; h &= h - 1) a ^= L.pw[dev::ffs64(h)];
+                    Q[(size_t)q * wc + g] = a;
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    if (dropped) return false;
+    // the last panel's pivot row words before any member extracts
+    if (!group_sync(bar, G, ++phase, err, L.s_ok)) return false;
+    const int r = L.s_r;
+    if (tid == 0 && mk == 0) *rank_out = (uint32_t)r;
+    if (want_null) {
+        const int f = n - r;
+        // prefix pivot counts per column word
+        if (tid == 0) {
+            int acc = 0;
+            for (int w = 0; w < wc; ++w) {
+                L.pbefore[w] = acc;
+                acc += dev::popc64(L.pmask[w]);
+            }
+        }
+        __syncthreads();
+        // per column word: free mask, compress masks, free columns before
+        // it; ws[j] = the column word that holds free column 64 j
+        for (int w = tid; w < wc; w += BT) {
+            const int valid = n - 64 * w < 64 ? n - 64 * w : 64;
+            const uint64_t fm = ~L.pmask[w] & (valid == 64 ? ~0ull : ((1ull << valid) - 1ull));
+            dev::compress_masks(fm, L.u.nl.mv[w]);
+            L.u.nl.fm[w] = fm;
+            const int fb = 64 * w - L.pbefore[w];
+            L.u.nl.fb[w] = fb;
+            const int j = (fb + 63) >> 6;
+            if (64 * j < fb + dev::popc64(fm)) L.u.nl.ws[j] = w;
+        }
+        __syncthreads();
+        uint64_t *Z = zarena + d.z_off;
+        const int ldz = (int)d.ldz;
+        const int fw = (f + 63) >> 6;
+        // Z row p: a pivot column's row is its pivot row's free columns
+        // (word jw gathers the compressed column words ws[jw]..ws[jw+1]),
+        // a free column's row is its unit vector.  Four words per thread
+        // per step, their row-word loads issued together.
+        constexpr int NZ = 4;
+        const int zshare = (n * ldz + G - 1) / G;
+        const int zend = min(n * ldz, (mk + 1) * zshare);
+        for (int i0 = mk * zshare + tid; i0 < zend; i0 += BT * NZ) {
+            const uint64_t *qrow[NZ];
+            int lo[NZ], wa[NZ], wb[NZ];
+            uint64_t word[NZ];
+#pragma unroll
+            for (int k = 0; k < NZ; ++k) {
+                const int i = i0 + k * BT;
+                const int p = i / ldz, jw = i - p * ldz;
+                qrow[k] = nullptr;
+                word[k] = 0;
+                wa[k] = 0;
+                wb[k] = -1;
+                lo[k] = 64 * jw;
+                if (i < zend && jw < fw) {
+                    const int pb = pivots_before(L, p);
+                    if ((L.pmask[p >> 6] >> (p & 63)) & 1ull) {
+                        qrow[k] = Q + (size_t)pb * wc;
+                        wa[k] = L.u.nl.ws[jw];
+                        wb[k] = jw + 1 < fw ? L.u.nl.ws[jw + 1] : wc - 1;
+                    } else {
+                        const int fi = p - pb;
+                        if ((fi >> 6) == jw) word[k] = 1ull << (fi & 63);
+                    }
+                }
+            }
+            // the first three column words of each output word, loaded
+            // together (unit rows and words past the end re-read Q)
+            uint64_t x0[NZ], x1[NZ], x2[NZ];
+#pragma unroll
+            for (int k = 0; k < NZ; ++k) {
+                const uint64_t *src = qrow[k] ? qrow[k] : Q;
+                x0[k] = src[wa[k] < wc ? wa[k] : wc - 1];
+                x1[k] = src[wa[k] + 1 < wc ? wa[k] + 1 : wc - 1];
+                x2[k] = src[wa[k] + 2 < wc ? wa[k] + 2 : wc - 1];
+            }
+#pragma unroll
+            for (int k = 0; k < NZ; ++k) {
+                // column word w holds free columns fb[w] .. fb[w] + popc - 1
+                auto put = [&](int w, uint64_t xv) {
+                    const int fb = L.u.nl.fb[w];
+                    if (fb >= lo[k] + 64) return;
+                    const uint64_t c = dev::compress_apply(xv, L.u.nl.fm[w], L.u.nl.mv[w]);
+                    const int sh = fb - lo[k];
+                    word[k] |= sh >= 0 ? c << sh : c >> (-sh);
+                };
+                const int nw = wb[k] - wa[k] + 1;
+                if (nw > 0) put(wa[k], x0[k]);
+                if (nw > 1) put(wa[k] + 1, x1[k]);
+                if (nw > 2) put(wa[k] + 2, x2[k]);
+                for (int w = wa[k] + 3; w <= wb[k]; ++w) put(w, qrow[k][w]);
+                const int i = i0 + k * BT;
+                if (i < zend) Z[i] = word[k];
+            }
+        }
+    }
+    return true;
+}
+
 template <int TW>
 __global__ void __launch_bounds__(BT)
 k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order,
@@ -133,7 +501,7 @@ k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order,
              int want_null) {
     extern __shared__ __align__(16) unsigned char smem[];
     BatchLDS<TW> &L = *reinterpret_cast<BatchLDS<TW> *>(smem);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x;
 
     // queue of slots: slot s = member slots[s].y of queue entry slots[s].x;
     // a matrix's G slots are consecutive, so at most one matrix at a time is
@@ -148,364 +516,89 @@ k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order,
         const int qi = sl.x, mk = sl.y;  // queue entry, member index
         const int mi = order[qi];
         const BatchMat d = mats[mi];
-        const int G = (int)d.groups;
-        unsigned *bar = gbar + qi;
-        unsigned phase = 0;
-        const int m = (int)d.m, n = (int)d.n;
-        const int wc = (n + 63) >> 6;
-        const int rmax = m < n ? m : n;
-        uint64_t *M = work + d.w_off;
-        uint64_t *Q = work + d.q_off;
-        const uint64_t *A = inputs[mi];
-
-        {
-            // member mk copies its share of the words
-            constexpr int CU = 8;  // words per thread with loads in flight together
-            const int share = (m * wc + G - 1) / G;
-            const int mw = min(m * wc, (mk + 1) * share);
-            for (int i0 = mk * share + tid; i0 < mw; i0 += BT * CU) {
-                uint64_t v[CU];
-#pragma unroll
-                for (int k = 0; k < CU; ++k) {
-                    const int i = i0 + k * BT < mw ? i0 + k * BT : i0;
-                    const int r = i / wc, w = i - r * wc;
-                    v[k] = A[(size_t)r * d.lda + w];
-                }
-#pragma unroll
-                for (int k = 0; k < CU; ++k)
-                    if (i0 + k * BT < mw) M[i0 + k * BT] = v[k];
-            }
-        }
-        for (int i = tid; i < m; i += BT) L.act[i] = (uint16_t)i;
-        for (int i = tid; i < wc; i += BT) L.pmask[i] = 0;
+        (void)batch_matrix<TW>(L, d, mk, gbar + qi, err, inputs[mi], work, zarena, ranks + mi, want_null);
+        __syncthreads();
+    }
+}
+// The device-resident serving loop: each workgroup takes published request
+// slots (compare-and-swap on head, in device memory, while head < tail),
+// eliminates the matrix with the batch machinery (one workgroup per matrix),
+// and reports it; it ends when the host sets the stop bit of tail, when no
+// slot was published for idle_ticks, or, between requests, once the launch
+// is life_ticks old (the host starts the next instance at once when work
+// remains: a kernel that never ended would hold up every stream that shares
+// its hardware queue).  An idle workgroup polls the host's tail every
+// ~0.3 us at first and every ~4 us after 50 us without work.
+template <int TW>
+__global__ void __launch_bounds__(BT)
+k_serve_loop(const ServeSlot *__restrict__ slots, ServeDone *__restrict__ done,
+             const uint64_t *__restrict__ tail, uint64_t *__restrict__ head, uint32_t cap,
+             uint64_t idle_ticks, uint64_t life_ticks) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    BatchLDS<TW> &L = *reinterpret_cast<BatchLDS<TW> *>(smem);
+    __shared__ long long s_seq;
+    __shared__ ServeSlot s_slot;
+    const int tid = threadIdx.x;
+    const uint64_t t_start = wall_clock64();
+    uint64_t t_idle = t_start;
+    for (;;) {
         if (tid == 0) {
-            L.s_r = 0;
-            L.s_nact = m;
-        }
-        if (!group_sync(bar, G, ++phase, err, L.s_ok)) continue;
-
-        // Members run the same panel selection on the same words (identical
-        // results); the tiles of each panel are dealt out round robin; one
-        // group barrier per panel.  Member 0 owns tile 0 (word g), so it
-        // updates word g of the earlier pivot rows, after the barrier (when
-        // every member has read it).
-        bool dropped = false;
-        for (int g = 0; g < wc; ++g) {
-            const int n_act = L.s_nact, r0 = L.s_r;
-            if (n_act == 0 || r0 >= rmax) break;
-            auto rowfn = [&](int q) { return M + (size_t)q * wc; };
-            const dev::PanelOut po =
-                dev::wg_panel_select<BT>(L.sel, L.act, n_act, g, rowfn, L.comb, L.selrow);
-            const uint64_t pm = po.pm;
-            if (pm == 0) continue;  // no pivot in this panel (uniform)
-            const int rho = po.rho;
-            if (tid == 0) {
-                L.pmask[g] = pm;
-                L.s_r = r0 + rho;
-                L.s_nact = po.n_act_new;
-            }
-            // the new pivot rows are zero left of their panel
-            for (int e = mk == 0 ? tid : rho * g; e < rho * g; e += BT) {
-                const int j = e / g, w = e - j * g;
-                Q[(size_t)(r0 + j) * wc + w] = 0ull;
-            }
-            // active rows and (for the null basis) earlier pivot rows
-            const int nact = po.n_act_new;
-            const int total = nact + (want_null ? r0 : 0);
-            // rows of sweep positions: active rows by row id, earlier pivot
-            // rows (Q) by 0x8000 | index
-            auto sweep_row = [&](uint32_t code) -> uint64_t * {
-                return code & 0x8000u ? Q + (size_t)(code & 0x7fffu) * wc : M + (size_t)code * wc;
-            };
-            // list of the positions whose multiplier (word g & pm) is nonzero
-            int nupd = 0;
-            {
-                constexpr int CK = 4;  // positions per thread per round, loads together
-                for (int b0 = 0; b0 < total; b0 += BT * CK) {
-                    uint32_t code[CK];
-                    bool f[CK];
-#pragma unroll
-                    for (int k = 0; k < CK; ++k) {
-                        const int q = b0 + k * BT + tid;
-                        const int qa = q < nact ? q : 0;
-                        code[k] = q < nact ? (uint32_t)L.act[qa] : 0x8000u | (uint32_t)(q - nact);
-                        if (q >= total) code[k] = 0;
+            long long got = -1;
+            for (;;) {
+                if (wall_clock64() - t_start > life_ticks) break;
+                const uint64_t t = __hip_atomic_load(tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+                if (t & SERVE_STOP) break;
+                const uint64_t h = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (h < t) {
+                    uint64_t expect = h;
+                    if (__hip_atomic_compare_exchange_strong(head, &expect, h + 1, __ATOMIC_RELAXED,
+                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        got = (long long)h;
+                        break;
                     }
-                    uint64_t xv[CK];
-#pragma unroll
-                    for (int k = 0; k < CK; ++k) xv[k] = sweep_row(code[k])[g];
-                    // wave-local slots, k-major: entry k of lane l goes to
-                    // (entries of rounds < k) + (lanes < l with entry k)
-                    int wsum = 0, off[CK];
-                    const uint64_t below = (1ull << lane) - 1ull;
-#pragma unroll
-                    for (int k = 0; k < CK; ++k) {
-                        f[k] = b0 + k * BT + tid < total && (xv[k] & pm) != 0ull;
-                        const uint64_t bal = __ballot(f[k]);
-                        off[k] = wsum + dev::popc64(bal & below);
-                        wsum += dev::popc64(bal);
-                    }
-                    if (lane == 0) L.sel.wcnt[wave] = wsum;
-                    __syncthreads();
-                    int base = nupd, all = 0;
-#pragma unroll
-                    for (int v = 0; v < BT_WAVES; ++v) {
-                        const int c = L.sel.wcnt[v];
-                        if (v < wave) base += c;
-                        all += c;
-                    }
-#pragma unroll
-                    for (int k = 0; k < CK; ++k)
-                        if (f[k]) L.upd[base + off[k]] = (uint16_t)code[k];
-                    nupd += all;
-                    __syncthreads();
+                    continue;
                 }
-            }
-            constexpr int RPP = 64 / TW;  // rows per wave pass, TW lanes (words) per row
-            const int sub = lane / TW, wl = lane % TW;
-            for (int w0 = g + mk * TW; w0 < wc; w0 += G * TW) {
-                // selected rows' words of the tile (loads issued together;
-                // slots past rho and words past wc read row 0 and store 0)
-                {
-                    constexpr int SE = 64 * TW / BT;
-                    uint64_t sv[SE];
-#pragma unroll
-                    for (int k = 0; k < SE; ++k) {
-                        const int e = tid + k * BT, slot = e / TW, ww = e - slot * TW;
-                        const bool ok = slot < rho && w0 + ww < wc;
-                        sv[k] = M[ok ? (size_t)L.selrow[slot] * wc + w0 + ww : 0];
-                    }
-#pragma unroll
-                    for (int k = 0; k < SE; ++k) {
-                        const int e = tid + k * BT, slot = e / TW, ww = e - slot * TW;
-                        L.u.S[slot][ww] = (slot < rho && w0 + ww < wc) ? sv[k] : 0ull;
-                    }
-                }
-                __syncthreads();
-                // Thread (t, ww) owns table t of tile word ww: first the 4-bit
-                // tables of the selected rows 4t..4t+3, from which it forms
-                // the pivot rows 4t..4t+3 (pivot row c = XOR of the selected
-                // rows in comb[c], 16 lookups), then the tables of those.
-                constexpr int TPT = (16 * TW + BT - 1) / BT;  // tables per thread
-                uint64_t q[TPT][4];
-#pragma unroll
-                for (int k = 0; k < TPT; ++k) {
-                    const int e = tid + k * BT, t = e / TW, ww = e - t * TW;
-#pragma unroll
-                    for (int b = 0; b < 4; ++b) q[k][b] = e < 16 * TW ? L.u.S[4 * t + b][ww] : 0ull;
-                }
-                __syncthreads();
-                auto build_tables = [&]() {
-#pragma unroll
-                    for (int k = 0; k < TPT; ++k) {
-                        const int e = tid + k * BT, t = e / TW, ww = e - t * TW;
-                        if (e >= 16 * TW) continue;
-                        uint64_t cur = 0;
-                        L.u.T[t][0][ww] = 0;
-#pragma unroll
-                        for (int v = 1; v < 16; ++v) {
-                            cur ^= q[k][__builtin_ctz(v)];
-                            L.u.T[t][v ^ (v >> 1)][ww] = cur;  // Gray-code order
-                        }
-                    }
-                };
-                build_tables();
-                __syncthreads();
-#pragma unroll
-                for (int k = 0; k < TPT; ++k) {
-                    const int e = tid + k * BT, t = e / TW, ww = e - t * TW;
-                    const int w = w0 + ww;
-#pragma unroll
-                    for (int b = 0; b < 4; ++b) {
-                        const int c = 4 * t + b;
-                        const bool pc = e < 16 * TW && ((pm >> c) & 1ull);
-                        const uint64_t cb = pc ? L.comb[c] : 0ull;
-                        uint64_t v = 0;
-#pragma unroll
-                        for (int u = 0; u < 16; ++u) v ^= L.u.T[u][(cb >> (4 * u)) & 15u][e < 16 * TW ? ww : 0];
-                        q[k][b] = v;
-                        if (pc && w < wc) Q[(size_t)(r0 + dev::popc64(pm & ((1ull << c) - 1ull))) * wc + w] = v;
-                        if (pc && w == g) L.pw[c] = v;
-                    }
-                }
-                __syncthreads();
-                if (d.mode == 1) {
-                    build_tables();
+                const uint64_t idle = wall_clock64() - t_idle;
+                if (idle > idle_ticks) break;
+                if (idle > 5000) {  // 50 us
+                    __builtin_amdgcn_s_sleep(127);
+                    __builtin_amdgcn_s_sleep(127);
                 } else {
-#pragma unroll
-                    for (int k = 0; k < TPT; ++k) {
-                        const int e = tid + k * BT, t = e / TW, ww = e - t * TW;
-                        if (e >= 16 * TW) continue;
-#pragma unroll
-                        for (int b = 0; b < 4; ++b) L.u.P[4 * t + b][ww] = q[k][b];
-                    }
-                }
-                __syncthreads();
-                // sweep words > g of the tile; SW_PASSES passes' loads together
-                const int w = w0 + wl;
-                const bool wok = w < wc && w > g;
-                const int wsafe = wok ? w : g;
-#pragma unroll 1
-                for (int i0 = wave * RPP + sub; i0 < nupd; i0 += BT_WAVES * RPP * SW_PASSES) {
-                    // every pass loads unconditionally (entries past the end
-                    // re-read row 0 of M) so all loads are in flight together
-                    uint64_t *rp[SW_PASSES];
-                    uint64_t x[SW_PASSES], a[SW_PASSES];
-#pragma unroll
-                    for (int j = 0; j < SW_PASSES; ++j) {
-                        const int i = i0 + j * BT_WAVES * RPP;
-                        rp[j] = i < nupd ? sweep_row(L.upd[i]) : M;
-                    }
-#pragma unroll
-                    for (int j = 0; j < SW_PASSES; ++j) x[j] = rp[j][g];
-#pragma unroll
-                    for (int j = 0; j < SW_PASSES; ++j) a[j] = rp[j][wsafe];
-#pragma unroll
-                    for (int j = 0; j < SW_PASSES; ++j) {
-                        const int i = i0 + j * BT_WAVES * RPP;
-                        x[j] = (i < nupd && wok) ? x[j] & pm : 0ull;
-                    }
-                    if (d.mode == 1) {
-                        // table-major: one lookup of every pass per table, so
-                        // the passes' LDS reads overlap (a zero multiplier
-                        // reads the zero entry)
-#pragma unroll
-                        for (int t = 0; t < 16; ++t)
-#pragma unroll
-                            for (int j = 0; j < SW_PASSES; ++j) a[j] ^= L.u.T[t][(x[j] >> (4 * t)) & 15u][wl];
-                    } else {
-#pragma unroll
-                        for (int j = 0; j < SW_PASSES; ++j)
-                            for (uint64_t xx = x[j]; xx; xx &= xx - 1) a[j] ^= L.u.P[dev::ffs64(xx)][wl];
-                    }
-#pragma unroll
-                    for (int j = 0; j < SW_PASSES; ++j)
-                        if (x[j]) rp[j][w] = a[j];
-                }
-                __syncthreads();
-            }
-            if (!group_sync(bar, G, ++phase, err, L.s_ok)) {
-                dropped = true;
-                break;
-            }
-            // word g of the earlier pivot rows (every member has read it)
-            if (want_null && mk == 0) {
-                constexpr int PU = 4;  // rows per thread with loads in flight together
-                for (int q0 = tid; q0 < r0; q0 += BT * PU) {
-                    uint64_t xv[PU];
-#pragma unroll
-                    for (int k = 0; k < PU; ++k) {
-                        const int q = q0 + k * BT < r0 ? q0 + k * BT : q0;
-                        xv[k] = Q[(size_t)q * wc + g];
-                    }
-#pragma unroll
-                    for (int k = 0; k < PU; ++k) {
-                        const int q = q0 + k * BT;
-                        if (q >= r0) continue;
-                        uint64_t a = xv[k];
-                        for (uint64_t h = xv[k] & pm; h; h &= h - 1) a ^= L.pw[dev::ffs64(h)];
-                        Q[(size_t)q * wc + g] = a;
-                    }
+                    __builtin_amdgcn_s_sleep(8);
                 }
             }
-            __syncthreads();
-        }
-
-        if (dropped) continue;
-        // the last panel's pivot row words before any member extracts
-        if (!group_sync(bar, G, ++phase, err, L.s_ok)) continue;
-        const int r = L.s_r;
-        if (tid == 0 && mk == 0) ranks[mi] = (uint32_t)r;
-        if (want_null) {
-            const int f = n - r;
-            // prefix pivot counts per column word
-            if (tid == 0) {
-                int acc = 0;
-                for (int w = 0; w < wc; ++w) {
-                    L.pbefore[w] = acc;
-                    acc += dev::popc64(L.pmask[w]);
-                }
+            if (got >= 0) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");  // the slot was written before tail
+                s_slot = slots[(uint64_t)got % cap];  // one read of host memory per request
             }
-            __syncthreads();
-            // per column word: free mask, compress masks, free columns before
-            // it; ws[j] = the column word that holds free column 64 j
-            for (int w = tid; w < wc; w += BT) {
-                const int valid = n - 64 * w < 64 ? n - 64 * w : 64;
-                const uint64_t fm = ~L.pmask[w] & (valid == 64 ? ~0ull : ((1ull << valid) - 1ull));
-                dev::compress_masks(fm, L.u.nl.mv[w]);
-                L.u.nl.fm[w] = fm;
-                const int fb = 64 * w - L.pbefore[w];
-                L.u.nl.fb[w] = fb;
-                const int j = (fb + 63) >> 6;
-                if (64 * j < fb + dev::popc64(fm)) L.u.nl.ws[j] = w;
-            }
-            __syncthreads();
-            uint64_t *Z = zarena + d.z_off;
-            const int ldz = (int)d.ldz;
-            const int fw = (f + 63) >> 6;
-            // Z row p: a pivot column's row is its pivot row's free columns
-            // (word jw gathers the compressed column words ws[jw]..ws[jw+1]),
-            // a free column's row is its unit vector.  Four words per thread
-            // per step, their row-word loads issued together.
-            constexpr int NZ = 4;
-            const int zshare = (n * ldz + G - 1) / G;
-            const int zend = min(n * ldz, (mk + 1) * zshare);
-            for (int i0 = mk * zshare + tid; i0 < zend; i0 += BT * NZ) {
-                const uint64_t *qrow[NZ];
-                int lo[NZ], wa[NZ], wb[NZ];
-                uint64_t word[NZ];
-#pragma unroll
-                for (int k = 0; k < NZ; ++k) {
-                    const int i = i0 + k * BT;
-                    const int p = i / ldz, jw = i - p * ldz;
-                    qrow[k] = nullptr;
-                    word[k] = 0;
-                    wa[k] = 0;
-                    wb[k] = -1;
-                    lo[k] = 64 * jw;
-                    if (i < zend && jw < fw) {
-                        const int pb = pivots_before(L, p);
-                        if ((L.pmask[p >> 6] >> (p & 63)) & 1ull) {
-                            qrow[k] = Q + (size_t)pb * wc;
-                            wa[k] = L.u.nl.ws[jw];
-                            wb[k] = jw + 1 < fw ? L.u.nl.ws[jw + 1] : wc - 1;
-                        } else {
-                            const int fi = p - pb;
-                            if ((fi >> 6) == jw) word[k] = 1ull << (fi & 63);
-                        }
-                    }
-                }
-                // the first three column words of each output word, loaded
-                // together (unit rows and words past the end re-read Q)
-                uint64_t x0[NZ], x1[NZ], x2[NZ];
-#pragma unroll
-                for (int k = 0; k < NZ; ++k) {
-                    const uint64_t *src = qrow[k] ? qrow[k] : Q;
-                    x0[k] = src[wa[k] < wc ? wa[k] : wc - 1];
-                    x1[k] = src[wa[k] + 1 < wc ? wa[k] + 1 : wc - 1];
-                    x2[k] = src[wa[k] + 2 < wc ? wa[k] + 2 : wc - 1];
-                }
-#pragma unroll
-                for (int k = 0; k < NZ; ++k) {
-                    // column word w holds free columns fb[w] .. fb[w] + popc - 1
-                    auto put = [&](int w, uint64_t xv) {
-                        const int fb = L.u.nl.fb[w];
-                        if (fb >= lo[k] + 64) return;
-                        const uint64_t c = dev::compress_apply(xv, L.u.nl.fm[w], L.u.nl.mv[w]);
-                        const int sh = fb - lo[k];
-                        word[k] |= sh >= 0 ? c << sh : c >> (-sh);
-                    };
-                    const int nw = wb[k] - wa[k] + 1;
-                    if (nw > 0) put(wa[k], x0[k]);
-                    if (nw > 1) put(wa[k] + 1, x1[k]);
-                    if (nw > 2) put(wa[k] + 2, x2[k]);
-                    for (int w = wa[k] + 3; w <= wb[k]; ++w) put(w, qrow[k][w]);
-                    const int i = i0 + k * BT;
-                    if (i < zend) Z[i] = word[k];
-                }
-            }
+            s_seq = got;
         }
         __syncthreads();
+        const long long seq = s_seq;
+        if (seq < 0) break;
+        const uint32_t idx = (uint32_t)((uint64_t)seq % cap);
+        const ServeSlot sl = s_slot;
+        BatchMat d{};
+        d.m = sl.m;
+        d.n = sl.n;
+        d.lda = sl.lda;
+        const uint32_t wc = (sl.n + 63) / 64;
+        d.ldz = wc;
+        d.w_off = 0;
+        d.q_off = (uint64_t)sl.m * wc;
+        d.z_off = 0;
+        d.mode = sl.mode;
+        d.groups = 1;
+        (void)batch_matrix<TW>(L, d, 0, nullptr, nullptr, sl.A, sl.work, sl.Z, &done[idx].rank,
+                               sl.op == MELA_OP_NULL ? 1 : 0);
+        // the rank, the null basis and the working rows are written: make them
+        // visible to the host and its copy engines, then report
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+        __syncthreads();
+        if (tid == 0)
+            __hip_atomic_store(&done[idx].seq1, (uint64_t)seq + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        t_idle = wall_clock64();
+        __syncthreads();  // s_seq is rewritten only after every thread has read it
     }
 }
 
@@ -534,6 +627,35 @@ k_batch_popcount(const BatchMat *__restrict__ mats, const uint64_t *const *__res
 }  // namespace
 
 size_t batch_lds_bytes(int tw) { return tw == 32 ? sizeof(BatchLDS<32>) : sizeof(BatchLDS<16>); }
+
+// Serving-loop workgroups: a quarter of the CUs.  The loop stays resident
+// between requests, so the CUs it does not occupy stay free for the kernels
+// of large requests on the worker streams (an MFMA product workgroup needs a
+// whole CU's registers); 64 workgroups of <= 0.35 ms requests still take
+// well over 10^5 requests/s.
+int serve_loop_grid(mela_ctx *ctx) {
+    const int cu = ctx->cu_count > 0 ? ctx->cu_count : 256;
+    return cu / 4 > 8 ? cu / 4 : 8;
+}
+
+mela_status launch_serve_loop(hipStream_t s, int grid, ServeSlot *slots, ServeDone *done, const uint64_t *tail,
+                              uint64_t *head, uint32_t cap, uint64_t idle_ticks, uint64_t life_ticks) {
+    static std::mutex mu;
+    static bool attr = false;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!attr) {
+            MELA_HIP(hipFuncSetAttribute((const void *)k_serve_loop<16>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)batch_lds_bytes(16)));
+            attr = true;
+        }
+    }
+    hipLaunchKernelGGL(k_serve_loop<16>, dim3((unsigned)grid), dim3(BT), batch_lds_bytes(16), s, slots, done,
+                       tail, head, cap, idle_ticks, life_ticks);
+    MELA_HIP(hipGetLastError());
+    return MELA_OK;
+}
 
 }  // namespace mela
 

@@ -111,6 +111,11 @@ inline void prof_mark(mela_ctx *ctx, bool start) {
 // Returns a device buffer of at least `bytes` for `slot` (kept across calls).
 mela_status scratch_get(mela_ctx *ctx, int slot, size_t bytes, void **out);
 
+// Device bytes the context's scratch cache holds, and its release (waits for
+// the context stream; drops captured eliminations that point into it).
+size_t ctx_scratch_bytes(mela_ctx *ctx);
+void ctx_release_scratch(mela_ctx *ctx);
+
 // Returns the context's host staging buffer (pinned, or ordinary host memory
 // when no pinned memory can be had), at least `bytes` long, once the context
 // stream is idle (no earlier copy still reads or writes it).
@@ -143,6 +148,36 @@ __device__ __forceinline__ bool gate_skip(const Gate &g, bool sparse_path) {
     for (int i = 0; i < GATE_SLOTS; ++i) t += g.ones[i];
     return (t < g.thr) != sparse_path;
 }
+
+// Device-resident serving loop (serve.cpp, kernel in batch.hip).  The slots,
+// the done words and the tail live in pinned host memory, which the kernel
+// reads and writes with plain system-scope loads and stores (no PCIe
+// read-modify-write); the take counter `head` lives in device memory.  The
+// host fills a slot, then publishes it by raising tail; a workgroup takes
+// slot `head` by compare-and-swap only while head < tail, eliminates the
+// matrix, and stores seq + 1 into the slot's done word once its rank and
+// null basis are visible system-wide.  Tail's top bit asks every workgroup
+// to end.
+struct ServeSlot {
+    const uint64_t *A;  // device input, m x lda words
+    uint64_t *work;     // device work: m x wc working rows, then min(m,n) x wc pivot rows
+    uint64_t *Z;        // device null basis (n x wc words), null for rank requests
+    uint32_t m, n, lda, op;  // op: MELA_OP_RANK or MELA_OP_NULL
+    uint32_t mode, pad;      // 0 GE, 1 M4RI sweep
+};
+struct ServeDone {
+    uint32_t rank;
+    uint32_t pad;
+    uint64_t seq1;  // sequence number + 1 once finished
+};
+constexpr uint64_t SERVE_STOP = 1ull << 63;  // in tail: end the loop
+// Workgroups of one serving-loop launch and the launcher: idle_ticks of the
+// 100 MHz wall clock without a published slot, or a launch life_ticks old,
+// end a workgroup between requests.
+int serve_loop_grid(mela_ctx *ctx);
+mela_status launch_serve_loop(hipStream_t s, int grid, ServeSlot *slots, ServeDone *done,
+                              const uint64_t *tail, uint64_t *head, uint32_t cap, uint64_t idle_ticks,
+                              uint64_t life_ticks);
 
 // ------------------------------------------------- kernel launchers ----
 // pack.hip

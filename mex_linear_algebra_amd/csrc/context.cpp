@@ -59,6 +59,29 @@ mela_status scratch_get(mela_ctx *ctx, int slot, size_t bytes, void **out) {
     return MELA_OK;
 }
 
+size_t ctx_scratch_bytes(mela_ctx *ctx) {
+    CtxLock lk(ctx->mu);
+    size_t b = 0;
+    for (const auto &sc : ctx->scratch) b += sc.bytes;
+    return b;
+}
+
+void ctx_release_scratch(mela_ctx *ctx) {
+    CtxLock lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &g : ctx->elim_graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
+    ctx->elim_graphs.clear();
+    for (auto &sc : ctx->scratch) {
+        if (sc.ptr) (void)hipFree(sc.ptr);
+        sc.ptr = nullptr;
+        sc.bytes = 0;
+    }
+}
+
 void pinned_release(mela_ctx *ctx) {
     if (ctx->pin) {
         if (ctx->pin_is_malloc)

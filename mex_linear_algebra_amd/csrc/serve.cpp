@@ -1,20 +1,31 @@
-// Serving loop: continuous batching of mixed rank / null-space / matmul
-// requests under an HBM budget (no reference counterpart; the reference
-// serves one synchronous MEX call at a time).
+// Serving: continuous batching of mixed rank / null-space / matmul requests
+// under an HBM budget (no reference counterpart; the reference serves one
+// synchronous MEX call at a time, e.g. mela_rank_m4ri.c:286,
+// mela_null_gf2.c:142, mela_matmul_gf2.c:72).
 //
-// submit() admits nothing by itself: it queues the request (rejecting one
-// that could never fit the budget).  A scheduler thread admits queued
-// requests in arrival order while their device-memory estimate fits the
-// budget, groups the admitted rank and null requests into micro-batches for
-// the persistent batch kernel (up to max_batch each), and hands jobs to
-// worker threads.  Each worker owns a context (its own HIP stream), so jobs
-// run concurrently on the device; requests that arrive while others run are
-// admitted as memory and workers free up.  Results stay on the device until
-// the client releases them.
+// submit() queues a request (rejecting one that could never fit the budget).
+// A scheduler thread admits queued requests in arrival order while the
+// device memory they need fits the budget, and routes each one:
+//  * rank and null requests that one workgroup finishes quickly go to the
+//    device-resident loop: a persistent kernel (batch.hip, k_serve_loop)
+//    whose workgroups take request slots from a ring in pinned host memory
+//    as the scheduler publishes them, so requests that arrive while others
+//    are on the GPU start as soon as a workgroup is free, with no launch per
+//    request or per batch.  A poller thread sees the completions and starts
+//    the loop again when new work arrives after it went idle.
+//  * larger rank / null requests and all products go to worker threads, each
+//    with its own context and stream (the single-matrix entry points, which
+//    spread one request over the whole device).
+// Accounting (all device bytes the server holds, which admission keeps at or
+// below the budget): the loop's buffer pool (blocks in use and cached), each
+// worker context's scratch cache, outputs waiting for release, and for every
+// running worker job an upper bound of the scratch and output it may add.
+// Device memory is freed outside the server lock.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstring>
 #include <deque>
 #include <map>
 #include <thread>
@@ -27,43 +38,57 @@ using Clock = std::chrono::steady_clock;
 
 namespace {
 
-constexpr size_t kBatchMaxM = 4096, kBatchMaxN = 8192;
+constexpr size_t kLoopMaxM = 4096, kLoopMaxN = 8192;
+constexpr uint32_t kRingCap = 4096;           // request slots of the device loop
+constexpr uint64_t kLoopIdleTicks = 50000;   // 0.5 ms (100 MHz clock) without work ends the loop
+constexpr uint64_t kLoopLifeTicks = 100000;   // a launch takes no new request after 1 ms
+constexpr size_t kPoolMinBlock = (size_t)64 << 10;
 
-struct Req {
-    uint64_t id = 0;
-    mela_request r{};
-    size_t est = 0;        // admission estimate (bytes)
-    size_t out_bytes = 0;  // bytes of the result kept until release
-    Clock::time_point t_submit;
-    enum State { PENDING, ADMITTED, DONE } state = PENDING;
-    bool cancelled = false;
-    mela_result res{};
-    std::string err;
-};
+size_t pool_class(size_t bytes) {
+    size_t c = kPoolMinBlock;
+    while (c < bytes) c <<= 1;
+    return c;
+}
 
-struct Job {
-    enum Kind { BATCH_RANK, BATCH_NULL, SINGLE } kind;
-    std::vector<uint64_t> ids;
-};
+size_t ld16(size_t n) { return (words64(n) + 15) & ~(size_t)15; }
 
-size_t est_bytes(const mela_request &r) {
-    const size_t m = r.A.rows, n = r.A.cols, wc = words64(n) ? words64(n) : 1;
+// Upper bound of the device bytes a worker job adds while it runs: scratch
+// the library may grow (working copy, pivot rows, metadata and active list
+// of an elimination; re-laid, expanded or transposed operands of a product)
+// plus the output it allocates.
+size_t est_single(const mela_request &r) {
+    const size_t m = r.A.rows, n = r.A.cols;
     if (r.op == MELA_OP_MATMUL) {
-        const size_t nn = r.B.cols, k = r.B.rows;
-        return m * (words64(nn) ? words64(nn) : 1) * 8 + nn * (words64(k) ? words64(k) : 1) * 8 + 4096;
+        const size_t k = r.A.cols, nn = r.B.cols;
+        const size_t mp = (m + 255) / 256 * 256, np = (nn + 255) / 256 * 256;
+        const size_t out = std::max<size_t>(m, 1) * std::max<size_t>(words64(nn), 1) * 8;
+        const size_t tiled = (mp + np) * ((k + 1023) / 1024 * 1024) / 8;      // fp4, packed planes
+        const size_t expanded = (mp + np) * ((k + 511) / 512 * 512) / 2;      // fp4, small grids
+        const size_t bt = nn * std::max<size_t>(words64(k), 1) * 8;           // AND+parity
+        return out + std::max(tiled, std::max(expanded, bt)) + ((size_t)1 << 20);
     }
-    size_t b = m * wc * 8 + std::min(m, n) * wc * 8 + 4096;
-    if (r.op == MELA_OP_NULL) b += n * wc * 8;
+    const size_t rmax = std::min(m, n);
+    size_t b = (m + rmax) * ld16(n) * 8 + m * 4 + rmax * 4 + ((size_t)64 << 10);
+    if (r.op == MELA_OP_NULL) b += n * std::max<size_t>(words64(n), 1) * 8 + n * 8 + words64(n) * 80;
     return b;
 }
 
-// Time (ms) that one workgroup of the persistent batch kernel takes over
-// one matrix, end to end (M4RI mode), fitted to single-matrix batches on
-// MI355X over 128..4096 rows x 128..8192 columns (within 35 %): a fixed
-// cost, a cost per 64-pivot panel, per panel and row (the selection scans
-// the active rows), per row, pivot and column word (the sweep) and per word
-// of the null basis written.
-double batch_member_ms(const mela_request &r) {
+// Device-loop buffers of a rank / null request: the working rows and pivot
+// rows, and the null basis (n rows of ceil(n/64) words, the widest it can be).
+size_t loop_work_bytes(const mela_request &r) {
+    const size_t m = r.A.rows, n = r.A.cols, wc = std::max<size_t>(words64(n), 1);
+    return (m + std::min(m, n)) * wc * 8 + 8;
+}
+size_t loop_z_bytes(const mela_request &r) {
+    return r.op == MELA_OP_NULL ? r.A.cols * std::max<size_t>(words64(r.A.cols), 1) * 8 + 8 : 0;
+}
+
+// Time (ms) that one workgroup of the batch machinery takes over one matrix,
+// end to end (M4RI mode), fitted to single-matrix batches on MI355X over
+// 128..4096 rows x 128..8192 columns (within 35 %): a fixed cost, a cost per
+// 64-pivot panel, per panel and row (the selection scans the active rows),
+// per row, pivot and column word (the sweep) and per word of the null basis.
+double loop_member_ms(const mela_request &r) {
     const double m = (double)r.A.rows, n = (double)r.A.cols;
     const double rk = std::min(m, n), wc = (double)words64(r.A.cols);
     const double p = std::ceil(rk / 64.0);
@@ -72,17 +97,33 @@ double batch_member_ms(const mela_request &r) {
     return 0.113 + 0.0054 * p + 0.0155e-3 * m * p + 0.0117e-6 * m * rk * wc + 0.1005e-6 * n * wc;
 }
 
-// A batch launch lasts as long as its slowest member, so a matrix joins a
-// micro-batch only while one workgroup finishes it about as fast as the
-// single-matrix path, which spreads each panel over the whole device
-// (0.06-0.4 ms for these small shapes, 1.4-2 ms at 4096 x 8192 where one
+// A request goes to the device loop while one workgroup finishes it about
+// as fast as the single-matrix path, which spreads each panel over the whole
+// device (0.06-0.4 ms for small shapes, 1.4-2 ms at 4096 x 8192 where one
 // workgroup would take 28-52 ms).
-constexpr double kBatchMemberMaxMs = 0.35;
+constexpr double kLoopMemberMaxMs = 0.35;
 
-bool batchable(const mela_request &r) {
-    return r.op != MELA_OP_MATMUL && r.A.rows <= kBatchMaxM && r.A.cols <= kBatchMaxN &&
-           batch_member_ms(r) <= kBatchMemberMaxMs;
+bool loop_eligible(const mela_request &r) {
+    return r.op != MELA_OP_MATMUL && r.A.rows && r.A.cols && r.A.rows <= kLoopMaxM &&
+           r.A.cols <= kLoopMaxN && loop_member_ms(r) <= kLoopMemberMaxMs;
 }
+
+struct Req {
+    uint64_t id = 0;
+    mela_request r{};
+    bool loop = false;      // device loop (else a worker job)
+    size_t reserve = 0;     // worker job: bytes reserved while it runs
+    size_t out_bytes = 0;   // worker output held until release
+    void *work = nullptr;   // loop: pool blocks (work rows, null basis)
+    void *z = nullptr;
+    size_t work_cls = 0, z_cls = 0;
+    uint64_t seq = 0;       // loop slot sequence number
+    Clock::time_point t_submit;
+    enum State { PENDING, ADMITTED, DONE } state = PENDING;
+    bool cancelled = false;
+    mela_result res{};
+    std::string err;
+};
 
 }  // namespace
 
@@ -93,36 +134,65 @@ struct mela_server {
     std::mutex mu;
     std::condition_variable cv_sched, cv_jobs, cv_done;
     std::deque<uint64_t> pending;
-    std::deque<Job> jobs;
+    std::deque<uint64_t> jobs;  // admitted worker jobs
     std::map<uint64_t, Req> reqs;
     uint64_t next_id = 1;
-    size_t in_use = 0, peak = 0;
     bool stop = false;
-    int busy_workers = 0;
-    int nworkers = 0;
-    std::thread sched;
+    std::thread sched, poller;
     std::vector<std::thread> workers;
+    int nworkers = 0;
+    // accounting (bytes)
+    size_t pool_held = 0;       // every pool block, in use or cached
+    size_t reserved = 0;        // running worker jobs' upper bounds
+    size_t outputs = 0;         // worker outputs waiting for release
+    std::vector<size_t> worker_cache;  // scratch held by each worker context
+    std::vector<char> worker_idle, worker_drop;
+    size_t peak = 0;
+    std::map<size_t, std::vector<void *>> pool_free;  // cached blocks by size class
+    // device loop
+    mela_ctx *loop_ctx = nullptr;  // stream the loop kernel runs on
+    std::string loop_err;
+    ServeSlot *slots = nullptr;
+    ServeDone *done = nullptr;
+    uint64_t *tail = nullptr;    // pinned host: slots published
+    uint64_t *d_head = nullptr;  // device: slots taken
+    uint64_t published = 0;
+    std::vector<uint64_t> inflight;  // ids of loop requests on the device
+    hipEvent_t loop_ev = nullptr;
+    bool loop_launched = false;
+    int loop_grid = 256;
     // stats
     uint64_t n_submitted = 0, n_completed = 0, n_rejected = 0, n_cancelled = 0, n_failed = 0;
     std::vector<double> lat_ms;
-    uint64_t n_jobs = 0, n_batched_reqs = 0;
+    uint64_t n_jobs = 0, n_loop_reqs = 0, n_loop_launches = 0;
+
+    size_t held() const {
+        size_t h = pool_held + reserved + outputs;
+        for (size_t c : worker_cache) h += c;
+        return h;
+    }
+    void note_peak() { peak = std::max(peak, held()); }
 };
 
 namespace {
 
-void finish(mela_server *s, Req &q, mela_status st, const std::string &err) {
-    // caller holds s->mu
+// caller holds s->mu; device pointers to free once it is released
+void finish(mela_server *s, Req &q, mela_status st, const std::string &err,
+            std::vector<void *> &to_free) {
     q.state = Req::DONE;
     q.res.status = st;
     q.err = err;
-    q.res.latency_ms =
-        std::chrono::duration<double, std::milli>(Clock::now() - q.t_submit).count();
-    s->in_use -= (q.est - (st == MELA_OK ? q.out_bytes : 0));
-    if (st != MELA_OK) q.out_bytes = 0;
-    if (q.cancelled && q.res.out.data) {  // nobody will read it
-        (void)hipFree(q.res.out.data);
+    q.res.latency_ms = std::chrono::duration<double, std::milli>(Clock::now() - q.t_submit).count();
+    if (q.cancelled || st != MELA_OK) {
+        // nobody will read the output
+        if (q.loop && q.z) {
+            s->pool_free[q.z_cls].push_back(q.z);
+            q.z = nullptr;
+        } else if (!q.loop && q.res.out.data) {
+            to_free.push_back(q.res.out.data);
+            s->outputs -= q.out_bytes;
+        }
         q.res.out.data = nullptr;
-        s->in_use -= q.out_bytes;
         q.out_bytes = 0;
     }
     if (q.cancelled) {
@@ -137,40 +207,233 @@ void finish(mela_server *s, Req &q, mela_status st, const std::string &err) {
     s->cv_sched.notify_all();
 }
 
+void free_all(const std::vector<void *> &ptrs, int device) {
+    if (ptrs.empty()) return;
+    (void)hipSetDevice(device);
+    for (void *p : ptrs) (void)hipFree(p);
+}
+
+// caller holds s->mu: starts the loop kernel when requests are on the
+// device and no instance is running (a finished instance's event is
+// complete).  A workgroup ends only while it holds no request, so requests
+// still unreported after the kernel ended were never taken.
+void ensure_loop(mela_server *s) {
+    if (!s->loop_ctx || s->inflight.empty()) return;
+    if (s->loop_launched) {
+        const hipError_t q = hipEventQuery(s->loop_ev);
+        if (q == hipErrorNotReady) return;  // still running: it will take the slots
+        (void)hipGetLastError();
+    }
+    (void)hipSetDevice(s->device);
+    hipStream_t st = (hipStream_t)mela_ctx_stream(s->loop_ctx);
+    if (launch_serve_loop(st, s->loop_grid, s->slots, s->done, s->tail, s->d_head, kRingCap, kLoopIdleTicks,
+                          kLoopLifeTicks) != MELA_OK ||
+        hipEventRecord(s->loop_ev, st) != hipSuccess) {
+        (void)hipGetLastError();
+        s->loop_err = mela_last_error();
+        return;
+    }
+    s->loop_launched = true;
+    ++s->n_loop_launches;
+}
+
+// caller holds lk on s->mu.  Blocks of sizes cw (and cz) from the pool, or
+// fresh ones if the budget allows their growth; false if neither (nothing
+// taken).  hipMalloc runs with the lock released.
+bool pool_take(mela_server *s, std::unique_lock<std::mutex> &lk, size_t cw, size_t cz, void **pw,
+               void **pz) {
+    auto &fw = s->pool_free[cw];
+    size_t grow;
+    if (cz == cw) {  // both blocks of one class
+        const size_t have = fw.size();
+        grow = (have >= 2 ? 0 : 2 - have) * cw;
+    } else {
+        grow = (fw.empty() ? cw : 0) + (cz && s->pool_free[cz].empty() ? cz : 0);
+    }
+    if (s->held() + grow > s->budget) return false;
+    *pw = *pz = nullptr;
+    if (!fw.empty()) {
+        *pw = fw.back();
+        fw.pop_back();
+    }
+    if (cz) {
+        auto &fz = s->pool_free[cz];
+        if (!fz.empty()) {
+            *pz = fz.back();
+            fz.pop_back();
+        }
+    }
+    const size_t need = (*pw ? 0 : cw) + (cz && !*pz ? cz : 0);
+    if (need == 0) return true;
+    s->pool_held += need;  // reserved before the allocation
+    s->note_peak();
+    lk.unlock();
+    (void)hipSetDevice(s->device);
+    bool ok = true;
+    if (!*pw && hipMalloc(pw, cw) != hipSuccess) {
+        *pw = nullptr;
+        ok = false;
+    }
+    if (ok && cz && !*pz && hipMalloc(pz, cz) != hipSuccess) {
+        *pz = nullptr;
+        ok = false;
+    }
+    if (!ok) (void)hipGetLastError();
+    lk.lock();
+    if (!ok) {
+        // give back what was reserved; keep what was allocated in the cache
+        s->pool_held -= need;
+        if (*pw) {
+            s->pool_held += cw;
+            s->pool_free[cw].push_back(*pw);
+        }
+        if (*pz) {
+            s->pool_held += cz;
+            s->pool_free[cz].push_back(*pz);
+        }
+        *pw = *pz = nullptr;
+    }
+    return ok;
+}
+
+// caller holds lk.  Frees cached pool blocks (outside the lock) and asks
+// idle workers to drop their scratch caches; true if anything was freed.
+bool reclaim(mela_server *s, std::unique_lock<std::mutex> &lk) {
+    std::vector<void *> blocks;
+    for (auto &kv : s->pool_free) {
+        for (void *p : kv.second) {
+            blocks.push_back(p);
+            s->pool_held -= kv.first;
+        }
+        kv.second.clear();
+    }
+    bool asked = false;
+    for (int w = 0; w < s->nworkers; ++w)
+        if (s->worker_cache[(size_t)w] && s->worker_idle[(size_t)w] && !s->worker_drop[(size_t)w]) {
+            s->worker_drop[(size_t)w] = 1;
+            asked = true;
+        }
+    if (asked) s->cv_jobs.notify_all();
+    if (blocks.empty()) return false;
+    lk.unlock();
+    free_all(blocks, s->device);
+    lk.lock();
+    return true;
+}
+
+// caller holds s->mu: a slot may be written once the request that used it
+// kRingCap sequence numbers earlier has been reported
+bool ring_has_room(const mela_server *s) {
+    if (s->inflight.size() >= (size_t)std::min<int>(s->max_batch, (int)kRingCap)) return false;
+    uint64_t oldest = s->published;
+    for (uint64_t id : s->inflight) oldest = std::min(oldest, s->reqs.at(id).seq);
+    return s->published - oldest < kRingCap;
+}
+
 void scheduler_main(mela_server *s) {
     std::unique_lock<std::mutex> lk(s->mu);
     while (!s->stop) {
-        // admit in arrival order while the estimate fits the budget
-        std::vector<uint64_t> admitted;
-        while (!s->pending.empty()) {
-            Req &q = s->reqs[s->pending.front()];
-            if (s->in_use + q.est > s->budget) break;
-            s->pending.pop_front();
-            q.state = Req::ADMITTED;
-            s->in_use += q.est;
-            s->peak = std::max(s->peak, s->in_use);
-            admitted.push_back(q.id);
-        }
-        if (!admitted.empty()) {
-            Job br{Job::BATCH_RANK, {}}, bn{Job::BATCH_NULL, {}};
-            for (uint64_t id : admitted) {
-                const mela_request &r = s->reqs[id].r;
-                if (batchable(r)) {
-                    Job &j = r.op == MELA_OP_NULL ? bn : br;
-                    j.ids.push_back(id);
-                    if ((int)j.ids.size() >= s->max_batch) {
-                        s->jobs.push_back(j);
-                        j.ids.clear();
-                    }
-                } else {
-                    s->jobs.push_back(Job{Job::SINGLE, {id}});
-                }
+        while (!s->pending.empty() && !s->stop) {
+            const uint64_t id = s->pending.front();
+            Req &q = s->reqs[id];
+            if (q.loop && !s->loop_ctx) {
+                // no device for the loop: the request fails with its error
+                s->pending.pop_front();
+                q.state = Req::ADMITTED;
+                ++s->n_loop_reqs;
+                std::vector<void *> none;
+                finish(s, q, MELA_ERR_DEVICE, s->loop_err, none);
+                continue;
             }
-            if (!br.ids.empty()) s->jobs.push_back(br);
-            if (!bn.ids.empty()) s->jobs.push_back(bn);
-            s->cv_jobs.notify_all();
+            if (q.loop) {
+                if (!ring_has_room(s)) break;  // the poller's completions wake us
+                const size_t cw = pool_class(loop_work_bytes(q.r));
+                const size_t cz = q.r.op == MELA_OP_NULL ? pool_class(loop_z_bytes(q.r)) : 0;
+                void *pw = nullptr, *pz = nullptr;
+                if (!pool_take(s, lk, cw, cz, &pw, &pz)) {
+                    if (reclaim(s, lk)) continue;  // freed cached blocks: try again
+                    break;  // wait for completions and releases
+                }
+                if (s->reqs.count(id) == 0 || s->pending.empty() || s->pending.front() != id) {
+                    // cancelled while the lock was released: the blocks go back
+                    if (pw) s->pool_free[cw].push_back(pw);
+                    if (pz) s->pool_free[cz].push_back(pz);
+                    continue;
+                }
+                s->pending.pop_front();
+                q.work = pw;
+                q.z = pz;
+                q.work_cls = cw;
+                q.z_cls = cz;
+                q.state = Req::ADMITTED;
+                q.seq = s->published;
+                ServeSlot &sl = s->slots[q.seq % kRingCap];
+                sl.A = q.r.A.data;
+                sl.work = (uint64_t *)pw;
+                sl.Z = (uint64_t *)pz;
+                sl.m = (uint32_t)q.r.A.rows;
+                sl.n = (uint32_t)q.r.A.cols;
+                sl.lda = (uint32_t)q.r.A.ld;
+                sl.op = (uint32_t)q.r.op;
+                sl.mode = q.r.variant == MELA_ELIM_GE ? 0u : 1u;
+                s->done[q.seq % kRingCap].seq1 = 0;
+                ++s->published;
+                __atomic_store_n(s->tail, s->published, __ATOMIC_RELEASE);
+                s->inflight.push_back(id);
+                ++s->n_loop_reqs;
+                ensure_loop(s);
+            } else {
+                const size_t est = est_single(q.r);
+                if (s->held() + est > s->budget) {
+                    if (reclaim(s, lk)) continue;
+                    break;
+                }
+                s->pending.pop_front();
+                q.state = Req::ADMITTED;
+                q.reserve = est;
+                s->reserved += est;
+                s->jobs.push_back(id);
+                s->cv_jobs.notify_one();
+            }
+            s->note_peak();
         }
         s->cv_sched.wait(lk);
+    }
+}
+
+// completions of the device loop, and restarts of the loop kernel
+void poller_main(mela_server *s) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        if (s->stop && s->inflight.empty()) break;
+        std::vector<void *> none;
+        bool any = false;
+        for (size_t i = 0; i < s->inflight.size();) {
+            Req &q = s->reqs[s->inflight[i]];
+            const uint64_t seq1 = __atomic_load_n(&s->done[q.seq % kRingCap].seq1, __ATOMIC_ACQUIRE);
+            if (seq1 != q.seq + 1) {
+                ++i;
+                continue;
+            }
+            q.res.rank = s->done[q.seq % kRingCap].rank;
+            if (q.r.op == MELA_OP_NULL) {
+                q.res.out.data = (uint64_t *)q.z;
+                q.res.out.rows = q.r.A.cols;
+                q.res.out.cols = q.r.A.cols - q.res.rank;
+                q.res.out.ld = std::max<size_t>(words64(q.r.A.cols), 1);
+            }
+            s->pool_free[q.work_cls].push_back(q.work);  // the working rows are cached for reuse
+            q.work = nullptr;
+            finish(s, q, MELA_OK, "", none);
+            s->inflight[i] = s->inflight.back();
+            s->inflight.pop_back();
+            any = true;
+        }
+        if (!s->inflight.empty()) ensure_loop(s);
+        if (any) s->cv_sched.notify_all();  // ring slots and pool blocks came free
+        lk.unlock();
+        std::this_thread::sleep_for(std::chrono::microseconds(s->inflight.empty() ? 200 : 10));
+        lk.lock();
     }
 }
 
@@ -189,7 +452,7 @@ mela_status run_single(mela_ctx *ctx, Req &q) {
         if (st != MELA_OK) return st;
         q.res.out = Z;
         q.res.rank = r.A.cols - f;
-        q.out_bytes = Z.rows * Z.ld * 8;
+        q.out_bytes = std::max<size_t>(Z.rows, 1) * Z.ld * 8;
         return MELA_OK;
     }
     mela_bitmat C{};
@@ -202,110 +465,59 @@ mela_status run_single(mela_ctx *ctx, Req &q) {
         return st;
     }
     q.res.out = C;
-    q.out_bytes = C.rows * C.ld * 8;
+    q.out_bytes = std::max<size_t>(C.rows, 1) * C.ld * 8;
     return MELA_OK;
 }
 
-void worker_main(mela_server *s) {
+void worker_main(mela_server *s, int w) {
     mela_ctx *ctx = nullptr;
     mela_status cst = mela_ctx_create(s->device, &ctx);
     std::string cerr = cst == MELA_OK ? "" : mela_last_error();
-    if (ctx) ctx->capture_ok = false;  // other workers issue synchronous HIP calls
+    if (ctx) ctx->capture_ok = false;  // other threads issue synchronous HIP calls
     std::unique_lock<std::mutex> lk(s->mu);
     for (;;) {
-        s->cv_jobs.wait(lk, [&] { return s->stop || !s->jobs.empty(); });
-        if (s->jobs.empty() && s->stop) break;
-        Job job = s->jobs.front();
+        s->worker_idle[(size_t)w] = 1;
+        s->cv_jobs.wait(lk, [&] { return s->stop || !s->jobs.empty() || s->worker_drop[(size_t)w]; });
+        if (s->worker_drop[(size_t)w] && ctx) {
+            // the scheduler needs the memory: drop the scratch cache
+            s->worker_drop[(size_t)w] = 0;
+            lk.unlock();
+            ctx_release_scratch(ctx);
+            lk.lock();
+            s->worker_cache[(size_t)w] = 0;
+            s->cv_sched.notify_all();
+            continue;
+        }
+        if (s->jobs.empty()) {
+            if (s->stop) break;
+            continue;
+        }
+        const uint64_t id = s->jobs.front();
         s->jobs.pop_front();
-        if (job.kind != Job::SINGLE) {
-            // continuous batching: fold the micro-batches of the same kind
-            // that queued up while every worker was busy into this launch
-            for (auto it = s->jobs.begin(); it != s->jobs.end();) {
-                if (it->kind == job.kind && job.ids.size() + it->ids.size() <= (size_t)s->max_batch) {
-                    job.ids.insert(job.ids.end(), it->ids.begin(), it->ids.end());
-                    it = s->jobs.erase(it);
-                } else {
-                    ++it;
-                }
-            }
-        }
-        ++s->busy_workers;
+        s->worker_idle[(size_t)w] = 0;
         ++s->n_jobs;
-        // snapshot the requests (the map entries stay put while admitted)
-        std::vector<Req *> rq;
-        for (uint64_t id : job.ids) rq.push_back(&s->reqs[id]);
+        Req &q = s->reqs[id];  // map entries stay put while admitted
         lk.unlock();
-
-        std::vector<mela_status> sts(rq.size(), MELA_OK);
-        std::vector<std::string> errs(rq.size());
-        if (cst != MELA_OK) {
-            for (size_t i = 0; i < rq.size(); ++i) {
-                sts[i] = cst;
-                errs[i] = cerr;
-            }
-        } else if (job.kind == Job::SINGLE) {
-            sts[0] = run_single(ctx, *rq[0]);
-            if (sts[0] != MELA_OK) errs[0] = mela_last_error();
-        } else {
-            std::vector<mela_bitmat> mats;
-            for (Req *q : rq) mats.push_back(q->r.A);
-            const bool want_null = job.kind == Job::BATCH_NULL;
-            mela_batch *b = nullptr;
-            mela_status st = mela_batch_create(
-                ctx, mats.data(), mats.size(),
-                (want_null ? MELA_BATCH_NULL : 0) | MELA_BATCH_CTX_SCRATCH, &b);
-            if (st == MELA_OK) st = mela_batch_run(b);
-            std::vector<uint32_t> ranks(rq.size());
-            std::vector<mela_bitmat> nulls(rq.size());
-            if (st == MELA_OK) st = mela_batch_results(b, ranks.data(), nulls.data());
-            // copy each null basis out of the batch arena (issued together,
-            // one synchronisation for the whole batch)
-            std::vector<mela_bitmat> outs(rq.size());
-            for (size_t i = 0; i < rq.size() && st == MELA_OK; ++i) {
-                rq[i]->res.rank = ranks[i];
-                if (!want_null) continue;
-                mela_status s2 = mela_bitmat_alloc(ctx, nulls[i].rows, nulls[i].cols, &outs[i]);
-                const mela_bitmat &Z = outs[i];
-                if (s2 == MELA_OK && Z.rows && nulls[i].cols) {
-                    // the batch view has the row pitch of the input width
-                    hipError_t e = hipMemcpy2DAsync(
-                        Z.data, Z.ld * 8, nulls[i].data, nulls[i].ld * 8, Z.ld * 8, Z.rows,
-                        hipMemcpyDeviceToDevice, (hipStream_t)mela_ctx_stream(ctx));
-                    if (e != hipSuccess) s2 = hip_fail(e, "hipMemcpy2DAsync");
-                }
-                if (s2 != MELA_OK) {
-                    sts[i] = s2;
-                    errs[i] = mela_last_error();
-                }
-            }
-            if (st == MELA_OK && want_null) {
-                mela_status s3 = mela_ctx_sync(ctx);
-                const std::string e3 = s3 == MELA_OK ? "" : mela_last_error();
-                for (size_t i = 0; i < rq.size(); ++i) {
-                    if (s3 != MELA_OK && sts[i] == MELA_OK) {
-                        sts[i] = s3;
-                        errs[i] = e3;
-                    }
-                    if (sts[i] != MELA_OK) {
-                        if (outs[i].data) mela_bitmat_free(ctx, &outs[i]);
-                        continue;
-                    }
-                    rq[i]->res.out = outs[i];
-                    rq[i]->out_bytes = outs[i].rows * outs[i].ld * 8;
-                }
-            }
-            if (st != MELA_OK)
-                for (size_t i = 0; i < rq.size(); ++i) {
-                    sts[i] = st;
-                    errs[i] = mela_last_error();
-                }
-            if (b) mela_batch_destroy(b);
+        mela_status st = cst;
+        std::string err = cerr;
+        if (cst == MELA_OK) {
+            st = run_single(ctx, q);
+            if (st != MELA_OK) err = mela_last_error();
         }
-
+        const size_t cache = ctx ? ctx_scratch_bytes(ctx) : 0;
         lk.lock();
-        --s->busy_workers;
-        if (job.kind != Job::SINGLE) s->n_batched_reqs += rq.size();
-        for (size_t i = 0; i < rq.size(); ++i) finish(s, *rq[i], sts[i], errs[i]);
+        s->reserved -= q.reserve;
+        q.reserve = 0;
+        s->worker_cache[(size_t)w] = cache;
+        if (st == MELA_OK) s->outputs += q.out_bytes;
+        std::vector<void *> to_free;
+        finish(s, q, st, err, to_free);
+        s->note_peak();
+        if (!to_free.empty()) {
+            lk.unlock();
+            free_all(to_free, s->device);
+            lk.lock();
+        }
     }
     lk.unlock();
     if (ctx) mela_ctx_destroy(ctx);
@@ -318,6 +530,21 @@ double percentile(std::vector<double> v, double p) {
     const size_t lo = (size_t)pos;
     const size_t hi = std::min(lo + 1, v.size() - 1);
     return v[lo] + (v[hi] - v[lo]) * (pos - (double)lo);
+}
+
+void loop_shutdown(mela_server *s) {
+    if (s->tail) __atomic_store_n(s->tail, s->published | SERVE_STOP, __ATOMIC_RELEASE);
+    if (s->loop_launched) (void)hipEventSynchronize(s->loop_ev);
+    if (s->loop_ev) (void)hipEventDestroy(s->loop_ev);
+    if (s->loop_ctx) mela_ctx_destroy(s->loop_ctx);
+    for (void *p : {(void *)s->slots, (void *)s->done, (void *)s->tail})
+        if (p) (void)hipHostFree(p);
+    if (s->d_head) (void)hipFree(s->d_head);
+    s->loop_ctx = nullptr;
+    s->slots = nullptr;
+    s->done = nullptr;
+    s->tail = nullptr;
+    s->d_head = nullptr;
 }
 
 }  // namespace
@@ -333,8 +560,43 @@ mela_status mela_server_create(int device, size_t hbm_budget, int workers, int m
     s->budget = hbm_budget;
     s->max_batch = max_batch;
     s->nworkers = workers;
+    s->worker_cache.assign((size_t)workers, 0);
+    s->worker_idle.assign((size_t)workers, 1);
+    s->worker_drop.assign((size_t)workers, 0);
+    // the device loop: its stream, and the ring in pinned (coherent) host memory
+    mela_status st = mela_ctx_create(device, &s->loop_ctx);
+    if (st == MELA_OK) {
+        s->loop_ctx->capture_ok = false;
+        s->loop_grid = serve_loop_grid(s->loop_ctx);
+        const unsigned fl = hipHostMallocCoherent | hipHostMallocMapped;
+        if (hipHostMalloc((void **)&s->slots, kRingCap * sizeof(ServeSlot), fl) != hipSuccess ||
+            hipHostMalloc((void **)&s->done, kRingCap * sizeof(ServeDone), fl) != hipSuccess ||
+            hipHostMalloc((void **)&s->tail, 64, fl) != hipSuccess ||
+            hipMalloc((void **)&s->d_head, sizeof(uint64_t)) != hipSuccess ||
+            hipEventCreateWithFlags(&s->loop_ev, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            loop_shutdown(s);
+            delete s;
+            return set_error(MELA_ERR_NOMEM, "pinned memory for the serving loop", "mela:memory");
+        }
+        std::memset(s->slots, 0, kRingCap * sizeof(ServeSlot));
+        std::memset(s->done, 0, kRingCap * sizeof(ServeDone));
+        std::memset(s->tail, 0, 64);
+        const uint64_t zero = 0;
+        if (hipMemcpy(s->d_head, &zero, sizeof(zero), hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipGetLastError();
+            loop_shutdown(s);
+            delete s;
+            return set_error(MELA_ERR_DEVICE, "serving loop set-up", "mela:device");
+        }
+    } else {
+        // no device: every request fails with the context's error
+        s->loop_err = mela_last_error();
+        s->loop_ctx = nullptr;
+    }
     s->sched = std::thread(scheduler_main, s);
-    for (int i = 0; i < workers; ++i) s->workers.emplace_back(worker_main, s);
+    s->poller = std::thread(poller_main, s);
+    for (int i = 0; i < workers; ++i) s->workers.emplace_back(worker_main, s, i);
     *out = s;
     return MELA_OK;
 }
@@ -344,15 +606,22 @@ mela_status mela_server_submit(mela_server *s, const mela_request *req, uint64_t
     const mela_request &r = *req;
     if (r.op < MELA_OP_RANK || r.op > MELA_OP_MATMUL)
         return set_error(MELA_ERR_INVALID, "unknown request op");
-    if (r.op == MELA_OP_MATMUL && r.A.cols != r.B.rows)
-        return set_error(MELA_ERR_DIM, "Inner dimensions must agree.");
+    if ((r.A.rows && r.A.cols && !r.A.data) || (r.A.cols && r.A.ld < words64(r.A.cols)))
+        return set_error(MELA_ERR_INVALID, "bad input descriptor");
+    if (r.op == MELA_OP_MATMUL) {
+        if (r.A.cols != r.B.rows) return set_error(MELA_ERR_DIM, "Inner dimensions must agree.");
+        if ((r.B.rows && r.B.cols && !r.B.data) || (r.B.cols && r.B.ld < words64(r.B.cols)))
+            return set_error(MELA_ERR_INVALID, "bad input descriptor");
+    }
     Req q;
     q.r = r;
-    q.est = est_bytes(r);
+    q.loop = loop_eligible(r);
     q.t_submit = Clock::now();
+    const size_t need = q.loop ? pool_class(loop_work_bytes(r)) + (r.op == MELA_OP_NULL ? pool_class(loop_z_bytes(r)) : 0)
+                               : est_single(r);
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) return set_error(MELA_ERR_CANCELLED, "server is shutting down");
-    if (q.est > s->budget) {
+    if (need > s->budget) {
         ++s->n_rejected;
         return set_error(MELA_ERR_ADMISSION, "request needs more device memory than the server budget");
     }
@@ -403,24 +672,23 @@ mela_status mela_server_cancel(mela_server *s, uint64_t ticket) {
 
 mela_status mela_server_release(mela_server *s, uint64_t ticket) {
     if (!s) return set_error(MELA_ERR_INVALID, "null server");
-    mela_bitmat out{};
-    size_t bytes = 0;
+    void *to_free = nullptr;
     {
         std::lock_guard<std::mutex> lk(s->mu);
         auto it = s->reqs.find(ticket);
         if (it == s->reqs.end()) return set_error(MELA_ERR_INVALID, "unknown ticket");
-        if (it->second.state != Req::DONE) return set_error(MELA_ERR_INVALID, "request still running");
-        out = it->second.res.out;
-        bytes = it->second.out_bytes;
+        Req &q = it->second;
+        if (q.state != Req::DONE) return set_error(MELA_ERR_INVALID, "request still running");
+        if (q.loop) {
+            if (q.z) s->pool_free[q.z_cls].push_back(q.z);  // cached for the next request
+        } else if (q.res.out.data) {
+            to_free = q.res.out.data;
+            s->outputs -= q.out_bytes;
+        }
         s->reqs.erase(it);
+        s->cv_sched.notify_all();
     }
-    if (out.data) {
-        (void)hipSetDevice(s->device);
-        (void)hipFree(out.data);
-    }
-    std::lock_guard<std::mutex> lk(s->mu);
-    s->in_use -= bytes;
-    s->cv_sched.notify_all();
+    if (to_free) free_all({to_free}, s->device);
     return MELA_OK;
 }
 
@@ -437,10 +705,11 @@ mela_status mela_server_stats_get(mela_server *s, mela_server_stats *st) {
     double sum = 0;
     for (double v : s->lat_ms) sum += v;
     st->mean_ms = s->lat_ms.empty() ? 0.0 : sum / (double)s->lat_ms.size();
-    st->hbm_in_use = s->in_use;
+    st->hbm_in_use = s->held();
     st->hbm_peak = s->peak;
     st->jobs = s->n_jobs;
-    st->batched_requests = s->n_batched_reqs;
+    st->batched_requests = s->n_loop_reqs;
+    st->loop_launches = s->n_loop_launches;
     return MELA_OK;
 }
 
@@ -449,8 +718,8 @@ mela_status mela_server_reset_stats(mela_server *s) {
     std::lock_guard<std::mutex> lk(s->mu);
     s->n_submitted = s->n_completed = s->n_rejected = s->n_cancelled = s->n_failed = 0;
     s->lat_ms.clear();
-    s->peak = s->in_use;
-    s->n_jobs = s->n_batched_reqs = 0;
+    s->peak = s->held();
+    s->n_jobs = s->n_loop_reqs = s->n_loop_launches = 0;
     return MELA_OK;
 }
 
@@ -473,9 +742,22 @@ void mela_server_destroy(mela_server *s) {
     }
     s->sched.join();
     for (auto &t : s->workers) t.join();
+    s->poller.join();  // returns once the loop's requests are all reported
+    loop_shutdown(s);
     (void)hipSetDevice(s->device);
-    for (auto &kv : s->reqs)
-        if (kv.second.res.out.data) (void)hipFree(kv.second.res.out.data);
+    std::vector<void *> blocks;
+    for (auto &kv : s->reqs) {
+        Req &q = kv.second;
+        if (q.loop) {
+            if (q.z) blocks.push_back(q.z);
+            if (q.work) blocks.push_back(q.work);
+        } else if (q.res.out.data) {
+            blocks.push_back(q.res.out.data);
+        }
+    }
+    for (auto &kv : s->pool_free)
+        for (void *p : kv.second) blocks.push_back(p);
+    free_all(blocks, s->device);
     delete s;
 }
 

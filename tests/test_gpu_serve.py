@@ -1,7 +1,16 @@
-"""Serving loop on the device: a mix of rank, null-space and matmul requests
-submitted together (so they are admitted, micro-batched and run concurrently
-on several streams) gives, request by request, exactly the result of the same
-request run alone through the single-matrix entry points (tolerance 0)."""
+"""The serving loop on the device, every result checked against the oracle.
+
+A mixed stream (70% rank, 20% null space, 10% products; sides log-uniform
+128..8192 for rank, up to 4096 for null bases and products; dense or column
+weight 3..6) is submitted while earlier requests are still on the GPU, so
+requests join the device-resident loop and the worker streams as they
+arrive.  Ranks equal the oracle's (mela_rank_m4ri.c:94-279 restated), null
+bases the canonical basis built from the oracle's RREF (Z(free,:) = I,
+Z(pivot i,:) = R(i, free), mela_null_gf2.c:88-274), products the oracle's
+product (mela_matmul_gf2.c:110-208); a sample of requests run again alone
+gives the same results; a tight HBM budget is never exceeded."""
+import time
+
 import numpy as np
 import pytest
 
@@ -16,53 +25,180 @@ def mela():
     return m
 
 
-def test_mix_equals_alone(mela):
-    ctx = mela.default_context()
-    rng = np.random.default_rng(123)
-    pool = []
-    for i in range(48):
-        m = int(np.exp(rng.uniform(np.log(64), np.log(1500))))
-        n = int(np.exp(rng.uniform(np.log(64), np.log(1500))))
-        P = W.dense_matrix(rng, m, n) if i % 2 else W.parity_check_matrix(rng, m, n, 3 + i % 4)
-        pool.append(ctx.upload(P, n))
-    big = ctx.bitmat(4500, 5000)  # beyond the batch kernel: single-matrix path
-    ctx.gen_bernoulli(9, big)
-    pool.append(big)
-    mm = []
-    for i in range(6):
-        m, k, n = (int(x) for x in rng.integers(64, 2000, 3))
-        A, B = ctx.bitmat(m, k), ctx.bitmat(k, n)
-        ctx.gen_bernoulli(100 + i, A)
-        ctx.gen_bernoulli(200 + i, B)
-        mm.append((A, B))
-    ctx.sync()
-    srv = mela.Server(hbm_budget=64 << 30, workers=4, max_batch=16)
-    tickets = []
-    for i, M in enumerate(pool):
-        op = mela.OP_NULL if i % 3 == 0 else mela.OP_RANK
-        tickets.append((op, M, None, srv.submit(op, M, variant=i % 2)))
-        if i % 8 == 0 and mm:
-            A, B = mm.pop()
-            tickets.append((mela.OP_MATMUL, A, B, srv.submit(mela.OP_MATMUL, A, B)))
-    for A, B in mm:
-        tickets.append((mela.OP_MATMUL, A, B, srv.submit(mela.OP_MATMUL, A, B)))
-    for op, A, B, t in tickets:
-        res = srv.wait(t, 120.0)
-        assert res.status == 0
-        if op == mela.OP_RANK:
-            assert res.rank == ctx.rank_packed(A)
-        elif op == mela.OP_NULL:
-            Z = ctx.null_packed(A)
-            assert res.out.cols == Z.cols and res.rank == A.cols - Z.cols
-            assert np.array_equal(ctx.download(res.out), ctx.download(Z))
-            ctx.free(Z)
+def _side(rng, lo, hi):
+    return int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+
+
+def _matrix(rng, m, n):
+    if rng.random() < 0.5:
+        return W.dense_matrix(rng, m, n)
+    return W.parity_check_matrix(rng, m, n, int(rng.integers(3, 7)))
+
+
+def _bits(P, cols):
+    P = np.ascontiguousarray(P, dtype=np.uint64)
+    return np.unpackbits(P.view(np.uint8), axis=1, bitorder="little")[:, :cols].astype(bool)
+
+
+def _check_null(orc, P, m, n, rank, Zp):
+    M, piv = orc.rref_packed(P, m, n, "gf2")
+    r = len(piv)
+    assert rank == r
+    f = n - r
+    Zb = _bits(Zp, f)
+    piv = np.asarray(piv, dtype=np.int64)
+    free = np.setdiff1d(np.arange(n), piv)
+    assert np.array_equal(Zb[free], np.eye(f, dtype=bool))
+    assert np.array_equal(Zb[piv], _bits(M[:r], n)[:, free])
+
+
+def _requests(rng, count):
+    out = []
+    for _ in range(count):
+        u = rng.random()
+        if u < 0.7:
+            m, n = _side(rng, 128, 8192), _side(rng, 128, 8192)
+            out.append(("rank", _matrix(rng, m, n), m, n, None, None))
+        elif u < 0.9:
+            m, n = _side(rng, 128, 4096), _side(rng, 128, 4096)
+            out.append(("null", _matrix(rng, m, n), m, n, None, None))
         else:
-            Cm = ctx.bitmat(A.rows, B.cols)
-            ctx.matmul_packed(A, B, Cm)
-            assert np.array_equal(ctx.download(res.out), ctx.download(Cm))
-            ctx.free(Cm)
-        srv.release(t)
-    st = srv.stats()
-    assert st["completed"] == len(tickets) and st["failed"] == 0
-    assert st["batched_requests"] > 0
-    srv.close()
+            m, k, n = _side(rng, 128, 3000), _side(rng, 128, 3000), _side(rng, 128, 3000)
+            out.append(("matmul", _matrix(rng, m, k), m, k, W.dense_matrix(rng, k, n), n))
+    return out
+
+
+def _submit(mela, srv, ctx, req):
+    kind, P, m, n, Bp, nb = req
+    dA = ctx.upload(P, n)
+    if kind == "matmul":
+        dB = ctx.upload(Bp, nb)
+        return srv.submit(mela.OP_MATMUL, dA, dB), (dA, dB)
+    op = mela.OP_RANK if kind == "rank" else mela.OP_NULL
+    return srv.submit(op, dA, variant=mela.ELIM_M4RI), (dA,)
+
+
+def _result(ctx, req, res):
+    kind = req[0]
+    if kind == "rank":
+        return res.rank
+    return res.rank, ctx.download(res.out)
+
+
+def _check(orc, req, got):
+    kind, P, m, n, Bp, nb = req
+    if kind == "rank":
+        assert got == int(orc.rank_packed(P, m, n, "m4ri")), (m, n)
+    elif kind == "null":
+        _check_null(orc, P, m, n, got[0], got[1])
+    else:
+        want = orc.matmul_packed(P, m, n, Bp, nb, "gf2")
+        assert np.array_equal(got[1][:, : want.shape[1]], want), (m, n, nb)
+
+
+def test_mixed_stream_against_oracle(mela, orc):
+    rng = np.random.default_rng(2026)
+    reqs = _requests(rng, 90)
+    ctx = mela.Context(0)
+    srv = mela.Server(hbm_budget=64 << 30, workers=4)
+    inputs, tickets = [], []
+    try:
+        # submitted in a stream: later requests arrive while earlier ones run
+        for i, q in enumerate(reqs):
+            t, dev = _submit(mela, srv, ctx, q)
+            tickets.append(t)
+            inputs.append(dev)
+            if i % 10 == 9:
+                time.sleep(0.002)
+        results = []
+        for q, t in zip(reqs, tickets):
+            res = srv.wait(t, 120.0)
+            assert res.status == 0
+            results.append(_result(ctx, q, res))
+            srv.release(t)
+        st = srv.stats()
+        assert st["completed"] == len(reqs) and st["failed"] == 0
+        assert st["batched_requests"] > 0 and st["jobs"] > 0  # both paths ran
+        print(f"[serve] loop {st['batched_requests']} workers {st['jobs']} "
+              f"launches {st['loop_launches']} p50 {st['p50_ms']:.3f} p99 {st['p99_ms']:.3f}", flush=True)
+        for q, got in zip(reqs, results):
+            _check(orc, q, got)
+        # the same requests alone, one at a time, give the same results
+        for i in range(0, len(reqs), 9):
+            t, dev = _submit(mela, srv, ctx, reqs[i])
+            res = srv.wait(t, 120.0)
+            alone = _result(ctx, reqs[i], res)
+            srv.release(t)
+            for M in dev:
+                ctx.free(M)
+            if reqs[i][0] == "rank":
+                assert alone == results[i]
+            else:
+                assert alone[0] == results[i][0] and np.array_equal(alone[1], results[i][1])
+    finally:
+        srv.close()
+        for dev in inputs:
+            for M in dev:
+                ctx.free(M)
+        ctx.close()
+
+
+def test_tight_budget_is_never_exceeded(mela, orc):
+    rng = np.random.default_rng(77)
+    budget = 48 << 20
+    ctx = mela.Context(0)
+    srv = mela.Server(hbm_budget=budget, workers=2)
+    inputs = []
+    try:
+        reqs = []
+        for i in range(60):
+            m, n = _side(rng, 128, 1500), _side(rng, 128, 2500)
+            reqs.append(("null" if i % 4 == 0 else "rank", _matrix(rng, m, n), m, n, None, None))
+        reqs.append(("rank", W.dense_matrix(rng, 3000, 6000), 3000, 6000, None, None))  # a worker job
+        tickets = []
+        for q in reqs:
+            t, dev = _submit(mela, srv, ctx, q)
+            tickets.append(t)
+            inputs.append(dev)
+        for q, t in zip(reqs, tickets):
+            res = srv.wait(t, 120.0)
+            got = _result(ctx, q, res)
+            srv.release(t)
+            _check(orc, q, got)
+        st = srv.stats()
+        assert 0 < st["hbm_peak"] <= budget, st
+        # one request that could never fit (working and pivot rows of 64 MiB
+        # plus a 32 MiB basis) is refused at submission
+        big = ctx.bitmat(16384, 16384)
+        with pytest.raises(mela.MelaError) as ei:
+            srv.submit(mela.OP_NULL, big)
+        assert ei.value.status == "ADMISSION"
+        ctx.free(big)
+    finally:
+        srv.close()
+        for dev in inputs:
+            for M in dev:
+                ctx.free(M)
+        ctx.close()
+
+
+def test_loop_restarts_after_idle(mela, orc):
+    rng = np.random.default_rng(5)
+    ctx = mela.Context(0)
+    srv = mela.Server(hbm_budget=1 << 30, workers=1)
+    inputs = []
+    try:
+        for round_ in range(3):
+            P = W.dense_matrix(rng, 300, 500)
+            d = ctx.upload(P, 500)
+            inputs.append(d)
+            t = srv.submit(mela.OP_RANK, d)
+            assert srv.wait(t, 30.0).rank == int(orc.rank_packed(P, 300, 500, "m4ri"))
+            srv.release(t)
+            time.sleep(0.05)  # longer than the loop's idle limit and lifetime
+        assert srv.stats()["loop_launches"] >= 2
+    finally:
+        srv.close()
+        for d in inputs:
+            ctx.free(d)
+        ctx.close()

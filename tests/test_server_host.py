@@ -38,8 +38,7 @@ def test_matmul_dimension_error():
 
 
 def test_queue_cancel_and_timeout():
-    # budget admits one 64 KiB request at a time; with no device the worker
-    # fails requests, which frees memory again
+    # with no device every request fails, which frees its memory again
     s = mela.Server(device=NO_DEVICE, hbm_budget=1 << 30, workers=1)
     tickets = [s.submit(mela.OP_RANK, fake(64, 64)) for _ in range(3)]
     for t in tickets:
@@ -58,7 +57,9 @@ def test_queue_cancel_and_timeout():
 
 def test_cancel_pending_before_admission():
     # a budget so small that a second request must queue behind the first
-    one = 64 * 1 * 8 * 2 + 4096
+    # (a 64 x 64 rank job reserves about 82 KiB: working and pivot rows,
+    # metadata and the active list)
+    one = 120 << 10
     s = mela.Server(device=NO_DEVICE, hbm_budget=one, workers=1)
     t1 = s.submit(mela.OP_RANK, fake(64, 64))
     t2 = s.submit(mela.OP_RANK, fake(64, 64))
@@ -77,7 +78,7 @@ def test_cancel_pending_before_admission():
 
 
 def test_wait_timeout_on_blocked_request():
-    one = 64 * 1 * 8 * 2 + 4096
+    one = 120 << 10
     s = mela.Server(device=NO_DEVICE, hbm_budget=one, workers=1)
     t1 = s.submit(mela.OP_RANK, fake(64, 64))
     with pytest.raises(mela.MelaError):
@@ -103,10 +104,10 @@ def test_wait_zero_timeout():
 
 
 def test_routing_by_batch_member_cost():
-    # the scheduler batches a rank / null request only while one workgroup of
-    # the persistent batch kernel finishes it about as fast as the
-    # single-matrix path; the device never exists here, so every request
-    # fails, but the routing is counted all the same
+    # the scheduler sends a rank / null request to the device-resident loop
+    # only while one workgroup finishes it about as fast as the single-matrix
+    # path; the device never exists here, so every request fails, but the
+    # routing is counted all the same
     s = mela.Server(device=NO_DEVICE, hbm_budget=1 << 34, workers=1)
     cases = [
         (mela.OP_RANK, 256, 256, True),     # small: batched
