@@ -6,12 +6,23 @@ step = one packed product of two 32768 x 32768 Bernoulli(1/2) bit matrices
 (128 MiB each, generated on the device and resident in HBM before the timed
 region) through the C ABI (mela_matmul_packed).  Work per step is W = 2*m*n*k
 bit operations (one AND and one XOR per (i,j,k) triple), reported as
-Tbit-op/s.  Prints one JSON line.
+Tbit-op/s.  Prints one JSON line, which also carries (unless --headline-only)
+the rest of the metric: `batched` (4096 uneven rank + null matrices in one
+persistent launch against one null_packed call per matrix), `batched_matmul`
+(2048 uneven products in one persistent launch against one matmul_packed call
+per product), `serve` (p50/p99 of the request server at two offered loads)
+and `c1` (the six entry points end to end on 2048 x 2048 logical arrays, GPU
+and CPU oracle side by side).
 
 Other workloads (--workload): elim (16384-scale rank + null space, GE and
-M4RI), batch (4096 uneven rank + null matrices in one persistent launch vs
-one matrix per launch), copy (a plain copy kernel of known size, used to scale
-HBM byte counters).  Each prints its own JSON line(s).
+M4RI), batch, batch_matmul, serve (a sweep of offered loads), c1, copy (a
+plain copy kernel of known size, used to scale HBM byte counters).  Each
+prints its own JSON line(s).
+
+CPU figures run with OMP_DYNAMIC=FALSE and, unless the environment already
+sets it, OMP_NUM_THREADS = the CPUs this process may run on (the reference
+benchmark's setting, benchmark_mex.m:163-164); the CPU model, CPU counts and
+threads used are recorded with them.
 
 --gpus N>1 is refused: this build measures one GPU.
 """
@@ -24,6 +35,10 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+
+# before any OpenMP runtime is loaded (it reads these once, at load)
+os.environ["OMP_DYNAMIC"] = "FALSE"
+os.environ.setdefault("OMP_NUM_THREADS", str(len(os.sched_getaffinity(0))))
 
 # v_bitop3_b32 retires one AND and one XOR per bit lane: 256 CU x 64 lanes x
 # 2.4 GHz x 32 bits x 2 = 2516 Tbit-op/s of W on the bitwise VALU path.
@@ -61,6 +76,11 @@ class HipEvents:
         return float(ms.value)
 
 
+def progress(msg):
+    """A progress line on stderr (stdout carries only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def reference_timing(fn):
     """The reference benchmark's timing policy (benchmark_mex.m:671-707): one
     warm-up call; double the iteration count until a round takes >= 10 ms;
@@ -89,13 +109,33 @@ def reference_timing(fn):
     return times[len(times) // 2], rounds, iters
 
 
-def cpu_matmul_baseline(m, k, n, budget_s=25.0):
+def host_metadata(orc):
+    """CPU model, CPU counts and the OpenMP settings the CPU figures ran with."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "cpus_allowed": len(os.sched_getaffinity(0)),
+            "threads": orc.num_threads(),
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"),
+            "OMP_DYNAMIC": os.environ.get("OMP_DYNAMIC")}
+
+
+def cpu_matmul_baseline(m, k, n, budget_s=25.0, ge_budget_s=8.0):
     """The oracle's restatement of the reference M4RI product (matmul_m4ri.c:
     178-248, kernel-only on packed operands, OpenMP over 256-row blocks) timed
     on the host cores with the reference's timing policy: the full n^3
     workload when it fits the budget, else an m x n x m sample (m halved until
-    it does)."""
+    it does).  Beside it, the restated AND+popcount product
+    (mela_matmul_gf2.c:110-208) on a row sample sized from one timed call."""
     from oracle import oracle as orc
+    m_full = m
     while True:
         # warm-up + doubling + 2 rounds of one call each ~ 4 calls at this size
         est_s = 2.0 * m * n * k / 40e12
@@ -105,12 +145,29 @@ def cpu_matmul_baseline(m, k, n, budget_s=25.0):
     Ap = orc.gen_bernoulli_packed(1, m, k)
     Bp = orc.gen_bernoulli_packed(2, k, n)
     dt, rounds, iters = reference_timing(lambda: orc.matmul_packed(Ap, m, k, Bp, n, "m4ri"))
-    return {"value": round(2.0 * m * n * k / dt / 1e12, 3), "unit": "Tbitop/s",
-            "cores": orc.num_threads(), "kind": "port",
-            "sample": f"oracle restatement of the reference M4RI product (k=8 tables, 256-row "
-                      f"blocks, OpenMP) on packed {m}x{k} * {k}x{n} Bernoulli(1/2); timing policy "
-                      f"of benchmark_mex.m:671-707 (warm-up, {rounds} rounds x {iters} call(s), "
-                      f"median): {dt:.2f} s per call"}
+    out = {"value": round(2.0 * m * n * k / dt / 1e12, 3), "unit": "Tbitop/s",
+           "cores": orc.num_threads(), "kind": "port",
+           "sample": f"oracle restatement of the reference M4RI product (k=8 tables, 256-row "
+                     f"blocks, OpenMP) on packed {m}x{k} * {k}x{n} Bernoulli(1/2); timing policy "
+                     f"of benchmark_mex.m:671-707 (warm-up, {rounds} rounds x {iters} call(s), "
+                     f"median): {dt:.2f} s per call"}
+    out.update(host_metadata(orc))
+    # GE restatement: time 64 rows once, then the largest power-of-two row
+    # count whose ~4 calls fit ge_budget_s
+    g = 64
+    Ag = orc.gen_bernoulli_packed(3, g, k)
+    t0 = time.perf_counter()
+    orc.matmul_packed(Ag, g, k, Bp, n, "gf2")
+    per_row = (time.perf_counter() - t0) / g
+    while g * 2 <= m_full and 4 * per_row * g * 2 <= ge_budget_s:
+        g *= 2
+    Ag = orc.gen_bernoulli_packed(3, g, k)
+    dg, rounds, iters = reference_timing(lambda: orc.matmul_packed(Ag, g, k, Bp, n, "gf2"))
+    out["ge"] = {"value": round(2.0 * g * n * k / dg / 1e12, 3), "unit": "Tbitop/s",
+                 "sample": f"oracle restatement of the reference AND+popcount product on packed "
+                           f"{g}x{k} * {k}x{n}; {rounds} rounds x {iters} call(s), median "
+                           f"{dg:.3f} s per call"}
+    return out
 
 
 def resolve_auto(m, k, n):
@@ -207,11 +264,25 @@ def bench_matmul(args, mela):
         "call_ms": round(dev_ms, 3),
         "kernel_ms": round(kern_ms, 3),
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_matmul_baseline(m, k, n)
-    print(json.dumps(out), flush=True)
     for M in (A, B, Cm):
         ctx.free(M)
+    progress(f"product {m}x{k}x{n}: {wall_ms:.3f} ms per step")
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_matmul_baseline(m, k, n)
+        progress("cpu baseline done")
+    if args.workload == "matmul" and not args.headline_only:
+        out["batched"] = bench_batch(args, mela)
+        progress("batched rank+null done")
+        out["batched_matmul"] = bench_batch_matmul(args, mela)
+        progress("batched matmul done")
+        saved = (args.serve_rates, args.serve_seconds)
+        args.serve_rates, args.serve_seconds = args.headline_serve_rates, 1.0
+        out["serve"] = bench_serve(args, mela)
+        args.serve_rates, args.serve_seconds = saved
+        progress("serve done")
+        out["c1"] = bench_c1(args, mela)
+        progress("c1 done")
+    print(json.dumps(out), flush=True)
 
 
 def elim_bytes_alg(m, n, pivots, rref):
@@ -307,12 +378,12 @@ def bench_batch(args, mela):
     wall = (time.perf_counter() - t0) / args.steps
     dev_ms = ev.elapsed_ms(e0, e1) / args.steps
     rate = len(items) / wall
-    # plain form: one matrix per call of the single-matrix API (rank + null)
+    # plain form: one elimination per matrix through the single-matrix API
+    # (null_packed gives the rank, n - Z.cols, and the basis)
     sidx = list(range(0, len(items), max(1, len(items) // args.plain_sample)))
     ctx.sync()
     t0 = time.perf_counter()
     for i in sidx:
-        ctx.rank_packed(dev[i])
         Z = ctx.null_packed(dev[i])
         ctx.free(Z)
     ctx.sync()
@@ -326,7 +397,7 @@ def bench_batch(args, mela):
         wc = (n + 63) // 64
         bytes_alg += sum(16 * m * (wc - j) for j in range((r + 63) // 64))
         bytes_alg += n * ((n - r + 63) // 64) * 8
-    print(json.dumps({
+    row = {
         "workload": f"batched rank+null, {len(items)} uneven matrices (m log-uniform 64..2048, "
                     f"n=m+U[0,m], 90% column-weight 3/4/6, 10% dense)",
         "batched_ms": round(wall * 1e3, 3), "batched_device_ms": round(dev_ms, 3),
@@ -336,10 +407,61 @@ def bench_batch(args, mela):
         "generate_s": round(gen_s, 2), "mean_rank": sum(ranks) / len(ranks),
         "bytes_alg_GB": round(bytes_alg / 1e9, 3),
         "GBps_alg": round(bytes_alg / (dev_ms * 1e-3) / 1e9, 1),
-        "hbm_frac": round(bytes_alg / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}), flush=True)
+        "hbm_frac": round(bytes_alg / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     b.close()
     for M in dev:
         ctx.free(M)
+    return row
+
+
+def bench_batch_matmul(args, mela):
+    """2048 uneven products C_i = A_i * B_i (workloads.batch_matmul_items: A_i
+    as in the rank/null batch, B_i dense k x b, b log-uniform 32..1024) in one
+    persistent, graph-captured launch (mela_matmul_batch_run) against one
+    mela_matmul_packed call (AUTO) per product on a sample."""
+    from mex_linear_algebra_amd import workloads as W
+    ctx = mela.Context(0)
+    items = W.batch_matmul_items(args.batch_mm_count, seed=5)
+    dA = [ctx.upload(A, k) for A, m, k, B, n, _ in items]
+    dB = [ctx.upload(B, n) for A, m, k, B, n, _ in items]
+    dC = [ctx.bitmat(m, n) for A, m, k, B, n, _ in items]
+    b = mela.MatmulBatch(ctx, dA, dB, dC)
+    algos = b.algos()
+    for _ in range(max(1, args.warmup)):
+        b.run()
+    ev = HipEvents()
+    e0, e1 = ev.event(), ev.event()
+    ctx.sync()
+    t0 = time.perf_counter()
+    ev.record(e0, ctx.stream)
+    for _ in range(args.steps):
+        b.run()
+    ev.record(e1, ctx.stream)
+    ctx.sync()
+    wall = (time.perf_counter() - t0) / args.steps
+    dev_ms = ev.elapsed_ms(e0, e1) / args.steps
+    sidx = list(range(0, len(items), max(1, len(items) // args.plain_sample)))
+    for i in sidx[:8]:
+        ctx.matmul_packed(dA[i], dB[i], dC[i])
+    ctx.sync()
+    t0 = time.perf_counter()
+    for i in sidx:
+        ctx.matmul_packed(dA[i], dB[i], dC[i])
+    ctx.sync()
+    plain_rate = len(sidx) / (time.perf_counter() - t0)
+    work = sum(2.0 * m * k * n for A, m, k, B, n, _ in items)
+    rate = len(items) / wall
+    row = {"workload": f"batched matmul, {len(items)} uneven products (A as the rank/null batch, "
+                       f"B dense k x b, b log-uniform 32..1024)",
+           "batched_ms": round(wall * 1e3, 3), "batched_device_ms": round(dev_ms, 3),
+           "products_per_s": round(rate, 1), "Tbitops": round(work / wall / 1e12, 2),
+           "plain_one_per_launch_products_per_s": round(plain_rate, 1),
+           "plain_sample": len(sidx), "speedup_vs_plain": round(rate / plain_rate, 2),
+           "items_andpop": algos.count(mela.ALGO_ANDPOP), "items_rowxor": algos.count(mela.ALGO_ROWXOR)}
+    b.close()
+    for M in dA + dB + dC:
+        ctx.free(M)
+    return row
 
 
 def bench_serve(args, mela):
@@ -434,18 +556,18 @@ def bench_serve(args, mela):
                "batched_requests": st["batched_requests"], "hbm_peak_GB": round(st["hbm_peak"] / 1e9, 3),
                "failed": len(errors)}
         results.append(row)
-        print(json.dumps({"workload": "serve", **row}), flush=True)
-    sat = max(r["achieved_req_per_s"] for r in results)
-    print(json.dumps({"workload": "serve summary", "saturation_req_per_s": sat,
-                      "workers": args.serve_workers, "hbm_cap_GB": 64,
-                      "mix": "70% rank / 20% null / 10% matmul, sides log-uniform 128..8192"}),
-          flush=True)
+        if args.workload == "serve":
+            print(json.dumps({"workload": "serve", **row}), flush=True)
     srv.close()
     for M in pool:
         ctx.free(M)
     for A, B in mm:
         ctx.free(A)
         ctx.free(B)
+    return {"loads": results, "saturation_req_per_s": max(r["achieved_req_per_s"] for r in results),
+            "workers": args.serve_workers, "hbm_cap_GB": 64,
+            "mix": "Poisson arrivals; 70% rank / 20% null / 10% matmul, sides log-uniform "
+                   "128..8192, half Bernoulli(1/2), half column weight 3..6"}
 
 
 def bench_c1(args, mela):
@@ -465,7 +587,7 @@ def bench_c1(args, mela):
     A = np.asfortranarray(rng.integers(0, 2, (n, n)).astype(bool))
     B = np.asfortranarray(rng.integers(0, 2, (n, n)).astype(bool))
     ctx = mela.Context(0)
-
+    rows = []
     for name, gfn, cfn, work in (
             ("matmul_gf2", lambda: mela.mela_matmul_gf2(A, B, ctx=ctx),
              lambda: orc.matmul(A, B, "gf2"), 2.0 * n ** 3),
@@ -484,7 +606,10 @@ def bench_c1(args, mela):
                         "cpu_threads": orc.num_threads()})
         if work:
             row["gpu_Tbitops_incl_pcie"] = round(work / (g * 1e-3) / 1e12, 2)
-        print(json.dumps(row), flush=True)
+        rows.append(row)
+        if args.workload == "c1":
+            print(json.dumps(row), flush=True)
+    return rows
 
 
 def bench_copy(args, mela):
@@ -507,7 +632,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="matmul", choices=["matmul", "elim", "batch", "serve", "c1", "copy"])
+    ap.add_argument("--workload", default="matmul",
+                    choices=["matmul", "elim", "batch", "batch_matmul", "serve", "c1", "copy"])
+    ap.add_argument("--headline-only", action="store_true",
+                    help="the 32768^3 product line without the batched / serve / c1 keys")
     ap.add_argument("--algo", default="auto",
                     choices=["auto", "m4rm", "andpop", "mfma_i8", "mfma_fp4"])
     ap.add_argument("--n", type=int, default=32768)
@@ -520,10 +648,13 @@ def main():
     ap.add_argument("--c1-n", type=int, default=2048,
                     help="side of the square host arrays of the c1 workload")
     ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--batch-mm-count", type=int, default=2048)
     ap.add_argument("--plain-sample", type=int, default=256)
     ap.add_argument("--bytes", type=int, default=1 << 30)
     ap.add_argument("--serve-rates", type=float, nargs="+",
                     default=[50, 100, 200, 400, 800, 1600, 3200, 6400, 12800])
+    ap.add_argument("--headline-serve-rates", type=float, nargs=2, default=[1600, 6400],
+                    help="the two offered loads of the headline line's serve key")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-pool", type=int, default=48)
     ap.add_argument("--serve-workers", type=int, default=8)
@@ -532,8 +663,16 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1"}))
         sys.exit(2)
     import mex_linear_algebra_amd as mela
-    {"matmul": bench_matmul, "elim": bench_elim, "batch": bench_batch, "serve": bench_serve,
-     "c1": bench_c1, "copy": bench_copy}[args.workload](args, mela)
+    if args.workload in ("batch", "batch_matmul"):
+        fn = bench_batch if args.workload == "batch" else bench_batch_matmul
+        print(json.dumps(fn(args, mela)), flush=True)
+    elif args.workload == "serve":
+        res = bench_serve(args, mela)
+        print(json.dumps({"workload": "serve summary",
+                          **{k: v for k, v in res.items() if k != "loads"}}), flush=True)
+    else:
+        {"matmul": bench_matmul, "elim": bench_elim, "c1": bench_c1,
+         "copy": bench_copy}[args.workload](args, mela)
 
 
 if __name__ == "__main__":

@@ -124,15 +124,62 @@ __device__ __forceinline__ bool group_sync(unsigned *bar, int G, unsigned phase,
     return s_ok != 0;
 }
 
-// One matrix of a batch (member mk of its d.groups workgroups): copy-in,
-// panel-by-panel elimination, rank and (want_null) the packed canonical null
-// basis into Z.  Returns false when a group barrier gave up (the batch then
-// fails as a whole).
-template <int TW>
-__device__ bool batch_matrix(BatchLDS<TW> &L, const BatchMat &d, int mk, unsigned *bar, int *err,
-                             const uint64_t *__restrict__ A, uint64_t *__restrict__ work,
-                             uint64_t *__restrict__ zarena, uint32_t *__restrict__ rank_out,
-                             int want_null) {
+// A value every lane of the wave holds alike, moved into scalar registers
+// (each 32-bit word read from the first active lane).  A work source reads
+// its descriptors through struct members, which carry no no-alias promise,
+// so the compiler loads them into vector registers; without this every
+// matrix parameter and everything derived from it would occupy VGPRs.
+template <class T>
+__device__ __forceinline__ T wave_uniform(const T &v) {
+    static_assert(sizeof(T) % 4 == 0, "32-bit words");
+    uint32_t w[sizeof(T) / 4];
+    __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) w[i] = (uint32_t)__builtin_amdgcn_readfirstlane((int)w[i]);
+    T out;
+    __builtin_memcpy(&out, w, sizeof(T));
+    return out;
+}
+
+// The matrix one workgroup works on next, as a work source hands it out:
+// member mk of the matrix's d.groups workgroups, its group barrier, input,
+// working memory (rows at work + d.w_off, pivot rows at work + d.q_off),
+// null-basis arena, and the source's index of the matrix (its rank slot).
+struct BatchItem {
+    BatchMat d;
+    const uint64_t *A;
+    uint64_t *work;
+    uint64_t *zarena;
+    unsigned *bar;
+    long long seq;  // the serving ring's sequence number (ServeSource)
+    int idx;
+    int mk;
+    int want_null;
+};
+
+// The elimination loop, one kernel for every work source: each workgroup
+// takes the next matrix from src.next() (called by every thread; false ends
+// the loop), copies it in, eliminates it panel by panel, writes its rank and
+// (want_null) its packed canonical null basis into Z, and hands it back to
+// src.finish(); src.rank() stores the rank.  A group barrier that gave up skips the rest of the matrix
+// (the batch then fails as a whole; only a batch source has groups).
+template <int TW, class Src>
+__global__ void __launch_bounds__(BT)
+k_rref_loop(const Src src_arg) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    BatchLDS<TW> &L = *reinterpret_cast<BatchLDS<TW> *>(smem);
+    Src src = src_arg;
+    int *err = src.err();
+    for (;;) {
+    BatchItem it;
+    if (!src.next(L, it)) break;
+    const BatchMat d = it.d;
+    const int mk = it.mk;
+    unsigned *bar = it.bar;
+    const uint64_t *A = it.A;
+    uint64_t *work = it.work;
+    uint64_t *zarena = it.zarena;
+    const int want_null = it.want_null;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int G = (int)d.groups;
     unsigned phase = 0;
@@ -166,7 +213,7 @@ __device__ bool batch_matrix(BatchLDS<TW> &L, const BatchMat &d, int mk, unsigne
         L.s_r = 0;
         L.s_nact = m;
     }
-    if (!group_sync(bar, G, ++phase, err, L.s_ok)) return false;
+    if (!group_sync(bar, G, ++phase, err, L.s_ok)) continue;
 
     // Members run the same panel selection on the same words (identical
     // results); the tiles of each panel are dealt out round robin; one
@@ -395,11 +442,11 @@ __device__ bool batch_matrix(BatchLDS<TW> &L, const BatchMat &d, int mk, unsigne
         __syncthreads();
     }
 
-    if (dropped) return false;
+    if (dropped) continue;
     // the last panel's pivot row words before any member extracts
-    if (!group_sync(bar, G, ++phase, err, L.s_ok)) return false;
+    if (!group_sync(bar, G, ++phase, err, L.s_ok)) continue;
     const int r = L.s_r;
-    if (tid == 0 && mk == 0) *rank_out = (uint32_t)r;
+    if (tid == 0 && mk == 0) src.rank(it, (uint32_t)r);
     if (want_null) {
         const int f = n - r;
         // prefix pivot counts per column word
@@ -489,60 +536,79 @@ __device__ bool batch_matrix(BatchLDS<TW> &L, const BatchMat &d, int mk, unsigne
             }
         }
     }
-    return true;
-}
-
-template <int TW>
-__global__ void __launch_bounds__(BT)
-k_batch_rref(const BatchMat *__restrict__ mats, const int *__restrict__ order,
-             const int2 *__restrict__ slots, int nslots, int *__restrict__ counter,
-             unsigned *__restrict__ gbar, const uint64_t *const *__restrict__ inputs,
-             uint64_t *__restrict__ work, uint64_t *__restrict__ zarena, uint32_t *__restrict__ ranks,
-             int want_null) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    BatchLDS<TW> &L = *reinterpret_cast<BatchLDS<TW> *>(smem);
-    const int tid = threadIdx.x;
-
-    // queue of slots: slot s = member slots[s].y of queue entry slots[s].x;
-    // a matrix's G slots are consecutive, so at most one matrix at a time is
-    // partly staffed and every member is taken by a running workgroup
-    int *err = counter + 1;
-    for (;;) {
-        if (tid == 0) L.s_idx = atomicAdd(counter, 1);
-        __syncthreads();
-        const int si = L.s_idx;
-        if (si >= nslots) break;
-        const int2 sl = slots[si];
-        const int qi = sl.x, mk = sl.y;  // queue entry, member index
-        const int mi = order[qi];
-        const BatchMat d = mats[mi];
-        (void)batch_matrix<TW>(L, d, mk, gbar + qi, err, inputs[mi], work, zarena, ranks + mi, want_null);
-        __syncthreads();
+    src.finish(it);
+    __syncthreads();
     }
 }
-// The device-resident serving loop: each workgroup takes published request
-// slots (compare-and-swap on head, in device memory, while head < tail),
-// eliminates the matrix with the batch machinery (one workgroup per matrix),
-// and reports it; it ends when the host sets the stop bit of tail, when no
-// slot was published for idle_ticks, or, between requests, once the launch
-// is life_ticks old (the host starts the next instance at once when work
-// remains: a kernel that never ended would hold up every stream that shares
-// its hardware queue).  An idle workgroup polls the host's tail every
-// ~0.3 us at first and every ~4 us after 50 us without work.
-template <int TW>
-__global__ void __launch_bounds__(BT)
-k_serve_loop(const ServeSlot *__restrict__ slots, ServeDone *__restrict__ done,
-             const uint64_t *__restrict__ tail, uint64_t *__restrict__ head, uint32_t cap,
-             uint64_t idle_ticks, uint64_t life_ticks) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    BatchLDS<TW> &L = *reinterpret_cast<BatchLDS<TW> *>(smem);
-    __shared__ long long s_seq;
-    __shared__ ServeSlot s_slot;
-    const int tid = threadIdx.x;
-    const uint64_t t_start = wall_clock64();
-    uint64_t t_idle = t_start;
-    for (;;) {
-        if (tid == 0) {
+
+// Work source of a batch launch.  Slot s of the schedule is member
+// slots[s].y of queue entry slots[s].x (matrix order[x]); a matrix's G slots
+// are consecutive, so at most one matrix at a time is partly staffed and
+// every member is taken by a running workgroup.
+struct BatchSource {
+    const BatchMat *__restrict__ mats;
+    const int *__restrict__ order;
+    const int2 *__restrict__ slots;
+    int *__restrict__ counter;  // [0] next slot, [1] error flag
+    unsigned *__restrict__ gbar;
+    const uint64_t *const *__restrict__ inputs;
+    uint64_t *__restrict__ work;
+    uint64_t *__restrict__ zarena;
+    uint32_t *__restrict__ ranks;
+    int nslots;
+    int want_null;
+
+    __device__ int *err() const { return counter + 1; }
+    template <class LDS>
+    __device__ bool next(LDS &L, BatchItem &it) {
+        if (threadIdx.x == 0) L.s_idx = atomicAdd(counter, 1);
+        __syncthreads();
+        const int si = __builtin_amdgcn_readfirstlane(L.s_idx);
+        if (si >= nslots) return false;
+        const int2 sl = wave_uniform(slots[si]);
+        const int mi = __builtin_amdgcn_readfirstlane(order[sl.x]);
+        it.d = wave_uniform(mats[mi]);
+        it.A = wave_uniform(inputs[mi]);
+        it.work = work;
+        it.zarena = zarena;
+        it.bar = gbar + sl.x;
+        it.seq = 0;
+        it.idx = mi;
+        it.mk = sl.y;
+        it.want_null = want_null;
+        return true;
+    }
+    __device__ void rank(const BatchItem &it, uint32_t r) { ranks[it.idx] = r; }
+    __device__ void finish(const BatchItem &) {}
+};
+
+// Work source of the device-resident serving loop: each workgroup takes
+// published request slots (compare-and-swap on head, in device memory, while
+// head < tail), one workgroup per matrix, and reports each one; it ends when
+// the host sets the stop bit of tail, when no slot was published for
+// idle_ticks, or, between requests, once the launch is life_ticks old (the
+// host starts the next instance at once when work remains: a kernel that
+// never ended would hold up every stream that shares its hardware queue).
+// An idle workgroup polls the host's tail every ~0.3 us at first and every
+// ~4 us after 50 us without work.
+struct ServeSource {
+    const ServeSlot *__restrict__ slots;
+    ServeDone *__restrict__ done;
+    const uint64_t *__restrict__ tail;
+    uint64_t *__restrict__ head;
+    uint64_t idle_ticks;
+    uint64_t life_ticks;
+    uint64_t t_start;  // set by the kernel's first next()
+    uint64_t t_idle;
+    uint32_t cap;
+
+    __device__ int *err() const { return nullptr; }  // one workgroup per matrix
+    template <class LDS>
+    __device__ bool next(LDS &, BatchItem &it) {
+        __shared__ long long s_seq;
+        __shared__ ServeSlot s_slot;
+        if (t_start == 0) t_start = t_idle = wall_clock64();
+        if (threadIdx.x == 0) {
             long long got = -1;
             for (;;) {
                 if (wall_clock64() - t_start > life_ticks) break;
@@ -574,33 +640,43 @@ k_serve_loop(const ServeSlot *__restrict__ slots, ServeDone *__restrict__ done,
             s_seq = got;
         }
         __syncthreads();
-        const long long seq = s_seq;
-        if (seq < 0) break;
-        const uint32_t idx = (uint32_t)((uint64_t)seq % cap);
-        const ServeSlot sl = s_slot;
-        BatchMat d{};
-        d.m = sl.m;
-        d.n = sl.n;
-        d.lda = sl.lda;
+        const long long seq = wave_uniform(s_seq);
+        if (seq < 0) return false;
+        const ServeSlot sl = wave_uniform(s_slot);
         const uint32_t wc = (sl.n + 63) / 64;
-        d.ldz = wc;
-        d.w_off = 0;
-        d.q_off = (uint64_t)sl.m * wc;
-        d.z_off = 0;
-        d.mode = sl.mode;
-        d.groups = 1;
-        (void)batch_matrix<TW>(L, d, 0, nullptr, nullptr, sl.A, sl.work, sl.Z, &done[idx].rank,
-                               sl.op == MELA_OP_NULL ? 1 : 0);
-        // the rank, the null basis and the working rows are written: make them
-        // visible to the host and its copy engines, then report
+        it.d = BatchMat{};
+        it.d.m = sl.m;
+        it.d.n = sl.n;
+        it.d.lda = sl.lda;
+        it.d.ldz = wc;
+        it.d.w_off = 0;
+        it.d.q_off = (uint64_t)sl.m * wc;
+        it.d.z_off = 0;
+        it.d.mode = sl.mode;
+        it.d.groups = 1;
+        it.A = sl.A;
+        it.work = sl.work;
+        it.zarena = sl.Z;
+        it.bar = nullptr;
+        it.seq = seq;
+        it.idx = (int)((uint64_t)seq % cap);
+        it.mk = 0;
+        it.want_null = sl.op == MELA_OP_NULL ? 1 : 0;
+        return true;
+    }
+    __device__ void rank(const BatchItem &it, uint32_t r) { done[it.idx].rank = r; }
+    // the rank, the null basis and the working rows are written: make them
+    // visible to the host and its copy engines, then report (the loop's
+    // closing barrier keeps s_seq until every thread has read it)
+    __device__ void finish(const BatchItem &it) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
         __syncthreads();
-        if (tid == 0)
-            __hip_atomic_store(&done[idx].seq1, (uint64_t)seq + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (threadIdx.x == 0)
+            __hip_atomic_store(&done[it.idx].seq1, (uint64_t)it.seq + 1, __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
         t_idle = wall_clock64();
-        __syncthreads();  // s_seq is rewritten only after every thread has read it
     }
-}
+};
 
 // queue counter, error flag and the group barrier counters start at zero
 __global__ void k_reset_batch(int *counter, unsigned *gbar, int count) {
@@ -645,14 +721,22 @@ mela_status launch_serve_loop(hipStream_t s, int grid, ServeSlot *slots, ServeDo
     {
         std::lock_guard<std::mutex> lk(mu);
         if (!attr) {
-            MELA_HIP(hipFuncSetAttribute((const void *)k_serve_loop<16>,
+            MELA_HIP(hipFuncSetAttribute((const void *)k_rref_loop<16, ServeSource>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)batch_lds_bytes(16)));
             attr = true;
         }
     }
-    hipLaunchKernelGGL(k_serve_loop<16>, dim3((unsigned)grid), dim3(BT), batch_lds_bytes(16), s, slots, done,
-                       tail, head, cap, idle_ticks, life_ticks);
+    ServeSource src{};
+    src.slots = slots;
+    src.done = done;
+    src.tail = tail;
+    src.head = head;
+    src.idle_ticks = idle_ticks;
+    src.life_ticks = life_ticks;
+    src.cap = cap;
+    hipLaunchKernelGGL((k_rref_loop<16, ServeSource>), dim3((unsigned)grid), dim3(BT), batch_lds_bytes(16), s,
+                       src);
     MELA_HIP(hipGetLastError());
     return MELA_OK;
 }
@@ -707,14 +791,22 @@ mela_status batch_launch_raw(mela_batch *b, hipStream_t s) {
                        b->d_counter, b->d_gbar, nb);
     const int want_null = (b->flags & MELA_BATCH_NULL) ? 1 : 0;
     const int grid = batch_grid(b);
+    BatchSource src{};
+    src.mats = b->d_mats;
+    src.order = b->d_order;
+    src.slots = b->d_slots;
+    src.counter = b->d_counter;
+    src.gbar = b->d_gbar;
+    src.inputs = b->d_inputs;
+    src.work = b->d_work;
+    src.zarena = b->d_z;
+    src.ranks = b->d_ranks;
+    src.nslots = b->nslots;
+    src.want_null = want_null;
     if (b->tw == 32)
-        hipLaunchKernelGGL(k_batch_rref<32>, dim3(grid), dim3(BT), batch_lds_bytes(32), s, b->d_mats,
-                           b->d_order, b->d_slots, b->nslots, b->d_counter, b->d_gbar, b->d_inputs,
-                           b->d_work, b->d_z, b->d_ranks, want_null);
+        hipLaunchKernelGGL((k_rref_loop<32, BatchSource>), dim3(grid), dim3(BT), batch_lds_bytes(32), s, src);
     else
-        hipLaunchKernelGGL(k_batch_rref<16>, dim3(grid), dim3(BT), batch_lds_bytes(16), s, b->d_mats,
-                           b->d_order, b->d_slots, b->nslots, b->d_counter, b->d_gbar, b->d_inputs,
-                           b->d_work, b->d_z, b->d_ranks, want_null);
+        hipLaunchKernelGGL((k_rref_loop<16, BatchSource>), dim3(grid), dim3(BT), batch_lds_bytes(16), s, src);
     MELA_HIP(hipGetLastError());
     return MELA_OK;
 }
@@ -789,10 +881,10 @@ mela_status mela_batch_create(mela_ctx *ctx, const mela_bitmat *mats, size_t cou
     static bool attr_set = false;
     std::lock_guard<std::mutex> alk(attr_mu);
     if (!attr_set) {
-        MELA_HIP(hipFuncSetAttribute((const void *)k_batch_rref<16>,
+        MELA_HIP(hipFuncSetAttribute((const void *)k_rref_loop<16, BatchSource>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)batch_lds_bytes(16)));
-        MELA_HIP(hipFuncSetAttribute((const void *)k_batch_rref<32>,
+        MELA_HIP(hipFuncSetAttribute((const void *)k_rref_loop<32, BatchSource>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)batch_lds_bytes(32)));
         attr_set = true;
@@ -889,7 +981,7 @@ mela_status mela_batch_create(mela_ctx *ctx, const mela_bitmat *mats, size_t cou
     }
     for (int v = 0; v < 2; ++v) {
         int occ = 0;
-        const void *fn = v ? (const void *)k_batch_rref<32> : (const void *)k_batch_rref<16>;
+        const void *fn = v ? (const void *)k_rref_loop<32, BatchSource> : (const void *)k_rref_loop<16, BatchSource>;
         MELA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, BT, batch_lds_bytes(v ? 32 : 16)));
         b->occ_tw[v] = occ < 1 ? 1 : occ;
     }
