@@ -318,7 +318,8 @@ def bench_elim(args, mela):
     for name, M, m, n in cases:
         Rm, piv = ctx.rref_packed(M)
         ctx.free(Rm)
-        for variant, vname in ((mela.ELIM_GE, "ge"), (mela.ELIM_M4RI, "m4ri")):
+        for variant, vname in ((mela.ELIM_GE, "ge"), (mela.ELIM_M4RI, "m4ri"),
+                               (mela.ELIM_PANEL, "panel"), (mela.ELIM_AUTO, "auto")):
             for op in ("rank", "null"):
                 # two untimed calls: the second one captures the panel sequence
                 # as a hipGraph, which the timed calls replay
@@ -330,6 +331,7 @@ def bench_elim(args, mela):
                 ctx.sync()
                 e0, e1 = ev.event(), ev.event()
                 reps = max(1, args.elim_reps)
+                c0 = ctx.elim_counts()
                 t0 = time.perf_counter()
                 ev.record(e0, ctx.stream)
                 for _ in range(reps):
@@ -343,11 +345,15 @@ def bench_elim(args, mela):
                 ctx.sync()
                 wall = (time.perf_counter() - t0) * 1e3 / reps
                 dev = ev.elapsed_ms(e0, e1) / reps
+                c1 = ctx.elim_counts()
+                path = ("blocked" if c1[0] > c0[0] else "") + ("fallback" if c1[1] > c0[1] else "") + \
+                       ("panel" if c1[2] > c0[2] else "")
                 bytes_alg = elim_bytes_alg(m, n, piv, op == "null")
                 if op == "null":
                     bytes_alg += n * ((n - len(piv) + 63) // 64) * 8
                 print(json.dumps({
-                    "workload": f"{op} {vname} {name}", "rank": r, "ms": round(wall, 3),
+                    "workload": f"{op} {vname} {name}", "path": path, "rank": r,
+                    "ms": round(wall, 3),
                     "device_ms": round(dev, 3), "bytes_alg_GB": round(bytes_alg / 1e9, 3),
                     "GBps_alg": round(bytes_alg / (dev * 1e-3) / 1e9, 1),
                     "hbm_frac": round(bytes_alg / (dev * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}),

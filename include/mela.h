@@ -69,6 +69,11 @@ mela_status mela_ctx_sync(mela_ctx *ctx);
    kernel time and the number of timed launches. */
 mela_status mela_ctx_profile_begin(mela_ctx *ctx);
 mela_status mela_ctx_profile_end(mela_ctx *ctx, double *total_ms, int *launches);
+/* Eliminations run by this context, by path (no reference counterpart; for
+   tests and benchmarks): blocked = finished on the blocked path, fallback =
+   blocked attempts redone on the panel path, panel = panel path only. */
+mela_status mela_ctx_elim_counts(mela_ctx *ctx, unsigned long long *blocked,
+                                 unsigned long long *fallback, unsigned long long *panel);
 /* message and MATLAB identifier of the last failure on this thread */
 const char *mela_last_error(void);
 const char *mela_last_error_id(void);
@@ -172,13 +177,26 @@ mela_status mela_unpack_dev(mela_ctx *ctx, const mela_bitmat *M, uint8_t *dst_de
 mela_status mela_matmul_packed(mela_ctx *ctx, const mela_bitmat *A,
                                const mela_bitmat *B, mela_bitmat *C, int algo);
 
-/* Elimination variants: GE (pivot rows applied one by one), M4RI (pivot
- * rows applied through 2^k-entry tables) and AUTO (per row batch: the walk
- * for multipliers with few set bits, the tables otherwise).  Outputs are
- * identical -- rank, the reduced row echelon form and the canonical null
- * basis are unique -- so the six host-array entry points use AUTO whichever
- * reference algorithm they stand for; GE and M4RI force one sweep. */
-enum { MELA_ELIM_GE = 0, MELA_ELIM_M4RI = 1, MELA_ELIM_AUTO = 2 };
+/* Elimination variants.  Outputs are identical -- rank, the reduced row
+ * echelon form and the canonical null basis are unique -- so the six
+ * host-array entry points use AUTO whichever reference algorithm they stand
+ * for.  The panel path eliminates 64 columns per step and sweeps every
+ * remaining row with the panel's pivot rows: GE applies them one by one, M4RI
+ * through 4-bit tables, PANEL decides per row batch (the walk for
+ * multipliers with few set bits, the tables otherwise).  The blocked path
+ * factors 512 columns at a time from the head of the remaining rows and
+ * applies them to every other row in one matrix-core product; when a block's
+ * pivots are not all in its head (sparse or structured matrices) the call
+ * redoes the elimination on the panel path, and the context keeps that
+ * shape on the panel path.  AUTO: blocked from 2048 x 2048 on, else PANEL;
+ * BLOCKED: the blocked path at any size. */
+enum {
+    MELA_ELIM_GE = 0,
+    MELA_ELIM_M4RI = 1,
+    MELA_ELIM_AUTO = 2,
+    MELA_ELIM_BLOCKED = 3,
+    MELA_ELIM_PANEL = 4
+};
 
 /* rank of A (A is left unchanged; work memory comes from the context) */
 mela_status mela_rank_packed(mela_ctx *ctx, const mela_bitmat *A, int variant,

@@ -25,7 +25,7 @@ __all__ = [
     "mela_null_gf2", "mela_null_m4ri", "mela_rref_gf2",
     "ALGO_AUTO", "ALGO_ANDPOP", "ALGO_M4RM", "ALGO_MFMA_I8", "ALGO_MFMA_FP4", "ALGO_ROWXOR",
     "ELIM_GE", "ELIM_M4RI",
-    "ELIM_AUTO",
+    "ELIM_AUTO", "ELIM_BLOCKED", "ELIM_PANEL",
     "Batch", "BATCH_NULL", "BATCH_GE", "BATCH_GRAPH", "MatmulBatch",
     "Server", "Request", "Result", "OP_RANK", "OP_NULL", "OP_MATMUL",
 ]
@@ -34,7 +34,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 
 MELA_LOGICAL, MELA_DOUBLE, MELA_OTHER, MELA_SPARSE = 0, 1, 2, 0x100
 ALGO_AUTO, ALGO_ANDPOP, ALGO_M4RM, ALGO_MFMA_I8, ALGO_MFMA_FP4, ALGO_ROWXOR = 0, 1, 2, 3, 4, 5
-ELIM_GE, ELIM_M4RI, ELIM_AUTO = 0, 1, 2
+ELIM_GE, ELIM_M4RI, ELIM_AUTO, ELIM_BLOCKED, ELIM_PANEL = 0, 1, 2, 3, 4
 
 STATUS = {0: "OK", 1: "SPARSE", 2: "DIM", 3: "TYPE", 4: "NOMEM", 5: "INVALID",
           6: "DEVICE", 7: "ADMISSION", 8: "CANCELLED", 9: "TIMEOUT", 10: "ARGS"}
@@ -106,6 +106,8 @@ def load():
         L.mela_ctx_stream.restype = vp
         L.mela_ctx_sync.argtypes = [ctxp]
         L.mela_ctx_sync.restype = st
+        L.mela_ctx_elim_counts.argtypes = [ctxp] + [C.POINTER(C.c_ulonglong)] * 3
+        L.mela_ctx_elim_counts.restype = st
         L.mela_ctx_profile_begin.argtypes = [ctxp]
         L.mela_ctx_profile_begin.restype = st
         L.mela_ctx_profile_end.argtypes = [ctxp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
@@ -242,6 +244,12 @@ class Context:
 
     def sync(self):
         _check(self.lib.mela_ctx_sync(self.handle))
+
+    def elim_counts(self):
+        """Eliminations run by this context: (blocked, fallback, panel)."""
+        v = [C.c_ulonglong() for _ in range(3)]
+        _check(self.lib.mela_ctx_elim_counts(self.handle, *[C.byref(x) for x in v]))
+        return tuple(int(x.value) for x in v)
 
     def profile_begin(self):
         """Start timing the dominant kernel of every matmul_packed call."""

@@ -280,7 +280,7 @@ mela_status null_entry(mela_ctx *ctx, const void *H, int hc, size_t m, size_t n,
 }
 
 mela_status check_variant(int variant) {
-    if (variant != MELA_ELIM_GE && variant != MELA_ELIM_M4RI && variant != MELA_ELIM_AUTO)
+    if (variant < MELA_ELIM_GE || variant > MELA_ELIM_PANEL)
         return set_error(MELA_ERR_INVALID, "unknown elimination variant");
     return MELA_OK;
 }

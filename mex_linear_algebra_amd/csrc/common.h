@@ -44,10 +44,14 @@ struct ElimGraph {
     int variant = 0, rref = 0;
     const void *bufs[4] = {nullptr, nullptr, nullptr, nullptr};
     int seen = 0;
-    // panels the last uncaptured run needed (the first panel whose selection
-    // found the elimination complete, plus one; the word count if none did)
+    // panels (blocks, on the blocked path) the last uncaptured run needed
+    // (the first one whose selection found the elimination complete, plus
+    // one; all of them if none did)
     size_t panels_needed = 0;
     size_t g_end = 0;  // panels in the captured sequence
+    // the blocked path (512-column blocks) missed a pivot on this shape once:
+    // its calls take the panel path from then on
+    bool blocked_failed = false;
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
 };
@@ -77,6 +81,8 @@ struct mela_ctx {
     // staging in ordinary host memory from the start (MELA_FORCE_PAGEABLE=1 at
     // context creation): the path taken when no pinned memory can be had
     bool force_pageable = false;
+    // eliminations by path (mela_ctx_elim_counts)
+    unsigned long long elim_blocked = 0, elim_fallback = 0, elim_panel = 0;
     // this device's allocation-free scratch (mela_rank_packed_noalloc), or
     // nullptr with the reason in noalloc_err
     void *noalloc_scratch = nullptr;
@@ -127,6 +133,7 @@ enum ScratchSlot {
     SLOT_ELIM_M, SLOT_ELIM_Q, SLOT_ELIM_META, SLOT_ELIM_ACT, SLOT_ELIM_TAB,
     SLOT_NULL_Z, SLOT_XA, SLOT_XB,
     SLOT_BATCH_DESC, SLOT_BATCH_WORK, SLOT_BATCH_Z, SLOT_BATCH_RANK, SLOT_NULL_TAB, SLOT_DENSITY,
+    SLOT_ELIM_BLK,
     SLOT_COUNT
 };
 

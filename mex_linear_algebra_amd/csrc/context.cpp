@@ -207,6 +207,16 @@ mela_status mela_ctx_sync(mela_ctx *ctx) {
     return MELA_OK;
 }
 
+mela_status mela_ctx_elim_counts(mela_ctx *ctx, unsigned long long *blocked,
+                                 unsigned long long *fallback, unsigned long long *panel) {
+    if (!ctx || !blocked || !fallback || !panel) return set_error(MELA_ERR_INVALID, "null argument");
+    CtxLock lk(ctx->mu);
+    *blocked = ctx->elim_blocked;
+    *fallback = ctx->elim_fallback;
+    *panel = ctx->elim_panel;
+    return MELA_OK;
+}
+
 mela_status mela_ctx_profile_begin(mela_ctx *ctx) {
     if (!ctx) return set_error(MELA_ERR_INVALID, "null context");
     CtxLock lk(ctx->mu);

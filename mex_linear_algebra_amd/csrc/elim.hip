@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <mutex>
 
+#include "elim_block.h"
 #include "elim_device.h"
 
 // Work memory of the allocation-free rank entry point
@@ -45,21 +46,6 @@ struct NullWord {
     long fb;         // free columns before the word
     int ws;          // (entry j) the column word holding free column 64 j
     int pad;
-};
-
-struct ElimMeta {
-    int n_act;  // active (not yet pivot) rows
-    int r;      // pivot rows found so far
-    int r0;     // pivot rows before the current panel
-    int rho;    // pivots in the current panel
-    uint64_t pm;        // pivot bits (columns 64g + c) of the current panel
-    int finished;       // every active row is zero: the remaining panels have no pivot
-    int last_swept;     // last panel with pivots (its sweep ran), -1 before the first
-    uint64_t pm_prev;   // pm and r0 of the previous panel (RREF: its word update is
-    int r0_prev;        // applied by the next panel's sweep)
-    int done_at;        // first panel whose selection found the elimination complete
-    int sel_row[64];           // selected row ids (selection slot -> row)
-    uint64_t comb[64];  // by pivot bit c: selection slots XORed into pivot row c
 };
 
 namespace {
@@ -538,6 +524,15 @@ size_t meta_panelw_off(size_t rmax) { return meta_swept_off(rmax) + NSWEPT * siz
 size_t round256(size_t x) { return (x + 255) & ~(size_t)255; }
 }  // namespace
 
+// Shapes the blocked path takes: MELA_ELIM_BLOCKED always, AUTO from
+// BLK_MIN_ROWS x BLK_MIN_COLS on (smaller matrices have few panels, and the
+// panel path's launches are short there).
+constexpr size_t BLK_MIN_ROWS = 2048, BLK_MIN_COLS = 2048;
+static bool blocked_candidate(int variant, size_t m, size_t n) {
+    if (variant == MELA_ELIM_BLOCKED) return true;
+    return variant == MELA_ELIM_AUTO && m >= BLK_MIN_ROWS && n >= BLK_MIN_COLS;
+}
+
 ElimLayout elim_layout(size_t m, size_t n) {
     ElimLayout L;
     const size_t wc = words64(n), rmax = m < n ? m : n;
@@ -567,10 +562,13 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
     if (m > (size_t)INT32_MAX || wc > (size_t)INT32_MAX)
         return set_error(MELA_ERR_INVALID, "matrix dimension exceeds 2^31-1");
 
-    void *pM = nullptr, *pQ = nullptr, *pMeta = nullptr, *pAct = nullptr;
+    void *pM = nullptr, *pQ = nullptr, *pMeta = nullptr, *pAct = nullptr, *pBlk = nullptr;
     mela_status st;
     const size_t swept_off = meta_swept_off(rmax);
     const size_t panelw_off = meta_panelw_off(rmax);
+    // the blocked path (elim_block.hip) for large matrices; never with the
+    // caller's fixed buffers
+    const bool blocked_shape = !fixed && blocked_candidate(variant, m, n);
     if (fixed) {
         pM = fixed->M;
         pQ = fixed->Q;
@@ -584,6 +582,9 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
             return st;
         if ((st = scratch_get(ctx, SLOT_ELIM_ACT, m * sizeof(uint32_t), &pAct)) != MELA_OK)
             return st;
+        if (blocked_shape &&
+            (st = scratch_get(ctx, SLOT_ELIM_BLK, block_scratch_bytes(m, n, rref), &pBlk)) != MELA_OK)
+            return st;
     }
     uint64_t *Mw = (uint64_t *)pM, *Q = (uint64_t *)pQ;
     ElimMeta *meta = (ElimMeta *)pMeta;
@@ -593,25 +594,36 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
     uint64_t *panelw = (uint64_t *)((char *)pMeta + panelw_off);
 
     const size_t tot = m * wc;
-    hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, A, lda,
-                       Mw, ld, m, wc);
-    MELA_HIP(hipGetLastError());
+    auto copy_rows = [&]() -> mela_status {
+        hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, A, lda,
+                           Mw, ld, m, wc);
+        MELA_HIP(hipGetLastError());
+        return MELA_OK;
+    };
+    if ((st = copy_rows()) != MELA_OK) return st;
 
     const int cu = ctx->cu_count > 0 ? ctx->cu_count : 256;
+    // the panel sweep: GE and M4RI force theirs, every other variant the
+    // per-batch choice
+    const int sweep = variant == MELA_ELIM_GE ? SW_GE : variant == MELA_ELIM_M4RI ? SW_M4RI : SW_AUTO;
+    auto init_state = [&]() -> mela_status {
+        hipLaunchKernelGGL(k_elim_init, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, meta, act,
+                           (int)m);
+        // pivot rows are zero left of their panel (only their words >= g are
+        // written); the RREF output and the null extraction read them whole
+        if (rref) {
+            mela_status zs = launch_fill_zero(s, Q, rmax * ld * sizeof(uint64_t));
+            if (zs != MELA_OK) return zs;
+        }
+        return MELA_OK;
+    };
     // panels g_begin .. g_end-1 (the first range also initialises the state);
     // `last`: for RREF, the final panel's word of the earlier pivot rows too
     auto enqueue_panels = [&](size_t g_begin, size_t g_end, bool last) -> mela_status {
         if (g_begin == 0) {
-            hipLaunchKernelGGL(k_elim_init, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s,
-                               meta, act, (int)m);
-            // pivot rows are zero left of their panel (only their words >= g
-            // are written); the RREF output and the null extraction read them
-            // whole
-            if (rref) {
-                mela_status zs = launch_fill_zero(s, Q, rmax * ld * sizeof(uint64_t));
-                if (zs != MELA_OK) return zs;
-            }
-            mela_status zs = launch_fill_zero(s, swept, NSWEPT * sizeof(unsigned));
+            mela_status zs = init_state();
+            if (zs != MELA_OK) return zs;
+            zs = launch_fill_zero(s, swept, NSWEPT * sizeof(unsigned));
             if (zs != MELA_OK) return zs;
         }
         for (size_t g = g_begin; g < g_end; ++g) {
@@ -630,11 +642,11 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
                 const unsigned ty_max = (unsigned)((rows_hint + per - 1) / per);
                 if (ty > ty_max) ty = ty_max;
                 if (ty < 1) ty = 1;
-                if (variant == MELA_ELIM_GE)
+                if (sweep == SW_GE)
                     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sweep<SWEEP_TILE, SW_GE>), dim3(tx, ty),
                                        dim3(UP_WAVES * 64), 0, s, Mw, ld, act, Q, ld, meta, swept, panelw, (int)g,
                                        (int)wc, rref ? 1 : 0, (int)m);
-                else if (variant == MELA_ELIM_M4RI)
+                else if (sweep == SW_M4RI)
                     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sweep<SWEEP_TILE, SW_M4RI>), dim3(tx, ty),
                                        dim3(UP_WAVES * 64), 0, s, Mw, ld, act, Q, ld, meta, swept, panelw, (int)g,
                                        (int)wc, rref ? 1 : 0, (int)m);
@@ -652,25 +664,40 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
         MELA_HIP(hipGetLastError());
         return MELA_OK;
     };
+    const size_t nblk = (wc + BLK_WORDS - 1) / BLK_WORDS;
+    auto enqueue_blk = [&](size_t b_begin, size_t b_end) -> mela_status {
+        if (b_begin == 0) {
+            mela_status zs = init_state();
+            if (zs != MELA_OK) return zs;
+        }
+        return enqueue_blocks(ctx, Mw, ld, act, meta, pivcols, Q, pBlk, m, n, rref, b_begin, b_end);
+    };
     ElimMeta hm;
-    auto read_meta = [&]() -> mela_status {
+    BlockState hb;
+    auto read_meta = [&](bool blk) -> mela_status {
         MELA_HIP(hipMemcpyAsync(&hm, meta, sizeof(ElimMeta), hipMemcpyDeviceToHost, s));
+        if (blk) MELA_HIP(hipMemcpyAsync(&hb, pBlk, sizeof(BlockState), hipMemcpyDeviceToHost, s));
         MELA_HIP(hipStreamSynchronize(s));
         return MELA_OK;
     };
-    if (fixed) {
-        // caller's buffers: every panel enqueued directly, no graph
-        if ((st = enqueue_panels(0, wc, true)) != MELA_OK) return st;
-        if ((st = read_meta()) != MELA_OK) return st;
-        res->rank = (size_t)hm.r;
-        if (want_pivots && hm.r > 0) {
-            res->pivots.resize((size_t)hm.r);
-            MELA_HIP(hipMemcpyAsync(res->pivots.data(), pivcols, (size_t)hm.r * sizeof(int),
+    auto read_pivots = [&]() -> mela_status {
+        const int r_dev = hm.r;
+        res->rank = (size_t)r_dev;
+        if (want_pivots && r_dev > 0) {
+            res->pivots.resize((size_t)r_dev);
+            MELA_HIP(hipMemcpyAsync(res->pivots.data(), pivcols, (size_t)r_dev * sizeof(int),
                                     hipMemcpyDeviceToHost, s));
             MELA_HIP(hipStreamSynchronize(s));
         }
         if (Q_out) *Q_out = Q;
         return MELA_OK;
+    };
+    if (fixed) {
+        // caller's buffers: every panel enqueued directly, no graph
+        if ((st = enqueue_panels(0, wc, true)) != MELA_OK) return st;
+        if ((st = read_meta(false)) != MELA_OK) return st;
+        ctx->elim_panel++;
+        return read_pivots();
     }
     // replay a captured sequence for a shape seen before (the work buffers are
     // the context's cached scratch, so their addresses repeat)
@@ -694,52 +721,75 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
         ctx->elim_graphs.push_back(e);
         eg = &ctx->elim_graphs.back();
     }
+    const bool blocked = blocked_shape && !eg->blocked_failed;
+    // work units: blocks or panels; a unit's selection records when it finds
+    // the elimination complete
+    const size_t units = blocked ? nblk : wc;
+    auto enqueue_units = [&](size_t u0, size_t u1) -> mela_status {
+        return blocked ? enqueue_blk(u0, u1) : enqueue_panels(u0, u1, u1 == wc);
+    };
+    auto done_unit = [&]() -> size_t {
+        const int d = blocked ? hb.done_blk : hm.done_at;
+        return d >= 0 && (size_t)d < units ? (size_t)d : units;
+    };
     eg->seen++;
     if (!eg->exec && eg->seen >= 2 && wc <= 4096 && ctx->capture_ok) {
-        // the captured sequence stops after the panels the last run needed
-        // (panels past the one whose selection finds the elimination complete
+        // the captured sequence stops after the units the last run needed
+        // (units past the one whose selection finds the elimination complete
         // are empty launches); a matrix that needs more continues below
-        const size_t g_end = eg->panels_needed ? eg->panels_needed : wc;
+        const size_t g_end = eg->panels_needed ? eg->panels_needed : units;
         MELA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        mela_status st = enqueue_panels(0, g_end, g_end == wc);
+        mela_status cst = enqueue_units(0, g_end);
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(s, &graph);
-        if (st != MELA_OK) return st;
+        if (cst != MELA_OK) return cst;
         if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
         eg->graph = graph;
         eg->g_end = g_end;
         MELA_HIP(hipGraphInstantiate(&eg->exec, graph, nullptr, nullptr, 0));
     }
+    auto drop_graph = [&]() {
+        if (eg->exec) (void)hipGraphExecDestroy(eg->exec);
+        if (eg->graph) (void)hipGraphDestroy(eg->graph);
+        eg->exec = nullptr;
+        eg->graph = nullptr;
+    };
     if (eg->exec) {
         MELA_HIP(hipGraphLaunch(eg->exec, s));
-        if ((st = read_meta()) != MELA_OK) return st;
-        if (eg->g_end < wc && (size_t)hm.done_at >= eg->g_end) {
-            // not complete within the captured panels: the rest uncaptured,
+        if ((st = read_meta(blocked)) != MELA_OK) return st;
+        if (eg->g_end < units && done_unit() >= eg->g_end && !(blocked && hb.redo)) {
+            // not complete within the captured units: the rest uncaptured,
             // and the next call captures the whole sequence
-            if ((st = enqueue_panels(eg->g_end, wc, true)) != MELA_OK) return st;
-            if ((st = read_meta()) != MELA_OK) return st;
-            (void)hipGraphExecDestroy(eg->exec);
-            (void)hipGraphDestroy(eg->graph);
-            eg->exec = nullptr;
-            eg->graph = nullptr;
-            eg->panels_needed = wc;
+            if ((st = enqueue_units(eg->g_end, units)) != MELA_OK) return st;
+            if ((st = read_meta(blocked)) != MELA_OK) return st;
+            drop_graph();
+            eg->panels_needed = units;
             eg->seen = 1;
         }
     } else {
+        if ((st = enqueue_units(0, units)) != MELA_OK) return st;
+        if ((st = read_meta(blocked)) != MELA_OK) return st;
+        eg->panels_needed = done_unit() < units ? done_unit() + 1 : units;
+    }
+    if (blocked && hb.redo) {
+        // a block's head missed pivots: the blocked result is void; this
+        // shape takes the panel path from now on
+        drop_graph();
+        eg->blocked_failed = true;
+        eg->seen = 0;
+        eg->panels_needed = 0;
+        eg->g_end = 0;
+        if ((st = copy_rows()) != MELA_OK) return st;
         if ((st = enqueue_panels(0, wc, true)) != MELA_OK) return st;
-        if ((st = read_meta()) != MELA_OK) return st;
+        if ((st = read_meta(false)) != MELA_OK) return st;
         eg->panels_needed = (size_t)hm.done_at < wc ? (size_t)hm.done_at + 1 : wc;
+        ctx->elim_fallback++;
+    } else if (blocked) {
+        ctx->elim_blocked++;
+    } else {
+        ctx->elim_panel++;
     }
-    const int r_dev = hm.r;
-    res->rank = (size_t)r_dev;
-    if (want_pivots && r_dev > 0) {
-        res->pivots.resize((size_t)r_dev);
-        MELA_HIP(hipMemcpyAsync(res->pivots.data(), pivcols, (size_t)r_dev * sizeof(int),
-                                hipMemcpyDeviceToHost, s));
-        MELA_HIP(hipStreamSynchronize(s));
-    }
-    if (Q_out) *Q_out = Q;
-    return MELA_OK;
+    return read_pivots();
 }
 
 mela_status launch_null_packed(mela_ctx *ctx, const uint64_t *Q, size_t ldq, const int *rowmap,
@@ -821,7 +871,7 @@ extern "C" mela_status mela_rank_packed_noalloc(mela_ctx *ctx, const mela_bitmat
     if (!ctx || !rank || !A) return set_error(MELA_ERR_INVALID, "null argument");
     if (A->rows && A->cols && !A->data) return set_error(MELA_ERR_INVALID, "null data in A");
     if (A->cols && A->ld < words64(A->cols)) return set_error(MELA_ERR_INVALID, "ld too small in A");
-    if (variant != MELA_ELIM_GE && variant != MELA_ELIM_M4RI && variant != MELA_ELIM_AUTO)
+    if (variant < MELA_ELIM_GE || variant > MELA_ELIM_PANEL)
         return set_error(MELA_ERR_INVALID, "unknown elimination variant");
     *rank = 0;
     const size_t m = A->rows, n = A->cols;

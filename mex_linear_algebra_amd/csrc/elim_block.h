@@ -1,0 +1,37 @@
+// Blocked elimination (elim_block.hip): block state shared with the host side
+// of eliminate() (elim.hip).
+#pragma once
+
+#include "elim_device.h"
+
+namespace mela {
+
+constexpr int BLK_WORDS = 8;                // packed words per block
+constexpr int BLK_COLS = 64 * BLK_WORDS;    // 512 columns
+
+// Device state of a blocked elimination; the T rows (the block's pivot rows
+// as combinations of its selected rows, BLK_COLS x BLK_WORDS words) follow at
+// BLOCK_T_OFF, then the two matrix-core operands of the trailing update.
+struct BlockState {
+    int rb;        // pivots of the current block
+    int r0;        // pivot rows before the current block
+    int n_new;     // active rows after the current block
+    int redo;      // a block's head missed pivots: the blocked result is void
+    int alive;     // b + 1 after block b's update left a nonzero active-row word past it
+    int done_blk;  // first block whose selection found the elimination complete
+    int active;    // the current block's selection ran (its other launches work)
+    int pad;
+    uint64_t pmb[BLK_WORDS];        // pivot bits of the block's panels
+    uint32_t slot_row[BLK_COLS];    // selection slot -> row id
+};
+constexpr size_t BLOCK_T_OFF = (sizeof(BlockState) + 255) & ~(size_t)255;
+
+// Device bytes of the block state and operands for an m x n elimination.
+size_t block_scratch_bytes(size_t m, size_t n, bool rref);
+// Enqueues blocks b_begin .. b_end-1 (block 0 also initialises the block
+// state; the caller initialises meta, act and, for RREF, zeroes Q).
+mela_status enqueue_blocks(mela_ctx *ctx, uint64_t *Mw, size_t ld, uint32_t *act, ElimMeta *meta,
+                           int *pivcols, uint64_t *Q, void *blk, size_t m, size_t n, bool rref,
+                           size_t b_begin, size_t b_end);
+
+}  // namespace mela

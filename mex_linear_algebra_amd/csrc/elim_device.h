@@ -6,6 +6,24 @@
 #include "common.h"
 
 namespace mela {
+
+// Device state of one single-matrix elimination (elim.hip, elim_block.hip);
+// the pivot columns follow it in the same buffer.
+struct ElimMeta {
+    int n_act;  // active (not yet pivot) rows
+    int r;      // pivot rows found so far
+    int r0;     // pivot rows before the current panel
+    int rho;    // pivots in the current panel
+    uint64_t pm;        // pivot bits (columns 64g + c) of the current panel
+    int finished;       // every active row is zero: the remaining panels have no pivot
+    int last_swept;     // last panel with pivots (its sweep ran), -1 before the first
+    uint64_t pm_prev;   // pm and r0 of the previous panel (RREF: its word update is
+    int r0_prev;        // applied by the next panel's sweep)
+    int done_at;        // first panel whose selection found the elimination complete
+    int sel_row[64];           // selected row ids (selection slot -> row)
+    uint64_t comb[64];  // by pivot bit c: selection slots XORed into pivot row c
+};
+
 namespace dev {
 
 __device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {

@@ -21,6 +21,7 @@
 #include <type_traits>
 
 #include "common.h"
+#include "mfma_device.h"
 #include "transpose_device.h"
 
 namespace mela {
@@ -531,27 +532,14 @@ k_mfma_ring(const uint8_t *__restrict__ A, const uint8_t *__restrict__ B, size_t
 //
 // The kernels above stream operands that were expanded to 4 bits per k in
 // HBM; here the matrix cores are fed from packed bits (1 bit per k), so HBM,
-// the L2 and the L1 carry 4x fewer bytes and no expanded copy exists.
-// The expansion is nearly free because of how e2m1 encodes small values:
-// the nibble bits 0, 1 and 2 alone are 0.5, 1.0 and 2.0.  A 32-bit word x
-// of packed bits therefore gives four fp4 operand dwords by masking:
-//   plane 0: x & 0x11111111  (bit 4t of x   -> nibble t, value 0.5)
-//   plane 1: x & 0x22222222  (bit 4t+1      -> nibble t, value 1.0)
-//   plane 2: x & 0x44444444  (bit 4t+2      -> nibble t, value 2.0)
-//   plane 3: (x >> 1) & 0x44444444 (bit 4t+3 -> nibble t, value 2.0)
-// Each MFMA takes one plane of both operands, so every product in it is
-// w * w for the plane's weight w, and the instruction's E8M0 block scales
-// (2^1, 2^0, 2^-1, 2^-1 on each operand) make every product exactly 1.0:
-// the fp32 accumulator counts the k with A(i,k) = B(k,j) = 1 exactly (< 2^24)
-// and C(i,j) = acc & 1.  A and B use the same map from (lane half, dword,
-// nibble, plane) to k, so which k lands where inside a fragment does not
-// matter.
+// the L2 and the L1 carry 4x fewer bytes and no expanded copy exists.  The
+// expansion is nearly free because of how e2m1 encodes small values
+// (mfma_device.h: four masked planes per packed dword, unit products by the
+// E8M0 block scales, C(i,j) = acc & 1).
 //
-// Stage-tiled packed layout (PK): rows padded to 256, k padded to a whole
-// number of loop trips (PK_DIRECT stages of 256 bits).  Block b of 256 rows,
-// stage s: 8 KiB at (b * nst + s) * 8 KiB, row r's 32 bytes (k bits 256 s ..
-// 256 s + 255) at r * 32; so each wave-wide fragment load reads 1 KiB of
-// consecutive bytes.
+// Stage-tiled packed layout (PK, mfma_device.h): rows padded to 256, k padded
+// to a whole number of loop trips (PK_DIRECT stages of 256 bits); each
+// wave-wide fragment load reads 1 KiB of consecutive bytes.
 constexpr int PK_STAGE = 32;                 // packed bytes of k per row per stage (256 k)
 constexpr int PK_SEG = XROWS * PK_STAGE;     // one stage of one 256-row block: 8 KiB
 #ifndef MELA_PK_DIRECT
@@ -631,38 +619,6 @@ k_transpose_tile_packed(const uint64_t *__restrict__ B, size_t ldb, size_t k, si
     p[1] = make_uint4((uint32_t)o[2], (uint32_t)(o[2] >> 32), (uint32_t)o[3], (uint32_t)(o[3] >> 32));
 }
 
-// Plane j of four packed dwords as an fp4 operand (the upper four dwords of
-// the 8-dword operand are unused by fp4).
-template <int J>
-__device__ __forceinline__ v8i pk_plane(const uint4 &x) {
-    constexpr uint32_t M = J == 3 ? 0x44444444u : (0x11111111u << J);
-    constexpr int SH = J == 3 ? 1 : 0;
-    return v8i{(int)((x.x >> SH) & M), (int)((x.y >> SH) & M), (int)((x.z >> SH) & M),
-               (int)((x.w >> SH) & M), 0, 0, 0, 0};
-}
-
-// E8M0 scale of plane j on each operand: w_j * w_j * 2^(2 e_j) = 1.
-template <int J>
-constexpr int pk_scale() { return J == 0 ? 128 : J == 1 ? 127 : 126; }
-
-// Plane J of one stage for the wave's 4 x 4 MFMA tiles: expand the 8
-// fragments (5 VALU per 4 dwords over the four planes), 16 MFMAs.
-template <int J>
-__device__ __forceinline__ void pk_mm(const uint4 (&ra)[4], const uint4 (&rb)[4], v16f (&acc)[4][4]) {
-    v8i fa[4], fb[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        fa[t] = pk_plane<J>(ra[t]);
-        fb[t] = pk_plane<J>(rb[t]);
-    }
-#pragma unroll
-    for (int tr = 0; tr < 4; ++tr)
-#pragma unroll
-        for (int tc = 0; tc < 4; ++tc)
-            acc[tr][tc] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
-                fa[tr], fb[tc], acc[tr][tc], 4, 4, 0, pk_scale<J>(), 0, pk_scale<J>());
-}
-
 // k_mfma_direct: 256 x 256 tiles, 4 waves in a 2 x 2 grid, each 128 x 128
 // (4 x 4 MFMA tiles, 256 accumulators), one workgroup per CU, the fragments
 // loaded straight into registers.  In the PK layout lane l's
@@ -716,10 +672,10 @@ k_mfma_direct(const uint8_t *__restrict__ A, const uint8_t *__restrict__ B, int 
     for (int st = 0; st < nst; st += D) {
 #pragma unroll
         for (int j = 0; j < D; ++j) {
-            pk_mm<0>(fa[j], fb[j], acc);
-            pk_mm<1>(fa[j], fb[j], acc);
-            pk_mm<2>(fa[j], fb[j], acc);
-            pk_mm<3>(fa[j], fb[j], acc);
+            pk::mm<0>(fa[j], fb[j], acc);
+            pk::mm<1>(fa[j], fb[j], acc);
+            pk::mm<2>(fa[j], fb[j], acc);
+            pk::mm<3>(fa[j], fb[j], acc);
             __builtin_amdgcn_sched_barrier(0);
             load(st + j + D, fa[j], fb[j]);
             __builtin_amdgcn_sched_barrier(0);

@@ -104,7 +104,7 @@ def _check_elimination_full(ctx, mela, orc, H, m, n, want_rank):
     f = n - r
     Rfree = _unpack_rows(M[:r], n)[:, free]  # R(1:r, free)
     vecs = None
-    for variant in (mela.ELIM_GE, mela.ELIM_M4RI, mela.ELIM_AUTO):
+    for variant in (mela.ELIM_GE, mela.ELIM_M4RI, mela.ELIM_AUTO, mela.ELIM_PANEL):
         print(f"[C2 {m}x{n}] variant={variant}", flush=True)
         assert ctx.rank_packed(H, variant) == r
         Rm, piv = ctx.rref_packed(H, variant)
