@@ -673,10 +673,10 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
         return enqueue_blocks(ctx, Mw, ld, act, meta, pivcols, Q, pBlk, m, n, rref, b_begin, b_end);
     };
     ElimMeta hm;
-    BlockState hb;
+    BlockShared hb;
     auto read_meta = [&](bool blk) -> mela_status {
         MELA_HIP(hipMemcpyAsync(&hm, meta, sizeof(ElimMeta), hipMemcpyDeviceToHost, s));
-        if (blk) MELA_HIP(hipMemcpyAsync(&hb, pBlk, sizeof(BlockState), hipMemcpyDeviceToHost, s));
+        if (blk) MELA_HIP(hipMemcpyAsync(&hb, pBlk, sizeof(BlockShared), hipMemcpyDeviceToHost, s));
         MELA_HIP(hipStreamSynchronize(s));
         return MELA_OK;
     };

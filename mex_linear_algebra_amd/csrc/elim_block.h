@@ -8,23 +8,32 @@ namespace mela {
 
 constexpr int BLK_WORDS = 8;                // packed words per block
 constexpr int BLK_COLS = 64 * BLK_WORDS;    // 512 columns
+constexpr int BLK_HEAD = 576;               // head rows a block selection eliminates
 
-// Device state of a blocked elimination; the T rows (the block's pivot rows
-// as combinations of its selected rows, BLK_COLS x BLK_WORDS words) follow at
-// BLOCK_T_OFF, then the two matrix-core operands of the trailing update.
-struct BlockState {
-    int rb;        // pivots of the current block
-    int r0;        // pivot rows before the current block
-    int n_new;     // active rows after the current block
-    int redo;      // a block's head missed pivots: the blocked result is void
-    int alive;     // b + 1 after block b's update left a nonzero active-row word past it
-    int done_blk;  // first block whose selection found the elimination complete
-    int active;    // the current block's selection ran (its other launches work)
+// State of one blocked elimination that outlives a block.
+struct BlockShared {
+    int redo;          // a block's head missed pivots (or a wait timed out): the result is void
+    int alive;         // b + 1 after block b's update left a nonzero active-row word past it
+    int done_blk;      // first block whose selection found the elimination complete
     int pad;
+};
+
+// State of one block; two of them, by block parity, since the selection of
+// block b+1 runs while the update of block b reads block b's state.
+struct BlockState {
+    int rb;        // pivots of the block
+    int r0;        // pivot rows before the block
+    int n_new;     // active rows after the block
+    int active;    // the block's selection ran (its other launches work)
+    int hn;        // head rows (entries of headmap)
+    int pad[3];
     uint64_t pmb[BLK_WORDS];        // pivot bits of the block's panels
     uint32_t slot_row[BLK_COLS];    // selection slot -> row id
+    uint32_t headmap[BLK_HEAD];     // active-list entries 0 .. hn-1 once the block's
+                                    // selected rows have left the list
 };
-constexpr size_t BLOCK_T_OFF = (sizeof(BlockState) + 255) & ~(size_t)255;
+constexpr size_t BLOCK_STATE_OFF = 256;
+constexpr size_t BLOCK_T_OFF = (BLOCK_STATE_OFF + 2 * sizeof(BlockState) + 255) & ~(size_t)255;
 
 // Device bytes of the block state and operands for an m x n elimination.
 size_t block_scratch_bytes(size_t m, size_t n, bool rref);

@@ -9,31 +9,38 @@
 // in one matrix-core product, which moves 8x fewer bytes than eight panel
 // sweeps and replaces their table lookups by MFMAs.
 //
-// Block b (words w0 = 8b .. w1), three launches:
-//   k_block_select  one workgroup: the head of the active list (its first
-//                   HMAX rows, block words only) is copied to LDS and
+// Block b (words w0 = 8b .. w1):
+//   block selection  one workgroup: the head of the active list (its first
+//                   BLK_HEAD rows, block words only) is copied to LDS and
 //                   eliminated panel by panel (the panel selection of
 //                   elim_device.h, pivot rows by 4-bit tables, table sweeps
-//                   of the other head rows).  Every head row also carries
-//                   512 "slot" bits: selected row s gets bit s, and the sweeps
-//                   XOR them along, so each pivot row ends up as a known
-//                   combination T of the selected rows as they were when the
-//                   block started.  The block's pivot rows (block words, RREF
-//                   on the block's pivot columns) go to Q, T to a side
-//                   buffer, the selected rows leave the active list.
+//                   of the other head rows; the next panel's selection runs
+//                   on one wave while the others finish the sweep).  Every
+//                   head row also carries 512 "slot" bits: selected row s
+//                   gets bit s, and the sweeps XOR them along, so each pivot
+//                   row ends up as a known combination T of the selected rows
+//                   as they were when the block started.  The block's pivot
+//                   rows (block words, RREF on the block's pivot columns) go
+//                   to Q, T to a side buffer, and the active list without the
+//                   selected rows to the block state (its head entries).
 //   k_block_pform   the pivot rows' words past the block, P = T x S, with S
 //                   the selected rows' words (4-bit tables over the slots),
 //                   into Q; P~ (the block's pivot rows indexed by block
 //                   column, zero rows at non-pivot columns) transposed into
-//                   the matrix-core operand layout; and the remaining rows'
-//                   block words re-laid as the other operand.
-//   k_block_tu      every active row (and, for RREF, every earlier pivot
+//                   the matrix-core operand layout; the remaining rows' block
+//                   words re-laid as the other operand; the new head of the
+//                   active list written back.
+//   trailing update every active row (and, for RREF, every earlier pivot
 //                   row) ^= (its block words) x P~ on the matrix cores (fp4,
 //                   K = 512).  Because P is in RREF on the block's pivot
 //                   columns, a row's coefficients are its own bits there.
 //                   An active row whose block words do not vanish had a
 //                   pivot the head missed: the call then discards the
 //                   blocked result and runs the panel path (elim.hip).
+// k_block_step runs the trailing update of block b on all workgroups but
+// one, which selects block b+1 meanwhile: the update tiles that hold the head
+// rows' words of block b+1 run first, in a launch of their own, so the
+// selection, one workgroup's latency-bound work, hides behind the product.
 #include <climits>
 
 #include "elim_device.h"
@@ -48,7 +55,7 @@ using dev::ffs64;
 using dev::popc64;
 
 constexpr int BS_THREADS = 1024;
-constexpr int HMAX = 576;           // head rows of a block selection (512 + 64)
+constexpr int HMAX = BLK_HEAD;
 constexpr int BRW = 2 * BLK_WORDS;  // head row: block words, then 512 slot bits
 constexpr int PF_TW = 4;            // words per k_block_pform workgroup (one per wave)
 
@@ -180,38 +187,39 @@ __device__ __forceinline__ int block_pivot_index(const uint64_t *pmb, int bc) {
     return i;
 }
 
-__global__ void k_block_init(BlockState *bs) {
+__global__ void k_block_init(BlockShared *sh, BlockState *bst) {
     if (threadIdx.x == 0) {
-        bs->rb = 0;
-        bs->r0 = 0;
-        bs->n_new = 0;
-        bs->redo = 0;
-        bs->alive = 0;
-        bs->done_blk = INT_MAX;
-        bs->active = 0;
+        sh->redo = 0;
+        sh->alive = 0;
+        sh->done_blk = INT_MAX;
+        bst[0].active = 0;
+        bst[1].active = 0;
     }
 }
 
-__global__ void __launch_bounds__(BS_THREADS)
-k_block_select(const uint64_t *__restrict__ M, size_t ld, uint32_t *__restrict__ act,
-               ElimMeta *__restrict__ meta, int *__restrict__ pivcols, BlockState *__restrict__ bs,
-               uint64_t *__restrict__ Tout, uint64_t *__restrict__ Q, size_t ldq, int b, int wc,
-               int rmax) {
-    __shared__ BlockLDS L;
+// Selection of block b into bs (all BS_THREADS threads of the workgroup).
+__device__ __forceinline__ void block_select(BlockLDS &L, const uint64_t *__restrict__ M, size_t ld,
+                             const uint32_t *__restrict__ act, ElimMeta *__restrict__ meta,
+                             int *__restrict__ pivcols, BlockShared *__restrict__ sh,
+                             BlockState *__restrict__ bs, uint64_t *__restrict__ Tout,
+                             uint64_t *__restrict__ Q, size_t ldq, int b, int wc, int rmax) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int w0 = b * BLK_WORDS;
     const int nbw = min(BLK_WORDS, wc - w0);
     const int n_act = meta->n_act, r0 = meta->r;
-    // the previous block's update tags alive with b when an active row kept a
-    // nonzero word past that block; without the tag every active row is zero
-    const bool finished = n_act == 0 || r0 >= rmax || (b > 0 && bs->alive != b);
-    if (bs->redo || finished) {
+    // the update of block b' tags alive with b'+1 when an active row kept a
+    // nonzero word past block b'.  The update of block b-1 may still be
+    // running; that of block b-2 is done, so a tag below b-1 means that every
+    // active row was zero after block b-2.
+    const bool finished = n_act == 0 || r0 >= rmax || (b >= 2 && sh->alive < b - 1);
+    if (sh->redo || finished) {
         if (tid == 0) {
             bs->rb = 0;
             bs->r0 = r0;
             bs->n_new = n_act;
+            bs->hn = 0;
             bs->active = 0;
-            if (finished && !bs->redo && bs->done_blk > b) bs->done_blk = b;
+            if (finished && !sh->redo && sh->done_blk > b) sh->done_blk = b;
         }
         return;
     }
@@ -408,7 +416,8 @@ k_block_select(const uint64_t *__restrict__ M, size_t ld, uint32_t *__restrict__
     }
     // the selected rows leave the active list: the unselected entries of
     // positions n_new .. n_act-1 (in order) fill the selected positions below
-    // n_new (in order); positions >= Hn were never selected
+    // n_new (in order); positions >= Hn were never selected.  The new head
+    // (entries 0 .. Hn-1) goes to bs->headmap; k_block_pform writes it back.
     const int n_new = n_act - rb;
     {
         const int pos = n_new + tid;
@@ -425,22 +434,33 @@ k_block_select(const uint64_t *__restrict__ M, size_t ld, uint32_t *__restrict__
     {
         const bool hole = tid < Hn && tid < n_new && L.hslot[tid] >= 0;
         const int k = wg_prefix(hole, L.wcnt, nullptr);
-        if (hole) act[tid] = L.movers[k];
+        if (tid < Hn) bs->headmap[tid] = hole ? L.movers[k] : L.hrow[tid];
     }
     if (tid < BLK_WORDS) bs->pmb[tid] = L.pmb[tid];
     if (tid == 0) {
         bs->rb = rb;
         bs->r0 = r0;
         bs->n_new = n_new;
+        bs->hn = Hn;
         bs->active = 1;
         meta->r = r0 + rb;
         meta->n_act = n_new;
     }
 }
 
+// Selection of block 0 (the later blocks' selections run inside k_block_step).
+__global__ void __launch_bounds__(BS_THREADS)
+k_block_select(const uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
+               ElimMeta *__restrict__ meta, int *__restrict__ pivcols, BlockShared *__restrict__ sh,
+               BlockState *__restrict__ bst, uint64_t *__restrict__ Tout, uint64_t *__restrict__ Q,
+               size_t ldq, int wc, int rmax) {
+    __shared__ BlockLDS L;
+    block_select(L, M, ld, act, meta, pivcols, sh, bst, Tout, Q, ldq, 0, wc, rmax);
+}
+
 // Workgroups 0 .. 8*ntw-1: pivot rows of the block's panel q = blockIdx % 8
 // over words wa .. wa+3 (wa = w0 + 4 (blockIdx / 8)); one wave per word, lane
-// = pivot bit.  Words inside the block come from Q (k_block_select wrote
+// = pivot bit.  Words inside the block come from Q (the selection wrote
 // them), words past it are P = T x S with 4-bit tables of the selected rows'
 // words (table t: slots 4t .. 4t+3, laid out [table][entry][word] so that a
 // wave's lookups, one table and word with 16 possible entries, hit distinct
@@ -450,27 +470,32 @@ k_block_select(const uint64_t *__restrict__ M, size_t ld, uint32_t *__restrict__
 // Workgroups from 8*ntw on: the other operand, the block words of the rows
 // the update covers (active rows in list order, then for RREF the pivot rows
 // found before the block), 32 bytes per (row, stage), zero rows up to a
-// multiple of 256.
+// multiple of 256; the first of them also writes the block's new head of the
+// active list into act (the others read it from bs->headmap meanwhile).
 __global__ void __launch_bounds__(256)
-k_block_pform(const uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
-              const BlockState *__restrict__ bs, const uint64_t *__restrict__ Tin,
-              uint64_t *__restrict__ Q, size_t ldq, uint8_t *__restrict__ Xa,
-              uint8_t *__restrict__ Bt, int b, int wc, int ntw, int rref, int xa_blocks) {
+k_block_pform(const uint64_t *__restrict__ M, size_t ld, uint32_t *__restrict__ act,
+              const BlockShared *__restrict__ sh, const BlockState *__restrict__ bs,
+              const uint64_t *__restrict__ Tin, uint64_t *__restrict__ Q, size_t ldq,
+              uint8_t *__restrict__ Xa, uint8_t *__restrict__ Bt, int b, int wc, int ntw, int rref,
+              int xa_blocks) {
     __shared__ uint64_t Tbl[BLK_COLS / 4][16][PF_TW];
-    if (!bs->active) return;
+    if (sh->redo || !bs->active) return;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int w0 = b * BLK_WORDS, w1 = min(w0 + BLK_WORDS, wc);
     const int r0 = bs->r0;
     if ((int)blockIdx.x >= 8 * ntw) {
-        const int n_new = bs->n_new;
+        const int n_new = bs->n_new, hn = bs->hn;
         const int rows = n_new + (rref ? r0 : 0);
         const int rows_pad = (rows + 255) & ~255;
+        if ((int)blockIdx.x == 8 * ntw)
+            for (int i = tid; i < hn; i += 256) act[i] = bs->headmap[i];
         for (int idx = ((int)blockIdx.x - 8 * ntw) * 256 + tid; idx < 2 * rows_pad;
              idx += xa_blocks * 256) {
             const int i = idx >> 1, s = idx & 1;
             uint64_t v[4] = {0ull, 0ull, 0ull, 0ull};
             if (i < rows) {
-                const uint64_t *src = i < n_new ? M + (size_t)act[i] * ld : Q + (size_t)(i - n_new) * ldq;
+                const uint32_t a = i < hn ? bs->headmap[i] : act[i];
+                const uint64_t *src = i < n_new ? M + (size_t)a * ld : Q + (size_t)(i - n_new) * ldq;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int w = w0 + 4 * s + q;
@@ -532,139 +557,162 @@ k_block_pform(const uint64_t *__restrict__ M, size_t ld, const uint32_t *__restr
                                   8 * (q & 3)) = o;
 }
 
-// Trailing update on the matrix cores: for every row of the update (active
-// rows in list order, then for RREF the pivot rows found before the block),
-// words w0 .. wc-1 ^= (row's block words) x P~, K = 512.  Persistent: the
-// workgroups walk the 256-row x 256-column tiles (row tile fastest).  Each of
-// the 4 waves computes 128 x 128 (4 x 4 MFMA tiles), the result bits are
-// gathered in LDS, and thread t then updates tile row t's 4 words.  Active
-// rows: words inside the block are only checked (they must vanish), words
-// past it are written; pivot rows: every word.  The words a thread updates
-// are loaded before the tile's MFMAs and the next tile's operand fragments
-// before its epilogue, so both loads wait behind work instead of in front of
-// it.
-__global__ void __launch_bounds__(256, 1)
-k_block_tu(const uint8_t *__restrict__ Xa, const uint8_t *__restrict__ Bt, uint64_t *__restrict__ M,
-           size_t ld, const uint32_t *__restrict__ act, uint64_t *__restrict__ Q, size_t ldq,
-           BlockState *__restrict__ bs, int b, int wc, int rref) {
-    __shared__ uint64_t Ct[256][4];
-    if (!bs->active) return;
+// Update tiles of block b that hold the head rows' words of block b+1: row
+// tiles 0..2 (active rows 0..767 include the BLK_HEAD head rows) x column
+// tiles 2 and 3 (words w0+8 .. w0+15).  They run first, in a launch of their
+// own, so that the selection of block b+1 can start with the next launch.
+__device__ __forceinline__ int head_rows_tiles(int RT) { return min(3, RT); }
+__device__ __forceinline__ int head_col_tiles(int CT, bool select_next) {
+    return select_next && CT > 2 ? min(4, CT) - 2 : 0;
+}
+constexpr int HEAD_TILES_MAX = 6;
+
+// Trailing update of block b on the matrix cores: for every row of the update
+// (active rows in list order, then for RREF the pivot rows found before the
+// block), words w0 .. wc-1 ^= (row's block words) x P~, K = 512.
+// head = 1: workgroup i computes head-slice tile i only.  head = 0: workgroup
+// 0 selects block b+1 (if any) while workgroups 1.. walk every other tile
+// (persistent, row tile fastest).  16 waves per workgroup, each 64 x 64 of
+// the 256 x 256 tile (2 x 2 MFMA tiles; four waves per SIMD, so one wave's
+// epilogue runs beside another's MFMAs).  Accumulators start at 2^23: every
+// count (< 2^10) then sits in the low mantissa bits, so bit 0 of the float is
+// its parity; each lane packs its 64 parities (register r of lane l: row
+// 8(r/4) + 4(l/32) + r%4, column l%32 of its tile) into one word, one
+// v_alignbit per bit, and the in-wave 64 x 64 bit transpose gives lane j the
+// 64 lanes' bit j, i.e. 32 columns of one row in each half; lanes j and j^16
+// (the two column tiles) combine their halves into the row's word.  Thread t
+// then updates word t%4 of tile row t/4: active rows' words inside the block
+// are only checked (they must vanish), past it written; pivot rows' every
+// word.  The word a thread updates is loaded before the tile's MFMAs, its
+// row's active-list entry one tile ahead.
+__global__ void __launch_bounds__(BS_THREADS)
+k_block_step(const uint8_t *__restrict__ Xa, const uint8_t *__restrict__ Bt, uint64_t *__restrict__ M,
+             size_t ld, const uint32_t *__restrict__ act, uint64_t *__restrict__ Q, size_t ldq,
+             ElimMeta *__restrict__ meta, int *__restrict__ pivcols, BlockShared *__restrict__ sh,
+             BlockState *__restrict__ bst, uint64_t *__restrict__ Tout, int b, int nblk, int wc,
+             int rref, int rmax, int head) {
+    __shared__ BlockLDS L;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const BlockState *bs = bst + (b & 1);
+    const bool select_next = b + 1 < nblk;
+    const bool active = bs->active != 0;
     const int w0 = b * BLK_WORDS, w1 = min(w0 + BLK_WORDS, wc);
     const int n_new = bs->n_new;
-    const int rows = n_new + (rref ? bs->r0 : 0);
+    const int rows = active ? n_new + (rref ? bs->r0 : 0) : 0;
     const int RT = (rows + 255) >> 8, CT = (wc - w0 + 3) >> 2;
-    const int ntiles = RT * CT;
-    constexpr int WT = 128;
-    const int wr = wave >> 1, wcol = wave & 1;
+    const int hrt = head_rows_tiles(RT), hct = head_col_tiles(CT, select_next);
+    if (!head && blockIdx.x == 0) {
+        if (select_next)
+            block_select(L, M, ld, act, meta, pivcols, sh, bst + ((b + 1) & 1), Tout, Q, ldq, b + 1,
+                         wc, rmax);
+        return;
+    }
+    if (!active) return;
+    uint64_t(*Ct)[4] = reinterpret_cast<uint64_t(*)[4]>(&L.H[0][0]);
+    const int ntiles = head ? min((int)blockIdx.x + 1, hrt * hct) : RT * CT;
+    const int G = head ? 1 : (int)gridDim.x - 1;
+    const int t_first = head ? (int)blockIdx.x : (int)blockIdx.x - 1;
+    const int wr = wave >> 2, wcl = wave & 3;
     const int half = lane >> 5, l32 = lane & 31;
     constexpr int SEG16 = pk::SEG / 16;
-    uint4 fa[2][4], fb[2][4];
-    auto load_frags = [&](int t) {
-        const int rt = t % RT, ct = t / RT;
-        const uint4 *pa = reinterpret_cast<const uint4 *>(Xa + (size_t)rt * 2 * pk::SEG) +
-                          (wr * WT + l32) * 2 + half;
-        const uint4 *pb = reinterpret_cast<const uint4 *>(Bt + (size_t)ct * 2 * pk::SEG) +
-                          (wcol * WT + l32) * 2 + half;
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                fa[s][k] = pa[s * SEG16 + k * 64];
-                fb[s][k] = pb[s * SEG16 + k * 64];
-            }
+    const int trow = tid >> 2, tk = tid & 3;  // the word this thread updates
+    // tile t: the t-th head-slice tile (head launch), else row tile t % RT of
+    // column tile t / RT
+    auto tile_rc = [&](int t, int &rt, int &ct) {
+        if (head) {
+            rt = t % hrt;
+            ct = 2 + t / hrt;
+        } else {
+            rt = t % RT;
+            ct = t / RT;
+        }
     };
-    // active-list entry of this thread's row in tile t (read one tile ahead)
+    auto is_head = [&](int rt, int ct) { return !head && rt < hrt && ct >= 2 && ct < 2 + hct; };
     auto act_of = [&](int t) -> uint32_t {
-        const int row = (t % RT) * 256 + tid;
-        return t < ntiles && row < n_new ? act[row] : 0u;
+        if (t >= ntiles) return 0u;
+        int rt, ct;
+        tile_rc(t, rt, ct);
+        const int row = rt * 256 + trow;
+        return row < n_new ? act[row] : 0u;
     };
     bool redo = false, alive = false;
-    int t = blockIdx.x;
-    if (t < ntiles) load_frags(t);
-    uint32_t arow = act_of(t);
-    for (; t < ntiles; t += gridDim.x) {
-        const int rt = t % RT, ct = t / RT;
-        const int row = rt * 256 + tid;
-        const int wb = w0 + 4 * ct;
+    uint32_t arow = act_of(t_first);
+    for (int t = t_first; t < ntiles; t += G) {
+        int rt, ct;
+        tile_rc(t, rt, ct);
+        if (is_head(rt, ct)) {
+            arow = act_of(t + G);
+            continue;
+        }
+        const int row = rt * 256 + trow;
+        const int w = w0 + 4 * ct + tk;
         uint64_t *dst = nullptr;
-        uint64_t old[4] = {0ull, 0ull, 0ull, 0ull};
-        if (row < rows) {
-            dst = row < n_new ? M + (size_t)arow * ld : Q + (size_t)(row - n_new) * ldq;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) old[k] = wb + k < wc ? dst[wb + k] : 0ull;
+        uint64_t old = 0;
+        if (row < rows && w < wc) {
+            dst = (row < n_new ? M + (size_t)arow * ld : Q + (size_t)(row - n_new) * ldq) + w;
+            old = *dst;
         }
-        // accumulators start at 2^23: every count (< 2^10) then sits in the
-        // low mantissa bits, so bit 0 of the float is the count's parity
-        pk::v16f acc[4][4];
+        const uint4 *pa = reinterpret_cast<const uint4 *>(Xa + (size_t)rt * 2 * pk::SEG) +
+                          (wr * 64 + l32) * 2 + half;
+        const uint4 *pb = reinterpret_cast<const uint4 *>(Bt + (size_t)ct * 2 * pk::SEG) +
+                          (wcl * 64 + l32) * 2 + half;
+        arow = act_of(t + G);
+        pk::v16f acc[2][2];
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int x = 0; x < 2; ++x)
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
+            for (int y = 0; y < 2; ++y)
 #pragma unroll
-                for (int r = 0; r < 16; ++r) acc[a][c][r] = 8388608.0f;
+                for (int r = 0; r < 16; ++r) acc[x][y][r] = 8388608.0f;
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            pk::mm<0>(fa[s], fb[s], acc);
-            pk::mm<1>(fa[s], fb[s], acc);
-            pk::mm<2>(fa[s], fb[s], acc);
-            pk::mm<3>(fa[s], fb[s], acc);
-        }
-        if (t + (int)gridDim.x < ntiles) load_frags(t + gridDim.x);
-        arow = act_of(t + gridDim.x);
-        // Result bits to row words.  For MFMA tile rows tr0, tr0+1 and columns
-        // tc0, tc0+1 each lane packs its 64 parities (register r of lane l:
-        // row 8(r/4) + 4(l/32) + r%4, column l%32 of its tile) into one
-        // 64-bit word, one v_alignbit per bit; the in-wave 64 x 64 bit
-        // transpose then gives lane j the 64 lanes' bit j, i.e. 32 columns of
-        // one row in each half, and lanes j, j^16 (the two column tiles)
-        // combine their halves into one 64-column row word each.
+        for (int st = 0; st < 2; ++st) {
+            uint4 fa[2], fb[2];
 #pragma unroll
-        for (int trp = 0; trp < 2; ++trp)
-#pragma unroll
-            for (int tcp = 0; tcp < 2; ++tcp) {
-                uint32_t xh[2] = {0u, 0u};
-#pragma unroll
-                for (int q = 0; q < 2; ++q)
-#pragma unroll
-                    for (int h = 0; h < 2; ++h)
-#pragma unroll
-                        for (int r = 0; r < 16; ++r)
-                            xh[q] = __builtin_amdgcn_alignbit(
-                                __float_as_uint(acc[2 * trp + q][2 * tcp + h][r]), xh[q], 1);
-                const uint64_t y = dev::transpose64(((uint64_t)xh[1] << 32) | xh[0], lane);
-                const uint64_t yp = dev::shfl_xor64(y, 16);
-                // lane j: tile row tr0 + j/32, column tile tc0 + (j/16)%2, r = j%16
-                const int tri = lane >> 5, tci = (lane >> 4) & 1, r = lane & 15;
-                const uint64_t word = tci == 0 ? ((y & 0xffffffffull) | (yp << 32))
-                                               : ((yp >> 32) | (y & 0xffffffff00000000ull));
-                const int row = wr * WT + (2 * trp + tri) * 32 + 8 * (r >> 2) + (r & 3) + 4 * tci;
-                Ct[row][wcol * 2 + tcp] = word;
+            for (int k = 0; k < 2; ++k) {
+                fa[k] = pa[st * SEG16 + k * 64];
+                fb[k] = pb[st * SEG16 + k * 64];
             }
+            pk::mm<0>(fa, fb, acc);
+            pk::mm<1>(fa, fb, acc);
+            pk::mm<2>(fa, fb, acc);
+            pk::mm<3>(fa, fb, acc);
+        }
+        {
+            uint32_t xh[2] = {0u, 0u};
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r)
+                        xh[q] = __builtin_amdgcn_alignbit(__float_as_uint(acc[q][h][r]), xh[q], 1);
+            const uint64_t y = dev::transpose64(((uint64_t)xh[1] << 32) | xh[0], lane);
+            const uint64_t yp = dev::shfl_xor64(y, 16);
+            // lane j: tile row j/32, column tile (j/16)%2, r = j%16
+            const int tri = lane >> 5, tci = (lane >> 4) & 1, r = lane & 15;
+            const uint64_t word = tci == 0 ? ((y & 0xffffffffull) | (yp << 32))
+                                           : ((yp >> 32) | (y & 0xffffffff00000000ull));
+            Ct[wr * 64 + tri * 32 + 8 * (r >> 2) + (r & 3) + 4 * tci][wcl] = word;
+        }
         lds_barrier();
         if (dst) {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int w = wb + k;
-                if (w >= wc) break;
-                const uint64_t bits = Ct[tid][k];
-                const uint64_t nv = old[k] ^ bits;
-                if (row < n_new) {
-                    if (w < w1) {
-                        redo |= nv != 0ull;
-                    } else {
-                        alive |= nv != 0ull;
-                        if (bits) dst[w] = nv;
-                    }
-                } else if (bits) {
-                    dst[w] = nv;
+            const uint64_t bits = Ct[trow][tk];
+            const uint64_t nv = old ^ bits;
+            if (row < n_new) {
+                if (w < w1) {
+                    redo |= nv != 0ull;
+                } else {
+                    alive |= nv != 0ull;
+                    if (bits) *dst = nv;
                 }
+            } else if (bits) {
+                *dst = nv;
             }
         }
         lds_barrier();
     }
-    if (__syncthreads_or(redo) && tid == 0) bs->redo = 1;
-    if (__syncthreads_or(alive) && tid == 0) bs->alive = b + 1;
+    if (__syncthreads_or(redo) && tid == 0) sh->redo = 1;
+    if (__syncthreads_or(alive) && tid == 0) atomicMax(&sh->alive, b + 1);
 }
 
 }  // namespace
@@ -682,27 +730,42 @@ mela_status enqueue_blocks(mela_ctx *ctx, uint64_t *Mw, size_t ld, uint32_t *act
                            size_t b_begin, size_t b_end) {
     hipStream_t s = ctx->stream;
     const size_t wc = words64(n), rmax = m < n ? m : n;
+    const size_t nblk = (wc + BLK_WORDS - 1) / BLK_WORDS;
     char *base = (char *)blk;
-    BlockState *bs = (BlockState *)base;
+    BlockShared *sh = (BlockShared *)base;
+    BlockState *bst = (BlockState *)(base + BLOCK_STATE_OFF);
     uint64_t *T = (uint64_t *)(base + BLOCK_T_OFF);
     const size_t rows = m + (rref ? rmax : 0);
     uint8_t *Xa = (uint8_t *)(base + BLOCK_T_OFF + (size_t)BLK_COLS * BLK_WORDS * 8);
     uint8_t *Bt = Xa + (rows + 255) / 256 * 2 * pk::SEG;
-    if (b_begin == 0) hipLaunchKernelGGL(k_block_init, dim3(1), dim3(64), 0, s, bs);
     const int cu = ctx->cu_count > 0 ? ctx->cu_count : 256;
     const int xa_blocks = (int)std::min<size_t>((2 * ((rows + 255) & ~(size_t)255) + 255) / 256, 1024);
-    for (size_t b = b_begin; b < b_end; ++b) {
+    auto pform = [&](size_t b) {
         const size_t w0 = b * BLK_WORDS;
         const int ntw = (int)((wc - w0 + PF_TW - 1) / PF_TW);
-        hipLaunchKernelGGL(k_block_select, dim3(1), dim3(BS_THREADS), 0, s, (const uint64_t *)Mw, ld, act,
-                           meta, pivcols, bs, T, Q, ld, (int)b, (int)wc, (int)rmax);
         hipLaunchKernelGGL(k_block_pform, dim3((unsigned)(8 * ntw + xa_blocks)), dim3(256), 0, s,
-                           (const uint64_t *)Mw, ld, (const uint32_t *)act, (const BlockState *)bs,
-                           (const uint64_t *)T, Q, ld, Xa, Bt, (int)b, (int)wc, ntw, rref ? 1 : 0,
-                           xa_blocks);
-        hipLaunchKernelGGL(k_block_tu, dim3((unsigned)cu), dim3(256), 0, s, (const uint8_t *)Xa,
-                           (const uint8_t *)Bt, Mw, ld, (const uint32_t *)act, Q, ld, bs, (int)b,
-                           (int)wc, rref ? 1 : 0);
+                           (const uint64_t *)Mw, ld, act, (const BlockShared *)sh,
+                           (const BlockState *)(bst + (b & 1)), (const uint64_t *)T, Q, ld, Xa, Bt,
+                           (int)b, (int)wc, ntw, rref ? 1 : 0, xa_blocks);
+    };
+    if (b_begin == 0) {
+        hipLaunchKernelGGL(k_block_init, dim3(1), dim3(64), 0, s, sh, bst);
+        hipLaunchKernelGGL(k_block_select, dim3(1), dim3(BS_THREADS), 0, s, (const uint64_t *)Mw, ld,
+                           (const uint32_t *)act, meta, pivcols, sh, bst, T, Q, ld, (int)wc, (int)rmax);
+        pform(0);
+    }
+    for (size_t b = b_begin; b < b_end; ++b) {
+        // the head slice of block b's update, then the rest of it beside the
+        // selection of block b+1
+        if (b + 1 < nblk)
+            hipLaunchKernelGGL(k_block_step, dim3(HEAD_TILES_MAX), dim3(BS_THREADS), 0, s,
+                               (const uint8_t *)Xa, (const uint8_t *)Bt, Mw, ld, (const uint32_t *)act, Q,
+                               ld, meta, pivcols, sh, bst, T, (int)b, (int)nblk, (int)wc, rref ? 1 : 0,
+                               (int)rmax, 1);
+        hipLaunchKernelGGL(k_block_step, dim3((unsigned)cu), dim3(BS_THREADS), 0, s, (const uint8_t *)Xa,
+                           (const uint8_t *)Bt, Mw, ld, (const uint32_t *)act, Q, ld, meta, pivcols, sh, bst,
+                           T, (int)b, (int)nblk, (int)wc, rref ? 1 : 0, (int)rmax, 0);
+        if (b + 1 < nblk) pform(b + 1);
     }
     MELA_HIP(hipGetLastError());
     return MELA_OK;

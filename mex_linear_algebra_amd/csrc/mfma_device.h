@@ -37,20 +37,20 @@ __device__ __forceinline__ v8i plane(const uint4 &x) {
 template <int J>
 constexpr int scale() { return J == 0 ? 128 : J == 1 ? 127 : 126; }
 
-// Plane J of one 256-bit k stage for a wave's 4 x 4 tiles of 32 x 32: expand
-// the 8 fragments (5 VALU per 4 dwords over the four planes), 16 MFMAs.
-template <int J>
-__device__ __forceinline__ void mm(const uint4 (&ra)[4], const uint4 (&rb)[4], v16f (&acc)[4][4]) {
-    v8i fa[4], fb[4];
+// Plane J of one 256-bit k stage for a wave's TA x TB tiles of 32 x 32:
+// expand the fragments (5 VALU per 4 dwords over the four planes), TA * TB
+// MFMAs.
+template <int J, int TA, int TB>
+__device__ __forceinline__ void mm(const uint4 (&ra)[TA], const uint4 (&rb)[TB], v16f (&acc)[TA][TB]) {
+    v8i fa[TA], fb[TB];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        fa[t] = plane<J>(ra[t]);
-        fb[t] = plane<J>(rb[t]);
-    }
+    for (int t = 0; t < TA; ++t) fa[t] = plane<J>(ra[t]);
 #pragma unroll
-    for (int tr = 0; tr < 4; ++tr)
+    for (int t = 0; t < TB; ++t) fb[t] = plane<J>(rb[t]);
 #pragma unroll
-        for (int tc = 0; tc < 4; ++tc)
+    for (int tr = 0; tr < TA; ++tr)
+#pragma unroll
+        for (int tc = 0; tc < TB; ++tc)
             acc[tr][tc] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
                 fa[tr], fb[tc], acc[tr][tc], 4, 4, 0, scale<J>(), 0, scale<J>());
 }
