@@ -255,6 +255,26 @@ __device__ __forceinline__ void block_select(BlockLDS &L, const uint64_t *__rest
 
     int n_cand = Hn, rb = 0;
     auto rowfn = [&](int q) { return HeadRow{L.H, q}; };
+    // a sparse matrix shows in its head: when fewer than a quarter of the
+    // head rows have a nonzero word in block 0 while other rows remain, the
+    // head would miss pivots; give up at once (the call takes the panel path).
+    // Dense matrices of any rank have about half of their rows nonzero there
+    // or more; column-weight-3 matrices a few percent.
+    if (b == 0 && Hn < n_act) {
+        bool nz = false;
+        if (tid < Hn)
+#pragma unroll
+            for (int w = 0; w < BLK_WORDS; ++w) nz |= L.H[w][tid] != 0ull;
+        int cnt = 0;
+        (void)wg_prefix(nz, L.wcnt, &cnt);
+        if (4 * cnt < Hn) {
+            if (tid == 0) {
+                sh->redo = 1;
+                bs->active = 0;
+            }
+            return;
+        }
+    }
     // selection of panel 0 by the whole workgroup; each later panel's
     // selection overlaps the previous panel's sweep (below)
     uint64_t pm = 0;
@@ -384,10 +404,12 @@ __device__ __forceinline__ void block_select(BlockLDS &L, const uint64_t *__rest
             }
         } else {
             if (rho_now > 0) {
+                // no later selection in this block: only the earlier pivot
+                // rows (which must end in RREF) still need the sweep
                 for (int idx = tid; idx < HMAX * nwd; idx += BS_THREADS) {
                     const int wi = idx / HMAX, h = idx - wi * HMAX;
                     if (wi == 1 && next) continue;
-                    if (h < Hn) sweep_word(word_of(wi), h, HMAX);
+                    if (h < Hn && L.hslot[h] >= 0) sweep_word(word_of(wi), h, HMAX);
                 }
                 __syncthreads();
             }
