@@ -318,8 +318,10 @@ def bench_elim(args, mela):
     for name, M, m, n in cases:
         Rm, piv = ctx.rref_packed(M)
         ctx.free(Rm)
-        for variant, vname in ((mela.ELIM_GE, "ge"), (mela.ELIM_M4RI, "m4ri"),
-                               (mela.ELIM_PANEL, "panel"), (mela.ELIM_AUTO, "auto")):
+        names = {"ge": mela.ELIM_GE, "m4ri": mela.ELIM_M4RI, "panel": mela.ELIM_PANEL,
+                 "auto": mela.ELIM_AUTO, "blocked": mela.ELIM_BLOCKED}
+        for vname in args.elim_variants.split(","):
+            variant = names[vname]
             for op in ("rank", "null"):
                 # two untimed calls: the second one captures the panel sequence
                 # as a hipGraph, which the timed calls replay
@@ -651,6 +653,8 @@ def main():
     ap.add_argument("--traffic", type=float, default=None,
                     help="HBM bytes per launch measured by rocprofv3 --pmc (scaled)")
     ap.add_argument("--elim-reps", type=int, default=2)
+    ap.add_argument("--elim-variants", default="ge,m4ri,panel,auto",
+                    help="elimination variants timed by --workload elim (ge, m4ri, panel, auto, blocked)")
     ap.add_argument("--c1-n", type=int, default=2048,
                     help="side of the square host arrays of the c1 workload")
     ap.add_argument("--batch-count", type=int, default=4096)
