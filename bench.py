@@ -232,6 +232,8 @@ def bench_matmul(args, mela):
         ctx.matmul_packed(A, B, Cm, algo)
     kern_total, launches = ctx.profile_end()
     kern_ms = kern_total / max(launches, 1)
+    check = freivalds_check(ctx, mela, A, B, Cm, m, k, n)
+    progress(f"product {m}x{k}x{n}: {check}")
     work = 2.0 * m * k * n
     tbit = work / (wall_ms * 1e-3) / 1e12
     achieved = work / (kern_ms * 1e-3) / 1e12
@@ -263,6 +265,7 @@ def bench_matmul(args, mela):
                                  big_tiles=((m + 255) // 256) * ((n + 255) // 256) >= 256),
         "call_ms": round(dev_ms, 3),
         "kernel_ms": round(kern_ms, 3),
+        "check": check,
     }
     for M in (A, B, Cm):
         ctx.free(M)
@@ -283,6 +286,29 @@ def bench_matmul(args, mela):
         out["c1"] = bench_c1(args, mela)
         progress("c1 done")
     print(json.dumps(out), flush=True)
+
+
+def freivalds_check(ctx, mela, A, B, Cm, m, k, n, vectors=64):
+    """C = A*B on the device, checked by 64 simultaneous Freivalds vectors:
+    X (n x 64 random bits), C*X against A*(B*X), the narrow products by the
+    AND+parity kernel (not the kernel timed).  A wrong C passes with
+    probability 2^-64.  Raises on a mismatch."""
+    X = ctx.bitmat(n, vectors)
+    Y = ctx.bitmat(k, vectors)
+    Z1 = ctx.bitmat(m, vectors)
+    Z2 = ctx.bitmat(m, vectors)
+    try:
+        ctx.gen_bernoulli(99, X)
+        ctx.matmul_packed(B, X, Y, mela.ALGO_ANDPOP)
+        ctx.matmul_packed(A, Y, Z1, mela.ALGO_ANDPOP)
+        ctx.matmul_packed(Cm, X, Z2, mela.ALGO_ANDPOP)
+        ok = bool((ctx.download(Z1) == ctx.download(Z2)).all())
+    finally:
+        for M in (X, Y, Z1, Z2):
+            ctx.free(M)
+    if not ok:
+        raise SystemExit(f"product {m}x{k}x{n}: C*X != A*(B*X), the timed product is wrong")
+    return f"Freivalds, {vectors} vectors: C*X == A*(B*X)"
 
 
 def elim_bytes_alg(m, n, pivots, rref):
