@@ -12,7 +12,11 @@
  * Matrices handed to the six entry points are MATLAB column-major arrays:
  * element (i,j) of an m x n array sits at index i + j*m.  A "logical" array
  * is one byte per element (nonzero = true); a "double" array is read as
- * ((int)x) & 1 (x86-64 semantics for NaN/Inf/out-of-range: parity 0).
+ * ((int)x) & 1 (x86-64 semantics for NaN/Inf/out-of-range: parity 0; the
+ * reference leaves these undefined, and its arm64 builds saturate instead, so
+ * +Inf and x >= 2^31 would have parity 1 there).  Complex doubles are passed
+ * by their real parts (the MEX gateways do this under either MATLAB array
+ * API, as the reference's mxGetData / mxGetPr reads do).
  *
  * All functions are safe to call from several host threads on distinct
  * contexts; one context serialises its own calls.  Passing ctx == NULL to an

@@ -9,6 +9,8 @@
  * Arity checks and their messages are the reference's; argument validation and
  * the work happen in libmela (which keeps the reference's validation order and
  * text), and a failure is re-raised with mexErrMsgIdAndTxt/mexErrMsgTxt.
+ * Complex double inputs count by their real parts under either MATLAB array
+ * API (real_data).
  */
 #include "mex.h"
 #include "mela.h"
@@ -17,6 +19,28 @@ static int mx_class(const mxArray *a) {
     int c = mxIsLogical(a) ? MELA_LOGICAL : (mxIsDouble(a) ? MELA_DOUBLE : MELA_OTHER);
     if (mxIsSparse(a)) c |= MELA_SPARSE;
     return c;
+}
+
+/* The elements the library reads.  A complex double array counts by its real
+ * parts, as in the reference, whose default (separate-complex) build reads
+ * them through mxGetData / mxGetPr (e.g. null_gf2.c:170, rank_m4ri.c:309).
+ * Under the interleaved complex API (MX_HAS_INTERLEAVED_COMPLEX, `mex
+ * -R2018a`) mxGetData returns (re, im) pairs, so the real parts are gathered
+ * into *tmp first (released by the caller with mxFree). */
+static const void *real_data(const mxArray *a, void **tmp) {
+    *tmp = NULL;
+#if MX_HAS_INTERLEAVED_COMPLEX
+    if (mxIsDouble(a) && mxIsComplex(a) && !mxIsSparse(a)) {
+        const size_t cnt = (size_t)mxGetM(a) * (size_t)mxGetN(a);
+        const double *src = (const double *)mxGetData(a);
+        double *re = (double *)mxCalloc(cnt ? cnt : 1, sizeof(double));
+        if (!re) mexErrMsgTxt("Out of memory.");
+        for (size_t i = 0; i < cnt; ++i) re[i] = src[2 * i];
+        *tmp = re;
+        return re;
+    }
+#endif
+    return mxGetData(a);
 }
 
 static void raise_status(void) {
@@ -46,13 +70,15 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
         raise_status();
     }
     plhs[0] = mxCreateLogicalMatrix(m, n);
+    void *ta, *tb;
+    const void *da = real_data(prhs[0], &ta), *db = real_data(prhs[1], &tb);
 #ifdef MELA_FN_MATMUL_GF2
-    st = mela_matmul_gf2(NULL, mxGetData(prhs[0]), ca, m, k, mxGetData(prhs[1]), cb, k2, n,
-                         (uint8_t *)mxGetLogicals(plhs[0]));
+    st = mela_matmul_gf2(NULL, da, ca, m, k, db, cb, k2, n, (uint8_t *)mxGetLogicals(plhs[0]));
 #else
-    st = mela_matmul_m4ri(NULL, mxGetData(prhs[0]), ca, m, k, mxGetData(prhs[1]), cb, k2, n,
-                          (uint8_t *)mxGetLogicals(plhs[0]));
+    st = mela_matmul_m4ri(NULL, da, ca, m, k, db, cb, k2, n, (uint8_t *)mxGetLogicals(plhs[0]));
 #endif
+    if (ta) mxFree(ta);
+    if (tb) mxFree(tb);
     if (st != MELA_OK) raise_status();
 }
 #endif
@@ -67,13 +93,14 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
 #endif
     double r = 0.0;
     mela_status st;
+    void *t;
+    const void *d = real_data(prhs[0], &t);
 #ifdef MELA_FN_RANK_GF2
-    st = mela_rank_gf2(NULL, mxGetData(prhs[0]), mx_class(prhs[0]), mxGetM(prhs[0]),
-                       mxGetN(prhs[0]), &r);
+    st = mela_rank_gf2(NULL, d, mx_class(prhs[0]), mxGetM(prhs[0]), mxGetN(prhs[0]), &r);
 #else
-    st = mela_rank_m4ri(NULL, mxGetData(prhs[0]), mx_class(prhs[0]), mxGetM(prhs[0]),
-                        mxGetN(prhs[0]), &r);
+    st = mela_rank_m4ri(NULL, d, mx_class(prhs[0]), mxGetM(prhs[0]), mxGetN(prhs[0]), &r);
 #endif
+    if (t) mxFree(t);
     if (st != MELA_OK) raise_status();
     plhs[0] = mxCreateDoubleScalar(r);
 }
@@ -93,13 +120,16 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
     size_t f = 0;
     mela_status st;
     mxArray *out = NULL;
+    void *t;
+    const void *d = real_data(prhs[0], &t);
 #ifdef MELA_FN_NULL_GF2
-    st = mela_null_gf2(NULL, mxGetData(prhs[0]), mx_class(prhs[0]), mxGetM(prhs[0]),
-                       mxGetN(prhs[0]), alloc_out, &out, &f);
+    st = mela_null_gf2(NULL, d, mx_class(prhs[0]), mxGetM(prhs[0]), mxGetN(prhs[0]), alloc_out,
+                       &out, &f);
 #else
-    st = mela_null_m4ri(NULL, mxGetData(prhs[0]), mx_class(prhs[0]), mxGetM(prhs[0]),
-                        mxGetN(prhs[0]), alloc_out, &out, &f);
+    st = mela_null_m4ri(NULL, d, mx_class(prhs[0]), mxGetM(prhs[0]), mxGetN(prhs[0]), alloc_out,
+                        &out, &f);
 #endif
+    if (t) mxFree(t);
     if (st != MELA_OK) raise_status();
     plhs[0] = out;
 }

@@ -11,22 +11,27 @@ struct mxArray_tag {
     int cls;
     int sparse;
     mwSize m, n;
-    void *data;
+    void *data;  /* SHIM_COMPLEX: real parts, or (re, im) pairs when interleaved */
+    double *pi;  /* SHIM_COMPLEX with separate storage: imaginary parts */
 };
 
 static __thread jmp_buf *t_jmp;
 static __thread char t_msg[1024];
 static __thread char t_id[256];
 
-static size_t esize(int cls) { return cls == SHIM_DOUBLE ? 8 : (cls == SHIM_INT32 ? 4 : 1); }
+static size_t esize(int cls) {
+    if (cls == SHIM_COMPLEX) return MX_HAS_INTERLEAVED_COMPLEX ? 16 : 8;
+    return cls == SHIM_DOUBLE ? 8 : (cls == SHIM_INT32 ? 4 : 1);
+}
 
 mwSize mxGetM(const mxArray *a) { return a->m; }
 mwSize mxGetN(const mxArray *a) { return a->n; }
 bool mxIsSparse(const mxArray *a) { return a->sparse != 0; }
 bool mxIsLogical(const mxArray *a) { return a->cls == SHIM_LOGICAL; }
-bool mxIsDouble(const mxArray *a) { return a->cls == SHIM_DOUBLE; }
+bool mxIsDouble(const mxArray *a) { return a->cls == SHIM_DOUBLE || a->cls == SHIM_COMPLEX; }
+bool mxIsComplex(const mxArray *a) { return a->cls == SHIM_COMPLEX; }
 void *mxGetData(const mxArray *a) { return a->data; }
-double *mxGetPr(const mxArray *a) { return a->cls == SHIM_DOUBLE ? (double *)a->data : NULL; }
+double *mxGetPr(const mxArray *a) { return mxIsDouble(a) ? (double *)a->data : NULL; }
 mxLogical *mxGetLogicals(const mxArray *a) {
     return a->cls == SHIM_LOGICAL ? (mxLogical *)a->data : NULL;
 }
@@ -40,13 +45,25 @@ mxArray *shim_create(int cls, mwSize m, mwSize n, int sparse, const void *data) 
     a->m = m;
     a->n = n;
     a->data = calloc((m * n) != 0 ? m * n : 1, esize(cls));
-    if (data && (m * n) != 0) memcpy(a->data, data, m * n * esize(cls));
+    a->pi = NULL;
+    if (cls == SHIM_COMPLEX && !MX_HAS_INTERLEAVED_COMPLEX) {
+        /* separate storage: split the (re, im) pairs */
+        a->pi = (double *)calloc((m * n) != 0 ? m * n : 1, sizeof(double));
+        const double *src = (const double *)data;
+        for (size_t i = 0; data && i < m * n; ++i) {
+            ((double *)a->data)[i] = src[2 * i];
+            a->pi[i] = src[2 * i + 1];
+        }
+    } else if (data && (m * n) != 0) {
+        memcpy(a->data, data, m * n * esize(cls));
+    }
     return a;
 }
 
 void shim_destroy(mxArray *a) {
     if (!a) return;
     free(a->data);
+    free(a->pi);
     free(a);
 }
 

@@ -21,7 +21,10 @@ def shim():
         pytest.skip("gateways not built")
     import mex_linear_algebra_amd as m
     m.load()  # libmela.so first, so the gateways resolve against it
-    S = C.CDLL(path, mode=C.RTLD_GLOBAL)
+    return _shim_types(C.CDLL(path, mode=C.RTLD_GLOBAL))
+
+
+def _shim_types(S):
     S.shim_create.restype = C.c_void_p
     S.shim_create.argtypes = [C.c_int, C.c_size_t, C.c_size_t, C.c_int, C.c_void_p]
     S.shim_call.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.c_int,
@@ -34,16 +37,36 @@ def shim():
     S.mxGetN.restype = C.c_size_t
     S.mxGetData.argtypes = [C.c_void_p]
     S.mxGetData.restype = C.c_void_p
+    S.mxGetPr.argtypes = [C.c_void_p]
+    S.mxGetPr.restype = C.c_void_p
+    S.mxIsDouble.argtypes = [C.c_void_p]
+    S.mxIsDouble.restype = C.c_bool
+    S.mxIsComplex.argtypes = [C.c_void_p]
+    S.mxIsComplex.restype = C.c_bool
     return S
 
 
-def gateway(name):
-    L = C.CDLL(os.path.join(MEXDIR, name + ".mexa64"))
+def _il_shim():
+    """The stand-in built for MATLAB's interleaved complex API (mex -R2018a),
+    loaded privately so that it and its gateways do not bind to the default
+    build's symbols."""
+    path = os.path.join(MEXDIR, "il", "libmexshim.so")
+    if not os.path.exists(path):
+        pytest.skip("interleaved-complex gateways not built")
+    return _shim_types(C.CDLL(path, mode=C.RTLD_LOCAL))
+
+
+def gateway(name, il=False):
+    path = os.path.join(MEXDIR, "il" if il else "", name + ".mexa64")
+    L = C.CDLL(path, mode=(os.RTLD_DEEPBIND | C.RTLD_LOCAL) if il else C.RTLD_LOCAL)
     return C.cast(L.mexFunction, C.c_void_p).value, L
 
 
 def mx(shim, A, sparse=False):
     A = np.asarray(A)
+    if A.dtype == np.complex128:
+        buf = np.asfortranarray(A)  # (re, im) pairs in column-major order
+        return shim.shim_create(3, A.shape[0], A.shape[1], int(sparse), buf.ctypes.data if buf.size else None)
     if A.dtype == np.bool_:
         cls, buf = 0, np.asfortranarray(A).view(np.uint8)
     elif A.dtype == np.float64:
@@ -53,8 +76,8 @@ def mx(shim, A, sparse=False):
     return shim.shim_create(cls, A.shape[0], A.shape[1], int(sparse), buf.ctypes.data if buf.size else None)
 
 
-def call(shim, name, nlhs, *args):
-    fn, keep = gateway(name)
+def call(shim, name, nlhs, *args, il=False):
+    fn, keep = gateway(name, il)
     prhs = (C.c_void_p * max(len(args), 1))(*args)
     plhs = (C.c_void_p * max(nlhs, 1))()
     raised = shim.shim_call(fn, nlhs, plhs, len(args), prhs)
@@ -113,3 +136,42 @@ def test_gateways_on_device(shim, orc):
         plhs, err, _ = call(shim, fn, 1, mx(shim, A))
         assert err is None
         assert np.array_equal(_logical(shim, plhs[0]), orc.null(A, "gf2"))
+
+
+def test_complex_storage_in_both_apis(shim):
+    """The stand-in's complex doubles: mxGetData / mxGetPr give the real parts
+    under the separate API and (re, im) pairs under the interleaved one; both
+    count as double and complex."""
+    Z = np.array([[1.5 + 2j, -3 + 0.5j], [4 - 1j, 7 + 7j]])
+    for S, il in ((shim, False), (_il_shim(), True)):
+        a = mx(S, Z)
+        assert S.mxIsDouble(a) and S.mxIsComplex(a)
+        n = 8 if il else 4
+        got = np.ctypeslib.as_array(C.cast(S.mxGetData(a), C.POINTER(C.c_double)), shape=(n,))
+        want = np.asfortranarray(Z).ravel(order="F")
+        want = np.column_stack([want.real, want.imag]).ravel() if il else want.real
+        assert np.array_equal(got, want)
+
+
+@pytest.mark.gpu
+def test_gateways_complex_double_use_real_part(shim, orc):
+    """Complex double inputs count by their real parts (the reference reads
+    them through mxGetData / mxGetPr, null_gf2.c:170, rank_m4ri.c:309), under
+    both MATLAB array APIs: each gateway's result equals the oracle's for the
+    real part alone."""
+    rng = np.random.default_rng(29)
+    Re = rng.integers(-3, 4, (60, 75)).astype(np.float64)
+    Im = rng.integers(-3, 4, (60, 75)).astype(np.float64)
+    Z = Re + 1j * Im
+    B = rng.integers(0, 2, (75, 30)).astype(bool)
+    R = (np.abs(Re.astype(np.int64)) & 1).astype(bool)
+    for S, il in ((shim, False), (_il_shim(), True)):
+        plhs, err, _ = call(S, "mela_matmul_gf2", 1, mx(S, Z), mx(S, B), il=il)
+        assert err is None, err
+        assert np.array_equal(_logical(S, plhs[0]), orc.matmul(R, B, "gf2"))
+        plhs, err, _ = call(S, "mela_rank_m4ri", 1, mx(S, Z), il=il)
+        assert err is None, err
+        assert C.cast(S.mxGetData(plhs[0]), C.POINTER(C.c_double))[0] == orc.rank(R, "gf2")
+        plhs, err, _ = call(S, "mela_null_gf2", 1, mx(S, Z), il=il)
+        assert err is None, err
+        assert np.array_equal(_logical(S, plhs[0]), orc.null(R, "gf2"))
