@@ -7,7 +7,7 @@
 // A scheduler thread admits queued requests in arrival order while the
 // device memory they need fits the budget, and routes each one:
 //  * rank and null requests that one workgroup finishes quickly go to the
-//    device-resident loop: a persistent kernel (batch.hip, k_serve_loop)
+//    device-resident loop: a persistent kernel (batch.hip, k_rref_loop with the serving source)
 //    whose workgroups take request slots from a ring in pinned host memory
 //    as the scheduler publishes them, so requests that arrive while others
 //    are on the GPU start as soon as a workgroup is free, with no launch per
