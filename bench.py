@@ -398,6 +398,11 @@ def bench_batch(args, mela):
     dev = [ctx.upload(P, n) for P, m, n, _ in items]
     b = mela.Batch(ctx, dev, mela.BATCH_NULL | mela.BATCH_GRAPH)
     tuned_ms = b.autotune()
+    tw, md, gr = b.plan()
+    # modes as they run: k = 8 tables only with 8-word tiles
+    eff = [0 if x == 0 else (8 if x == 2 and tw == 8 else 4) for x in md]
+    plan = {"tile_words": tw, "ge": eff.count(0), "m4ri_k4": eff.count(4), "m4ri_k8": eff.count(8),
+            "multi_workgroup_matrices": sum(g > 1 for g in gr)}
     b.run()
     b.results()
     ev = HipEvents()
@@ -435,7 +440,7 @@ def bench_batch(args, mela):
         "workload": f"batched rank+null, {len(items)} uneven matrices (m log-uniform 64..2048, "
                     f"n=m+U[0,m], 90% column-weight 3/4/6, 10% dense)",
         "batched_ms": round(wall * 1e3, 3), "batched_device_ms": round(dev_ms, 3),
-        "matrices_per_s": round(rate, 1), "autotune_best_ms": round(tuned_ms, 3),
+        "matrices_per_s": round(rate, 1), "autotune_best_ms": round(tuned_ms, 3), "plan": plan,
         "plain_one_per_launch_matrices_per_s": round(plain_rate, 1),
         "plain_sample": len(sidx), "speedup_vs_plain": round(rate / plain_rate, 2),
         "generate_s": round(gen_s, 2), "mean_rank": sum(ranks) / len(ranks),

@@ -260,8 +260,18 @@ enum {
 };
 mela_status mela_batch_create(mela_ctx *ctx, const mela_bitmat *mats, size_t count,
                               int flags, mela_batch **batch);
-/* per-matrix elimination mode: 0 = GE, 1 = M4RI 4-bit tables */
+/* per-matrix elimination mode: 0 = GE (XOR each pivot row), 1 = M4RI with
+ * k = 4 (16 lookups per row word), 2 = M4RI with k = 8 (8 lookups per row
+ * word; takes effect with 8-word tiles, runs as k = 4 with wider tiles).
+ * The reference's M4RI k is a compile-time constant (mela_rank_m4ri.c:18,
+ * mela_null_m4ri.c:28); here it is chosen per matrix. */
 mela_status mela_batch_set_modes(mela_batch *batch, const int *modes);
+/* update tile width in 64-bit words: 8 (the only width with k = 8 tables),
+ * 16 (default) or 32 */
+mela_status mela_batch_set_tile(mela_batch *batch, int tile_words);
+/* the plan the next run uses: tile width, per-matrix modes and workgroups
+ * per matrix (each output may be NULL; modes and groups hold count entries) */
+mela_status mela_batch_get_plan(mela_batch *batch, int *tile_words, int *modes, int *groups);
 /* per-matrix workgroups (1..8) that eliminate the matrix together: the
  * tiles of each panel are dealt out among them, one device-scope barrier per
  * panel.  Every workgroup of the launch is resident at once, so the group
@@ -269,9 +279,10 @@ mela_status mela_batch_set_modes(mela_batch *batch, const int *modes);
  * work a barrier that waits over 0.5 s fails the run (MELA_ERR_DEVICE from
  * mela_batch_results).  Default 1. */
 mela_status mela_batch_set_groups(mela_batch *batch, const int *groups);
-/* times candidate tile widths, mode assignments and workgroups-per-matrix
- * for the largest matrices on the batch itself and keeps the fastest;
- * *best_ms receives its time (may be NULL) */
+/* times candidate plans on the batch itself and keeps the fastest: tile
+ * width x per-matrix mode (GE, or M4RI with k = 4 or 8), then workgroups
+ * per matrix for the largest matrices; *best_ms receives its time (may be
+ * NULL); mela_batch_get_plan reports the choice */
 mela_status mela_batch_autotune(mela_batch *batch, float *best_ms);
 mela_status mela_batch_run(mela_batch *batch);
 mela_status mela_batch_results(mela_batch *batch, uint32_t *ranks, mela_bitmat *nulls);

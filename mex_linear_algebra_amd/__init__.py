@@ -166,6 +166,10 @@ def load():
         L.mela_batch_create.restype = st
         L.mela_batch_set_modes.argtypes = [bp, C.POINTER(C.c_int)]
         L.mela_batch_set_modes.restype = st
+        L.mela_batch_set_tile.argtypes = [bp, C.c_int]
+        L.mela_batch_set_tile.restype = st
+        L.mela_batch_get_plan.argtypes = [bp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.mela_batch_get_plan.restype = st
         L.mela_batch_set_groups.argtypes = [bp, C.POINTER(C.c_int)]
         L.mela_batch_set_groups.restype = st
         L.mela_batch_autotune.argtypes = [bp, C.POINTER(C.c_float)]
@@ -408,8 +412,22 @@ class Batch:
         self.handle = h
 
     def set_modes(self, modes):
+        """Per matrix: 0 = GE, 1 = M4RI with k = 4, 2 = M4RI with k = 8
+        (8-word tiles; k = 4 with wider tiles)."""
         arr = (C.c_int * max(self.count, 1))(*[int(x) for x in modes])
         _check(self.lib.mela_batch_set_modes(self.handle, arr))
+
+    def set_tile(self, words: int):
+        """Update tile width: 8, 16 or 32 words."""
+        _check(self.lib.mela_batch_set_tile(self.handle, int(words)))
+
+    def plan(self):
+        """(tile words, per-matrix modes, per-matrix workgroups) of the next run."""
+        tw = C.c_int()
+        modes = (C.c_int * max(self.count, 1))()
+        groups = (C.c_int * max(self.count, 1))()
+        _check(self.lib.mela_batch_get_plan(self.handle, C.byref(tw), modes, groups))
+        return tw.value, [modes[i] for i in range(self.count)], [groups[i] for i in range(self.count)]
 
     def set_groups(self, groups):
         """Workgroups (1..8) that eliminate each matrix together."""

@@ -7,14 +7,16 @@
 // inside the workgroup (panel select -> pivot rows -> table or direct XOR
 // sweep -> earlier pivot rows), then write its rank and its packed null
 // basis.  Per-matrix mode (chosen by the autotuner): GE (XOR each selected
-// pivot row) or M4RI with 4-bit tables (16 table lookups per row word); the
-// update tile width (16 or 32 words) is chosen by the autotuner per batch.
+// pivot row) or M4RI with k-bit tables, k = 4 (16 table lookups per row
+// word) or, in the kernel with 8-word tiles, k = 8 (8 lookups per row word,
+// 8 x 256-entry tables per tile word); the update tile width (8, 16 or 32
+// words) is chosen by the autotuner per batch.
 //
 // Per panel and tile of TW words: the selected rows' words are staged in LDS,
 // the tile's pivot rows are formed from 4-bit tables of them (and stored to
-// Q), the M4RI tables are built from those, and the sweep runs over the
-// panel's list of rows with a nonzero multiplier, 16 row passes per wave with
-// their loads in flight together, looked up table by table.
+// Q), the matrix's M4RI tables are built from those, and the sweep runs over
+// the panel's list of rows with a nonzero multiplier, 16 row passes per wave
+// with their loads in flight together, looked up table by table.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -32,14 +34,12 @@ struct BatchMat {
     uint64_t z_off;  // words: packed null basis (n x ldz) in the Z arena
     uint32_t m, n;
     uint32_t lda, ldz;
-    uint32_t mode;    // 0 = GE, 1 = M4RI (4-bit tables)
+    uint32_t mode;    // 0 = GE, 1 = M4RI k = 4, 2 = M4RI k = 8 (8-word tiles; k = 4 otherwise)
     uint32_t groups;  // workgroups eliminating the matrix together (1..MAX_GROUP)
 };
 
 namespace {
 
-constexpr int BT = 256;          // threads per workgroup
-constexpr int BT_WAVES = BT / 64;
 constexpr int MAX_M = 4096;
 constexpr int MAX_N = 8192;
 constexpr int MAX_WC = MAX_N / 64;
@@ -49,9 +49,17 @@ constexpr int MAX_GROUP = 8;     // workgroups per matrix at most
 // for its partners gives up and flags the batch as failed
 constexpr uint64_t GROUP_WAIT_TICKS = 50000000ull;
 
+// The 8-bit tables of a tile: 8 tables x 256 entries x 8 words = 128 KiB,
+// so one workgroup per CU; that kernel runs 512 threads per workgroup (two
+// waves per SIMD, as the 256-thread kernels with 16- and 32-word tiles get
+// from two resident workgroups).
+constexpr int K8_TW = 8;
+constexpr int T8_N = 8;
+constexpr int batch_threads(int tw) { return tw == K8_TW ? 512 : 256; }
+
 template <int TW>
 struct BatchLDS {
-    dev::SelectLDS<BT> sel;
+    dev::SelectLDS<batch_threads(TW)> sel;
     uint64_t comb[64];
     int selrow[64];
     uint16_t act[MAX_M];
@@ -65,7 +73,10 @@ struct BatchLDS {
     union {
         uint64_t S[64][TW];      // selected rows' words of the tile
         uint64_t T[16][16][TW];  // 4-bit tables (of the selected rows, then of the pivot rows)
-        uint64_t P[64][TW];      // pivot rows of the tile, by pivot bit (GE sweep)
+        uint64_t P[64][TW];      // pivot rows of the tile, by pivot bit (GE sweep, 8-bit tables)
+        // 8-bit tables of the pivot rows (8-word tiles only): table t,
+        // entry v = XOR of pivot rows 8t + b over the set bits b of v
+        uint64_t T8[TW == K8_TW ? T8_N : 1][TW == K8_TW ? 256 : 1][TW];
         struct {
             uint64_t fm[MAX_WC];     // free-column mask of each column word
             uint64_t mv[MAX_WC][6];  // its compress masks
@@ -164,8 +175,10 @@ struct BatchItem {
 // src.finish(); src.rank() stores the rank.  A group barrier that gave up skips the rest of the matrix
 // (the batch then fails as a whole; only a batch source has groups).
 template <int TW, class Src>
-__global__ void __launch_bounds__(BT)
+__global__ void __launch_bounds__(batch_threads(TW))
 k_rref_loop(const Src src_arg) {
+    constexpr int BT = batch_threads(TW);  // threads per workgroup
+    constexpr int BT_WAVES = BT / 64;
     extern __shared__ __align__(16) unsigned char smem[];
     BatchLDS<TW> &L = *reinterpret_cast<BatchLDS<TW> *>(smem);
     Src src = src_arg;
@@ -186,6 +199,9 @@ k_rref_loop(const Src src_arg) {
     const int m = (int)d.m, n = (int)d.n;
     const int wc = (n + 63) >> 6;
     const int rmax = m < n ? m : n;
+    // table bits of the sweep: 0 (GE), 4, or 8 (8-word tiles only)
+    // (the host passes mode 2 to the 8-word tile kernel only)
+    const int kb = d.mode == 1 ? 4 : (TW == K8_TW && d.mode == 2 ? 8 : 0);
     uint64_t *M = work + d.w_off;
     uint64_t *Q = work + d.q_off;
 
@@ -293,6 +309,9 @@ k_rref_loop(const Src src_arg) {
             }
         }
         constexpr int RPP = 64 / TW;  // rows per wave pass, TW lanes (words) per row
+        // row passes per wave with loads in flight together (the 512-thread
+        // kernel has twice the waves per SIMD and half the registers)
+        constexpr int SWP = TW == K8_TW ? SW_PASSES / 2 : SW_PASSES;
         const int sub = lane / TW, wl = lane % TW;
         for (int w0 = g + mk * TW; w0 < wc; w0 += G * TW) {
             // selected rows' words of the tile (loads issued together;
@@ -360,7 +379,7 @@ k_rref_loop(const Src src_arg) {
                 }
             }
             __syncthreads();
-            if (d.mode == 1) {
+            if (kb == 4) {
                 build_tables();
             } else {
 #pragma unroll
@@ -370,47 +389,76 @@ k_rref_loop(const Src src_arg) {
 #pragma unroll
                     for (int b = 0; b < 4; ++b) L.u.P[4 * t + b][ww] = q[k][b];
                 }
+                if constexpr (TW == K8_TW) {
+                    if (kb == 8) {
+                        // thread (t8, ch, ww) builds entries v = 8 j + ch of
+                        // table t8 for tile word ww, j in Gray-code order (a
+                        // wave's stores cover 512 consecutive bytes)
+                        static_assert(BT == T8_N * 8 * K8_TW, "one thread per eighth of a table");
+                        __syncthreads();
+                        const int ww = tid % K8_TW, ch = (tid / K8_TW) & 7, t8 = tid / (8 * K8_TW);
+                        uint64_t pr[8];
+#pragma unroll
+                        for (int b = 0; b < 8; ++b) pr[b] = L.u.P[8 * t8 + b][ww];
+                        __syncthreads();
+                        uint64_t cur = ((ch & 1) ? pr[0] : 0ull) ^ ((ch & 2) ? pr[1] : 0ull) ^
+                                       ((ch & 4) ? pr[2] : 0ull);
+                        L.u.T8[t8][ch][ww] = cur;
+#pragma unroll
+                        for (int j = 1; j < 32; ++j) {
+                            cur ^= pr[3 + __builtin_ctz(j)];
+                            L.u.T8[t8][8 * (j ^ (j >> 1)) + ch][ww] = cur;
+                        }
+                    }
+                }
             }
             __syncthreads();
-            // sweep words > g of the tile; SW_PASSES passes' loads together
+            // sweep words > g of the tile; SWP passes' loads together
             const int w = w0 + wl;
             const bool wok = w < wc && w > g;
             const int wsafe = wok ? w : g;
 #pragma unroll 1
-            for (int i0 = wave * RPP + sub; i0 < nupd; i0 += BT_WAVES * RPP * SW_PASSES) {
+            for (int i0 = wave * RPP + sub; i0 < nupd; i0 += BT_WAVES * RPP * SWP) {
                 // every pass loads unconditionally (entries past the end
                 // re-read row 0 of M) so all loads are in flight together
-                uint64_t *rp[SW_PASSES];
-                uint64_t x[SW_PASSES], a[SW_PASSES];
+                uint64_t *rp[SWP];
+                uint64_t x[SWP], a[SWP];
 #pragma unroll
-                for (int j = 0; j < SW_PASSES; ++j) {
+                for (int j = 0; j < SWP; ++j) {
                     const int i = i0 + j * BT_WAVES * RPP;
                     rp[j] = i < nupd ? sweep_row(L.upd[i]) : M;
                 }
 #pragma unroll
-                for (int j = 0; j < SW_PASSES; ++j) x[j] = rp[j][g];
+                for (int j = 0; j < SWP; ++j) x[j] = rp[j][g];
 #pragma unroll
-                for (int j = 0; j < SW_PASSES; ++j) a[j] = rp[j][wsafe];
+                for (int j = 0; j < SWP; ++j) a[j] = rp[j][wsafe];
 #pragma unroll
-                for (int j = 0; j < SW_PASSES; ++j) {
+                for (int j = 0; j < SWP; ++j) {
                     const int i = i0 + j * BT_WAVES * RPP;
                     x[j] = (i < nupd && wok) ? x[j] & pm : 0ull;
                 }
-                if (d.mode == 1) {
+                if (kb == 4) {
                     // table-major: one lookup of every pass per table, so
                     // the passes' LDS reads overlap (a zero multiplier
                     // reads the zero entry)
 #pragma unroll
                     for (int t = 0; t < 16; ++t)
 #pragma unroll
-                        for (int j = 0; j < SW_PASSES; ++j) a[j] ^= L.u.T[t][(x[j] >> (4 * t)) & 15u][wl];
+                        for (int j = 0; j < SWP; ++j) a[j] ^= L.u.T[t][(x[j] >> (4 * t)) & 15u][wl];
+                } else if (kb == 8) {
+                    if constexpr (TW == K8_TW) {
+#pragma unroll
+                        for (int t = 0; t < T8_N; ++t)
+#pragma unroll
+                            for (int j = 0; j < SWP; ++j) a[j] ^= L.u.T8[t][(x[j] >> (8 * t)) & 255u][wl];
+                    }
                 } else {
 #pragma unroll
-                    for (int j = 0; j < SW_PASSES; ++j)
+                    for (int j = 0; j < SWP; ++j)
                         for (uint64_t xx = x[j]; xx; xx &= xx - 1) a[j] ^= L.u.P[dev::ffs64(xx)][wl];
                 }
 #pragma unroll
-                for (int j = 0; j < SW_PASSES; ++j)
+                for (int j = 0; j < SWP; ++j)
                     if (x[j]) rp[j][w] = a[j];
             }
             __syncthreads();
@@ -702,7 +750,14 @@ k_batch_popcount(const BatchMat *__restrict__ mats, const uint64_t *const *__res
 
 }  // namespace
 
-size_t batch_lds_bytes(int tw) { return tw == 32 ? sizeof(BatchLDS<32>) : sizeof(BatchLDS<16>); }
+static_assert(sizeof(BatchLDS<K8_TW>) <= 160 * 1024, "8-word tile LDS exceeds a CU's 160 KiB");
+
+size_t batch_lds_bytes(int tw) {
+    return tw == 32 ? sizeof(BatchLDS<32>) : tw == K8_TW ? sizeof(BatchLDS<K8_TW>) : sizeof(BatchLDS<16>);
+}
+
+// index of a tile width (8, 16, 32 words) in per-width tables
+int tw_slot(int tw) { return tw == K8_TW ? 0 : tw == 16 ? 1 : 2; }
 
 // Serving-loop workgroups: a quarter of the CUs.  The loop stays resident
 // between requests, so the CUs it does not occupy stay free for the kernels
@@ -735,8 +790,8 @@ mela_status launch_serve_loop(hipStream_t s, int grid, ServeSlot *slots, ServeDo
     src.idle_ticks = idle_ticks;
     src.life_ticks = life_ticks;
     src.cap = cap;
-    hipLaunchKernelGGL((k_rref_loop<16, ServeSource>), dim3((unsigned)grid), dim3(BT), batch_lds_bytes(16), s,
-                       src);
+    hipLaunchKernelGGL((k_rref_loop<16, ServeSource>), dim3((unsigned)grid), dim3(batch_threads(16)),
+                       batch_lds_bytes(16), s, src);
     MELA_HIP(hipGetLastError());
     return MELA_OK;
 }
@@ -765,8 +820,8 @@ struct mela_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int tuned = -1;  // index of the autotuned mode assignment
-    int tw = 16;     // update tile width in words (16 or 32, autotuned)
-    int occ_tw[2] = {1, 1};   // resident workgroups per CU for tile widths 16 and 32
+    int tw = 16;     // update tile width in words (8, 16 or 32, autotuned)
+    int occ_tw[3] = {1, 1, 1};  // resident workgroups per CU for tile widths 8, 16 and 32
     bool ctx_scratch = false;  // arenas borrowed from the context (not freed)
 };
 
@@ -778,10 +833,16 @@ using namespace mela;
 
 namespace mela {
 
+const void *batch_kernel(int tw) {
+    return tw == 32 ? (const void *)k_rref_loop<32, BatchSource>
+                    : tw == K8_TW ? (const void *)k_rref_loop<K8_TW, BatchSource>
+                                  : (const void *)k_rref_loop<16, BatchSource>;
+}
+
 // Persistent grid: every workgroup resident at once (the group barriers wait
 // for partners that must be running), never more than the queue's slots.
 int batch_grid(const mela_batch *b) {
-    const int occ = b->occ_tw[b->tw == 32 ? 1 : 0];
+    const int occ = b->occ_tw[tw_slot(b->tw)];
     return std::max(1, std::min(b->nslots, occ * b->ctx->cu_count));
 }
 
@@ -803,11 +864,27 @@ mela_status batch_launch_raw(mela_batch *b, hipStream_t s) {
     src.ranks = b->d_ranks;
     src.nslots = b->nslots;
     src.want_null = want_null;
+    const dim3 blk((unsigned)batch_threads(b->tw));
     if (b->tw == 32)
-        hipLaunchKernelGGL((k_rref_loop<32, BatchSource>), dim3(grid), dim3(BT), batch_lds_bytes(32), s, src);
+        hipLaunchKernelGGL((k_rref_loop<32, BatchSource>), dim3(grid), blk, batch_lds_bytes(32), s, src);
+    else if (b->tw == K8_TW)
+        hipLaunchKernelGGL((k_rref_loop<K8_TW, BatchSource>), dim3(grid), blk, batch_lds_bytes(K8_TW), s, src);
     else
-        hipLaunchKernelGGL((k_rref_loop<16, BatchSource>), dim3(grid), dim3(BT), batch_lds_bytes(16), s, src);
+        hipLaunchKernelGGL((k_rref_loop<16, BatchSource>), dim3(grid), blk, batch_lds_bytes(16), s, src);
     MELA_HIP(hipGetLastError());
+    return MELA_OK;
+}
+
+// Descriptors as the kernel of the batch's tile width reads them: mode 2
+// (k = 8) runs as mode 1 (k = 4) unless the tiles are 8 words wide.
+mela_status batch_upload_mats(mela_batch *b, hipStream_t s) {
+    if (b->mats.empty()) return MELA_OK;
+    std::vector<BatchMat> dm = b->mats;
+    if (b->tw != K8_TW)
+        for (BatchMat &d : dm)
+            if (d.mode == 2) d.mode = 1;
+    MELA_HIP(hipMemcpyAsync(b->d_mats, dm.data(), dm.size() * sizeof(BatchMat), hipMemcpyHostToDevice, s));
+    MELA_HIP(hipStreamSynchronize(s));  // dm is freed on return
     return MELA_OK;
 }
 
@@ -822,12 +899,10 @@ mela_status batch_upload_plan(mela_batch *b) {
     }
     b->nslots = (int)sl.size();
     hipStream_t s = b->ctx->stream;
-    if (!b->mats.empty()) {
-        MELA_HIP(hipMemcpyAsync(b->d_mats, b->mats.data(), b->mats.size() * sizeof(BatchMat),
-                                hipMemcpyHostToDevice, s));
+    if (!b->mats.empty())
         MELA_HIP(hipMemcpyAsync(b->d_slots, sl.data(), sl.size() * sizeof(int2), hipMemcpyHostToDevice, s));
-    }
-    MELA_HIP(hipStreamSynchronize(s));
+    mela_status st = batch_upload_mats(b, s);
+    if (st != MELA_OK) return st;
     // a captured graph holds the old grid size
     if (b->exec) {
         (void)hipGraphExecDestroy(b->exec);
@@ -881,12 +956,9 @@ mela_status mela_batch_create(mela_ctx *ctx, const mela_bitmat *mats, size_t cou
     static bool attr_set = false;
     std::lock_guard<std::mutex> alk(attr_mu);
     if (!attr_set) {
-        MELA_HIP(hipFuncSetAttribute((const void *)k_rref_loop<16, BatchSource>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)batch_lds_bytes(16)));
-        MELA_HIP(hipFuncSetAttribute((const void *)k_rref_loop<32, BatchSource>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)batch_lds_bytes(32)));
+        for (int tw : {K8_TW, 16, 32})
+            MELA_HIP(hipFuncSetAttribute(batch_kernel(tw), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)batch_lds_bytes(tw)));
         attr_set = true;
     }
     mela_batch *b = new mela_batch();
@@ -979,11 +1051,11 @@ mela_status mela_batch_create(mela_ctx *ctx, const mela_bitmat *mats, size_t cou
         MELA_HIP(hipMemcpyAsync(b->d_order, b->order.data(), nb * sizeof(int), hipMemcpyHostToDevice, s));
         MELA_HIP(hipMemcpyAsync(b->d_inputs, b->inputs.data(), nb * sizeof(void *), hipMemcpyHostToDevice, s));
     }
-    for (int v = 0; v < 2; ++v) {
+    for (int tw : {K8_TW, 16, 32}) {
         int occ = 0;
-        const void *fn = v ? (const void *)k_rref_loop<32, BatchSource> : (const void *)k_rref_loop<16, BatchSource>;
-        MELA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, BT, batch_lds_bytes(v ? 32 : 16)));
-        b->occ_tw[v] = occ < 1 ? 1 : occ;
+        MELA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batch_kernel(tw), batch_threads(tw),
+                                                              batch_lds_bytes(tw)));
+        b->occ_tw[tw_slot(tw)] = occ < 1 ? 1 : occ;
     }
     if ((st = batch_upload_plan(b)) != MELA_OK) {
         free_batch_device(b);
@@ -1006,11 +1078,11 @@ mela_status set_groups_locked(mela_batch *b, const int *groups) {
 }
 
 mela_status set_modes_locked(mela_batch *b, const int *modes) {
-    for (size_t i = 0; i < b->count; ++i) b->mats[i].mode = modes[i] ? 1u : 0u;
-    MELA_HIP(hipMemcpyAsync(b->d_mats, b->mats.data(), b->mats.size() * sizeof(BatchMat),
-                            hipMemcpyHostToDevice, b->ctx->stream));
-    MELA_HIP(hipStreamSynchronize(b->ctx->stream));
-    return MELA_OK;
+    for (size_t i = 0; i < b->count; ++i)
+        if (modes[i] < 0 || modes[i] > 2)
+            return set_error(MELA_ERR_INVALID, "batch modes are 0 (GE), 1 (M4RI k = 4) or 2 (M4RI k = 8)");
+    for (size_t i = 0; i < b->count; ++i) b->mats[i].mode = (uint32_t)modes[i];
+    return batch_upload_mats(b, b->ctx->stream);
 }
 
 mela_status autotune_locked(mela_batch *b, float *best_ms);
@@ -1029,6 +1101,27 @@ mela_status mela_batch_set_modes(mela_batch *b, const int *modes) {
     CtxLock lk(b->ctx->mu);
     (void)hipSetDevice(b->ctx->device);
     return set_modes_locked(b, modes);
+}
+
+mela_status mela_batch_set_tile(mela_batch *b, int tile_words) {
+    if (!b) return set_error(MELA_ERR_INVALID, "null batch");
+    if (tile_words != 8 && tile_words != 16 && tile_words != 32)
+        return set_error(MELA_ERR_INVALID, "batch tile width must be 8, 16 or 32 words");
+    CtxLock lk(b->ctx->mu);
+    (void)hipSetDevice(b->ctx->device);
+    b->tw = tile_words;
+    return batch_upload_plan(b);  // drops a graph captured with the old kernel and grid
+}
+
+mela_status mela_batch_get_plan(mela_batch *b, int *tile_words, int *modes, int *groups) {
+    if (!b) return set_error(MELA_ERR_INVALID, "null batch");
+    CtxLock lk(b->ctx->mu);
+    if (tile_words) *tile_words = b->tw;
+    for (size_t i = 0; i < b->count; ++i) {
+        if (modes) modes[i] = (int)b->mats[i].mode;
+        if (groups) groups[i] = (int)b->mats[i].groups;
+    }
+    return MELA_OK;
 }
 
 mela_status mela_batch_autotune(mela_batch *b, float *best_ms) {
@@ -1098,25 +1191,43 @@ mela_status autotune_locked(mela_batch *b, float *best_ms) {
     mela_status st;
     std::vector<int> ones(nb, 1);
     if ((st = set_groups_locked(b, ones.data())) != MELA_OK) return st;
-    // search 1: tile width (16 / 32 words) x per-matrix mode assignment
+    // search 1: tile width x per-matrix mode assignment.  The 16- and
+    // 32-word tiles run M4RI with k = 4; the 8-word tiles also offer k = 8,
+    // tried for every M4RI matrix of a candidate and for those of >= 512 rows
+    // (a tile's 8-bit tables cost 64 LDS stores per thread, repaid by 8
+    // fewer lookups per row word once a panel updates a few hundred rows).
+    struct Plan {
+        int tw;
+        std::vector<int> md;
+    };
+    std::vector<Plan> plans;
+    for (int tw : {16, 32})
+        for (const auto &c : cands) plans.push_back({tw, c});
+    for (const auto &c : cands)
+        for (uint32_t rows : {0u, 512u}) {
+            std::vector<int> md = c;
+            bool any = false;
+            for (size_t i = 0; i < nb; ++i)
+                if (md[i] && b->mats[i].m >= rows) {
+                    md[i] = 2;
+                    any = true;
+                }
+            if (any) plans.push_back({K8_TW, md});
+        }
     float best = 1e30f;
     size_t bi = 0;
-    int best_tw = 16;
-    for (int tw : {16, 32}) {
-        b->tw = tw;
-        for (size_t c = 0; c < cands.size(); ++c) {
-            if ((st = set_modes_locked(b, cands[c].data())) != MELA_OK) return st;
-            float ms = 0.f;
-            if ((st = timed(&ms)) != MELA_OK) return st;
-            if (ms < best) {
-                best = ms;
-                bi = c;
-                best_tw = tw;
-            }
+    for (size_t c = 0; c < plans.size(); ++c) {
+        b->tw = plans[c].tw;
+        if ((st = set_modes_locked(b, plans[c].md.data())) != MELA_OK) return st;
+        float ms = 0.f;
+        if ((st = timed(&ms)) != MELA_OK) return st;
+        if (ms < best) {
+            best = ms;
+            bi = c;
         }
     }
-    b->tw = best_tw;
-    if ((st = set_modes_locked(b, cands[bi].data())) != MELA_OK) return st;
+    b->tw = plans[bi].tw;
+    if ((st = set_modes_locked(b, plans[bi].md.data())) != MELA_OK) return st;
     b->tuned = (int)bi;
     // search 2: workgroups per matrix for the head of the queue (the largest
     // matrices, whose single-workgroup time bounds the launch)

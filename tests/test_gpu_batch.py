@@ -1,7 +1,9 @@
 """Batched persistent rank + null kernel: every matrix's rank and canonical
-null basis equal the oracle's, in both elimination modes, with and without
-hipGraph replay, and equal the single-matrix path's result for the same
-matrix (same alone as in the batch)."""
+null basis equal the oracle's, in every elimination mode (GE, M4RI with
+k = 4 and k = 8 tables) and tile width (8, 16, 32 words), with and without
+hipGraph replay, with the plan the autotuner picks, and equal the
+single-matrix path's result for the same matrix (same alone as in the
+batch)."""
 import numpy as np
 import pytest
 
@@ -115,6 +117,89 @@ def test_batch_groups_rejects_bad_counts(mela, ctx):
     for bad in (0, 9, -1):
         with pytest.raises(mela.MelaError):
             b.set_groups([bad] * len(items))
+    b.close()
+    for M in dev:
+        ctx.free(M)
+
+
+def _check_batch(orc, ctx, b, items):
+    ranks, nulls = b.results()
+    for i, (P, m, n, kind) in enumerate(items):
+        r, Z = _oracle_null(orc, P, m, n)
+        assert ranks[i] == r, (i, kind, m, n)
+        got = _unpack_cols(ctx.download(nulls[i]), n, n - ranks[i])
+        assert np.array_equal(got, Z), (i, kind, m, n)
+
+
+def _tile_items():
+    # uneven sparse matrices plus dense ones wide enough for several tiles of
+    # every width and tall enough for panels of hundreds of updated rows
+    rng = np.random.default_rng(29)
+    items = W.batch_c4(60, seed=13, mmin=8, mmax=512)
+    for m, n in [(1500, 3000), (700, 1100), (2048, 520), (64, 8192), (300, 65), (1024, 2047)]:
+        items.append((W.dense_matrix(rng, m, n), m, n, "dense"))
+    items.append((W.parity_check_matrix(rng, 1800, 3000, 3), 1800, 3000, "cw3"))
+    return items
+
+
+@pytest.mark.parametrize("tw,modes,groups", [(8, "k8", None), (8, "mix", "mix"), (16, "k8", None),
+                                             (32, "mix", None), (32, "k8", "mix"), (8, "k4", 3)])
+def test_batch_tiles_and_table_bits_match_oracle(mela, ctx, orc, tw, modes, groups):
+    """Every tile width with k = 8 requested (8-bit tables with 8-word
+    tiles, 4-bit tables otherwise), GE / k = 4 / k = 8 mixed per matrix, and
+    several workgroups per matrix."""
+    items = _tile_items()
+    print(f"[batch tiles] tw={tw} modes={modes} groups={groups}", flush=True)
+    dev = [ctx.upload(P, n) for P, m, n, _ in items]
+    b = mela.Batch(ctx, dev, mela.BATCH_NULL | mela.BATCH_GRAPH)
+    b.set_tile(tw)
+    md = {"k8": [2] * len(items), "k4": [1] * len(items), "mix": [i % 3 for i in range(len(items))]}[modes]
+    b.set_modes(md)
+    if groups == "mix":
+        b.set_groups([1 + i % 8 for i in range(len(items))])
+    elif groups:
+        b.set_groups([groups] * len(items))
+    assert b.plan()[0] == tw and b.plan()[1] == md
+    for rep in range(2):
+        b.run()
+        _check_batch(orc, ctx, b, items)
+    b.close()
+    for M in dev:
+        ctx.free(M)
+
+
+def test_batch_autotuned_plan_matches_oracle(mela, ctx, orc):
+    """The autotuner's choice of tile width, per-matrix mode (GE, k = 4,
+    k = 8) and workgroups per matrix is reported by plan() and gives the
+    oracle's results."""
+    items = _tile_items()
+    dev = [ctx.upload(P, n) for P, m, n, _ in items]
+    b = mela.Batch(ctx, dev, mela.BATCH_NULL | mela.BATCH_GRAPH)
+    ms = b.autotune()
+    tw, md, gr = b.plan()
+    print(f"[batch autotune] {ms:.3f} ms tile={tw} modes={np.bincount(md, minlength=3).tolist()} "
+          f"groups>1: {sum(g > 1 for g in gr)}", flush=True)
+    assert tw in (8, 16, 32) and set(md) <= {0, 1, 2} and all(1 <= g <= 8 for g in gr)
+    assert ms > 0
+    for rep in range(2):
+        b.run()
+        _check_batch(orc, ctx, b, items)
+    b.close()
+    for M in dev:
+        ctx.free(M)
+
+
+def test_batch_plan_setters_reject_bad_values(mela, ctx):
+    items = W.batch_c4(4, seed=2, mmin=16, mmax=64)
+    dev = [ctx.upload(P, n) for P, m, n, _ in items]
+    b = mela.Batch(ctx, dev, mela.BATCH_NULL)
+    for bad in (0, 4, 12, 64):
+        with pytest.raises(mela.MelaError):
+            b.set_tile(bad)
+    for bad in (-1, 3):
+        with pytest.raises(mela.MelaError):
+            b.set_modes([bad] * len(items))
+    assert b.plan()[0] == 16 and b.plan()[1] == [1] * len(items)
     b.close()
     for M in dev:
         ctx.free(M)
