@@ -665,12 +665,14 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
         return MELA_OK;
     };
     const size_t nblk = (wc + BLK_WORDS - 1) / BLK_WORDS;
-    auto enqueue_blk = [&](size_t b_begin, size_t b_end) -> mela_status {
-        if (b_begin == 0) {
+    // blocks b_begin .. b_end-1; `start`: the state's initialisation and
+    // block 0's selection first (when b_begin == 0)
+    auto enqueue_blk = [&](size_t b_begin, size_t b_end, bool start) -> mela_status {
+        if (start && b_begin == 0) {
             mela_status zs = init_state();
             if (zs != MELA_OK) return zs;
         }
-        return enqueue_blocks(ctx, Mw, ld, act, meta, pivcols, Q, pBlk, m, n, rref, b_begin, b_end);
+        return enqueue_blocks(ctx, Mw, ld, act, meta, pivcols, Q, pBlk, m, n, rref, b_begin, b_end, start);
     };
     ElimMeta hm;
     BlockShared hb;
@@ -726,7 +728,7 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
     // the elimination complete
     const size_t units = blocked ? nblk : wc;
     auto enqueue_units = [&](size_t u0, size_t u1) -> mela_status {
-        return blocked ? enqueue_blk(u0, u1) : enqueue_panels(u0, u1, u1 == wc);
+        return blocked ? enqueue_blk(u0, u1, true) : enqueue_panels(u0, u1, u1 == wc);
     };
     auto done_unit = [&]() -> size_t {
         const int d = blocked ? hb.done_blk : hm.done_at;
@@ -766,6 +768,17 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
             eg->panels_needed = units;
             eg->seen = 1;
         }
+    } else if (blocked && eg->seen == 1) {
+        // a shape's first call: block 0's selection alone, so that a matrix
+        // whose head is too sparse for the blocked path (it gives up there)
+        // does not queue every block before it falls back
+        if ((st = enqueue_blk(0, 0, true)) != MELA_OK) return st;
+        if ((st = read_meta(true)) != MELA_OK) return st;
+        if (!hb.redo) {
+            if ((st = enqueue_blk(0, units, false)) != MELA_OK) return st;
+            if ((st = read_meta(true)) != MELA_OK) return st;
+        }
+        eg->panels_needed = done_unit() < units ? done_unit() + 1 : units;
     } else {
         if ((st = enqueue_units(0, units)) != MELA_OK) return st;
         if ((st = read_meta(blocked)) != MELA_OK) return st;

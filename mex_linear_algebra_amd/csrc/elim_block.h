@@ -37,10 +37,14 @@ constexpr size_t BLOCK_T_OFF = (BLOCK_STATE_OFF + 2 * sizeof(BlockState) + 255) 
 
 // Device bytes of the block state and operands for an m x n elimination.
 size_t block_scratch_bytes(size_t m, size_t n, bool rref);
-// Enqueues blocks b_begin .. b_end-1 (block 0 also initialises the block
-// state; the caller initialises meta, act and, for RREF, zeroes Q).
+// Enqueues blocks b_begin .. b_end-1.  With `start` (and b_begin == 0) the
+// block state's initialisation and block 0's selection come first; the
+// caller initialises meta, act and, for RREF, zeroes Q.  A call with start
+// and b_end == 0 enqueues only those, so the caller can read whether block 0
+// already gave up (BlockShared::redo) before it enqueues the rest with start
+// false.
 mela_status enqueue_blocks(mela_ctx *ctx, uint64_t *Mw, size_t ld, uint32_t *act, ElimMeta *meta,
                            int *pivcols, uint64_t *Q, void *blk, size_t m, size_t n, bool rref,
-                           size_t b_begin, size_t b_end);
+                           size_t b_begin, size_t b_end, bool start);
 
 }  // namespace mela

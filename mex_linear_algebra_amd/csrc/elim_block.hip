@@ -374,6 +374,24 @@ __device__ __forceinline__ void block_select(BlockLDS &L, const uint64_t *__rest
             rb += rho;
             __syncthreads();
         }
+        // Block 0's first word with fewer than 64 pivots and rows past the
+        // head: 512 rows spread over the rest of the active list, reduced by
+        // the panel's pivot rows (Pn, RREF in word 0); a nonzero word is a
+        // pivot the head missed, found here after one panel (the check after
+        // the last panel covers the other words)
+        if (b == 0 && p == 0 && rho < 64 && Hn < n_act) {
+            const int pos = Hn + (int)((long long)tid * (n_act - Hn) / BS_THREADS);
+            const uint64_t x = M[(size_t)act[pos] * ld + w0];
+            uint64_t r = x;
+            for (uint64_t mb = x & pm; mb; mb &= mb - 1) r ^= L.Pn[0][ffs64(mb)];
+            if (__syncthreads_or(r != 0ull)) {
+                if (tid == 0) {
+                    sh->redo = 1;
+                    bs->active = 0;
+                }
+                return;
+            }
+        }
         // the rest of the sweep (word p and the words past p+1) on waves
         // 1..15 while wave 0 selects panel p+1 from the candidates' word p+1
         // (the fast path; the whole workgroup takes the general path when
@@ -415,6 +433,44 @@ __device__ __forceinline__ void block_select(BlockLDS &L, const uint64_t *__rest
             }
             pm = 0;
             rho = 0;
+        }
+    }
+
+    // Block 0 with fewer pivots than columns and rows past the head: reduce 64
+    // rows spread over the rest of the active list by the head's pivot rows
+    // (in RREF, so row x reduces to x ^ XOR of P_c over the pivot columns c
+    // where x is set).  A nonzero remainder is a pivot the head missed, which
+    // the update would only report after it ran; give up now (the call takes
+    // the panel path).  Dense blocks of full rank skip this.
+    if (b == 0 && Hn < n_act && rb < 64 * nbw) {
+        for (int h = tid; h < Hn; h += BS_THREADS)
+            if (L.hpiv[h] >= 0) L.movers[L.hpiv[h]] = (uint32_t)h;  // pivot column -> head position
+        __syncthreads();
+        bool miss = false;
+        if (tid < 64 * BLK_WORDS) {
+            const int smp = tid / BLK_WORDS, j = tid % BLK_WORDS;
+            const int pos = Hn + (int)((long long)smp * (n_act - Hn) / 64);
+            const size_t row = act[pos];
+            uint64_t xw[BLK_WORDS];
+#pragma unroll
+            for (int q = 0; q < BLK_WORDS; ++q) xw[q] = q < nbw ? M[row * ld + w0 + q] : 0ull;
+            uint64_t r = 0;
+#pragma unroll
+            for (int q = 0; q < BLK_WORDS; ++q)
+                if (q == j) r = xw[q];
+            if (j < nbw) {
+                for (int q = 0; q < nbw; ++q)
+                    for (uint64_t mb = xw[q] & L.pmb[q]; mb; mb &= mb - 1)
+                        r ^= L.H[j][L.movers[64 * q + ffs64(mb)]];
+                miss = r != 0ull;
+            }
+        }
+        if (__syncthreads_or(miss)) {
+            if (tid == 0) {
+                sh->redo = 1;
+                bs->active = 0;
+            }
+            return;
         }
     }
 
@@ -749,7 +805,7 @@ size_t block_scratch_bytes(size_t m, size_t n, bool rref) {
 
 mela_status enqueue_blocks(mela_ctx *ctx, uint64_t *Mw, size_t ld, uint32_t *act, ElimMeta *meta,
                            int *pivcols, uint64_t *Q, void *blk, size_t m, size_t n, bool rref,
-                           size_t b_begin, size_t b_end) {
+                           size_t b_begin, size_t b_end, bool start) {
     hipStream_t s = ctx->stream;
     const size_t wc = words64(n), rmax = m < n ? m : n;
     const size_t nblk = (wc + BLK_WORDS - 1) / BLK_WORDS;
@@ -770,7 +826,7 @@ mela_status enqueue_blocks(mela_ctx *ctx, uint64_t *Mw, size_t ld, uint32_t *act
                            (const BlockState *)(bst + (b & 1)), (const uint64_t *)T, Q, ld, Xa, Bt,
                            (int)b, (int)wc, ntw, rref ? 1 : 0, xa_blocks);
     };
-    if (b_begin == 0) {
+    if (start && b_begin == 0) {
         hipLaunchKernelGGL(k_block_init, dim3(1), dim3(64), 0, s, sh, bst);
         hipLaunchKernelGGL(k_block_select, dim3(1), dim3(BS_THREADS), 0, s, (const uint64_t *)Mw, ld,
                            (const uint32_t *)act, meta, pivcols, sh, bst, T, Q, ld, (int)wc, (int)rmax);
