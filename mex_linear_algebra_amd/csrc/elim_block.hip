@@ -292,6 +292,11 @@ __device__ __forceinline__ void block_select(BlockLDS &L, const uint64_t *__rest
         const int twn = (rb + rho + 63) >> 6;
         const int nw1 = nbw - p, nwd = nw1 + twn;
         auto word_of = [&](int wi) { return wi < nw1 ? p + wi : BLK_WORDS + (wi - nw1); };
+        // x / nwd for 0 <= x < 256 (nwd <= 16) by a multiply and shift: the
+        // rounding error x * (ceil(2^16 / nwd) - 2^16 / nwd) < 256 stays
+        // below the 2^16 / nwd gap between multiples
+        const uint32_t nwd_inv = (65536u + (uint32_t)nwd - 1u) / (uint32_t)nwd;
+        auto div_nwd = [&](int x) { return (int)(((uint32_t)x * nwd_inv) >> 16); };
         // sweep of head rows h = i0, i0 + step, ... over word w with the
         // panel's tables (multipliers in xs)
         auto sweep_word = [&](int w, int i0, int step) {
@@ -318,13 +323,19 @@ __device__ __forceinline__ void block_select(BlockLDS &L, const uint64_t *__rest
             __syncthreads();
             // 4-bit tables of the selected rows (table t: selection slots 4t
             // .. 4t+3), one entry per thread (lanes = entries: consecutive
-            // stores)
+            // stores); the four rows' words are read unconditionally, so
+            // their loads issue together
             for (int idx = tid; idx < 16 * 16 * nwd; idx += BS_THREADS) {
-                const int e = idx & 15, tw = idx >> 4, t = tw / nwd, w = word_of(tw - t * nwd);
+                const int e = idx & 15, tw = idx >> 4, t = div_nwd(tw), w = word_of(tw - t * nwd);
+                uint64_t a[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int k = 4 * t + q;
+                    a[q] = L.H[w][L.selrow[k < rho ? k : 0]];
+                }
                 uint64_t v = 0;
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    if (((e >> q) & 1) && 4 * t + q < rho) v ^= L.H[w][L.selrow[4 * t + q]];
+                for (int q = 0; q < 4; ++q) v ^= (((e >> q) & 1) && 4 * t + q < rho) ? a[q] : 0ull;
                 L.Tb[t][w][e] = v;
             }
             __syncthreads();
@@ -346,11 +357,13 @@ __device__ __forceinline__ void block_select(BlockLDS &L, const uint64_t *__rest
             // and the multipliers of every head entry but this panel's
             // selected ones
             for (int idx = tid; idx < 16 * 16 * nwd; idx += BS_THREADS) {
-                const int e = idx & 15, tw = idx >> 4, t = tw / nwd, w = word_of(tw - t * nwd);
+                const int e = idx & 15, tw = idx >> 4, t = div_nwd(tw), w = word_of(tw - t * nwd);
+                uint64_t a[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) a[q] = L.Pn[w][4 * t + q];
                 uint64_t v = 0;
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    if ((e >> q) & 1) v ^= L.Pn[w][4 * t + q];
+                for (int q = 0; q < 4; ++q) v ^= ((e >> q) & 1) ? a[q] : 0ull;
                 L.Tb[t][w][e] = v;
             }
             for (int h = tid; h < Hn; h += BS_THREADS) {
