@@ -90,7 +90,9 @@ const char *mela_version(void);
  *   C (m x n logical, caller-allocated m*n bytes) = A (m x k) * B (k2 x n)
  * Checks in reference order: sparse ("Sparse matrices not supported. Convert
  * to full using full()."), k == k2 ("Inner dimensions must agree."), classes
- * ("Inputs must be logical or double matrices."). */
+ * ("Inputs must be logical or double matrices.").  Both run MELA_ALGO_AUTO
+ * (the product is the same whatever the algorithm; MELA_MATMUL_ALGO=andpop,
+ * m4rm, mfma_i8 or mfma_fp4 in the environment forces one). */
 mela_status mela_matmul_gf2(mela_ctx *ctx, const void *A, int a_class,
                             size_t m, size_t k, const void *B, int b_class,
                             size_t k2, size_t n, uint8_t *C);

@@ -303,9 +303,14 @@ mela_status mela_matmul_gf2(mela_ctx *ctx, const void *A, int a_class, size_t m,
     return matmul_entry(ctx, A, a_class, m, k, B, b_class, k2, n, C, env_algo(MELA_ALGO_AUTO));
 }
 
+// Both product gateways run AUTO: the product does not depend on the
+// algorithm (the reference's M4RI gateway differs from its GE one only in
+// speed), so the M4RI one takes the fastest engine too; k_m4rm stays
+// available through mela_matmul_packed (MELA_ALGO_M4RM) or
+// MELA_MATMUL_ALGO=m4rm.
 mela_status mela_matmul_m4ri(mela_ctx *ctx, const void *A, int a_class, size_t m, size_t k,
                              const void *B, int b_class, size_t k2, size_t n, uint8_t *C) {
-    return matmul_entry(ctx, A, a_class, m, k, B, b_class, k2, n, C, env_algo(MELA_ALGO_M4RM));
+    return matmul_entry(ctx, A, a_class, m, k, B, b_class, k2, n, C, env_algo(MELA_ALGO_AUTO));
 }
 
 mela_status mela_rank_gf2(mela_ctx *ctx, const void *A, int a_class, size_t m, size_t n,
