@@ -583,7 +583,7 @@ def bench_serve(args, mela):
                 A, B = mm[picks[j] % len(mm)]
                 t = srv.submit(mela.OP_MATMUL, A, B)
             else:
-                t = srv.submit(int(ops[j]), pool[picks[j] % len(pool)], variant=1)
+                t = srv.submit(int(ops[j]), pool[picks[j] % len(pool)], variant=mela.ELIM_AUTO)
             with lock:
                 tickets.append(t)
         th.join()

@@ -516,7 +516,11 @@ class Server:
         _check(self.lib.mela_server_create(device, hbm_budget, workers, max_batch, C.byref(h)))
         self.handle = h
 
-    def submit(self, op: int, A: BitMatrix, B: BitMatrix | None = None, variant: int = 0) -> int:
+    def submit(self, op: int, A: BitMatrix, B: BitMatrix | None = None, variant: int | None = None) -> int:
+        """variant: ELIM_* for rank / null, ALGO_* for matmul; AUTO of the
+        request's kind when None."""
+        if variant is None:
+            variant = ALGO_AUTO if op == OP_MATMUL else ELIM_AUTO
         req = Request(op, variant, A, B if B is not None else BitMatrix())
         t = C.c_uint64()
         _check(self.lib.mela_server_submit(self.handle, C.byref(req), C.byref(t)))

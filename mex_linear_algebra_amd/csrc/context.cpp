@@ -1,4 +1,5 @@
 // Context, errors and device memory of libmela.
+#include <algorithm>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
@@ -32,6 +33,12 @@ mela_status scratch_get(mela_ctx *ctx, int slot, size_t bytes, void **out) {
     if (bytes == 0) bytes = 16;
     bytes = (bytes + 255) & ~(size_t)255;
     if (sc.bytes < bytes) {
+        // a slot that has to grow at least doubles (exact size if that
+        // fails), so calls of growing sizes reallocate a few times, not at
+        // every new maximum
+        const size_t want = bytes;
+        const bool regrow = sc.ptr != nullptr;
+        if (regrow) bytes = std::max(bytes, 2 * sc.bytes);
         if (sc.ptr) {
             // the previous block may still be read by queued work
             hipError_t e = hipStreamSynchronize(ctx->stream);
@@ -48,6 +55,11 @@ mela_status scratch_get(mela_ctx *ctx, int slot, size_t bytes, void **out) {
         }
         void *p = nullptr;
         hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess && regrow && bytes > want) {
+            (void)hipGetLastError();
+            bytes = want;
+            e = hipMalloc(&p, bytes);
+        }
         if (e != hipSuccess) {
             (void)hipGetLastError();
             return set_error(MELA_ERR_NOMEM, "Out of memory.", "mela:memory");

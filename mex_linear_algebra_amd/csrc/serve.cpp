@@ -54,8 +54,10 @@ size_t ld16(size_t n) { return (words64(n) + 15) & ~(size_t)15; }
 
 // Upper bound of the device bytes a worker job adds while it runs: scratch
 // the library may grow (working copy, pivot rows, metadata and active list
-// of an elimination; re-laid, expanded or transposed operands of a product)
-// plus the output it allocates.
+// of an elimination, the blocked path's block state and operands; re-laid,
+// expanded or transposed operands of a product) plus the output it
+// allocates.  A scratch slot that grows at least doubles, so it can end at
+// up to twice what the job needs.
 size_t est_single(const mela_request &r) {
     const size_t m = r.A.rows, n = r.A.cols;
     if (r.op == MELA_OP_MATMUL) {
@@ -65,10 +67,13 @@ size_t est_single(const mela_request &r) {
         const size_t tiled = (mp + np) * ((k + 1023) / 1024 * 1024) / 8;      // fp4, packed planes
         const size_t expanded = (mp + np) * ((k + 511) / 512 * 512) / 2;      // fp4, small grids
         const size_t bt = nn * std::max<size_t>(words64(k), 1) * 8;           // AND+parity
-        return out + std::max(tiled, std::max(expanded, bt)) + ((size_t)1 << 20);
+        return out + 2 * (std::max(tiled, std::max(expanded, bt)) + ((size_t)1 << 20));
     }
     const size_t rmax = std::min(m, n);
-    size_t b = (m + rmax) * ld16(n) * 8 + m * 4 + rmax * 4 + ((size_t)64 << 10);
+    const size_t rows = m + rmax;
+    size_t scratch = rows * ld16(n) * 8 + m * 4 + rmax * 4 + ((size_t)64 << 10);
+    scratch += rows * 64 + n * 64 + ((size_t)64 << 10);  // blocked path: operands, block state
+    size_t b = 2 * scratch;
     if (r.op == MELA_OP_NULL) b += n * std::max<size_t>(words64(n), 1) * 8 + n * 8 + words64(n) * 80;
     return b;
 }
@@ -146,6 +151,7 @@ struct mela_server {
     size_t reserved = 0;        // running worker jobs' upper bounds
     size_t outputs = 0;         // worker outputs waiting for release
     std::vector<size_t> worker_cache;  // scratch held by each worker context
+    int warm_left = 0;                 // workers still warming up (mela_server_create waits)
     std::vector<char> worker_idle, worker_drop;
     size_t peak = 0;
     std::map<size_t, std::vector<void *>> pool_free;  // cached blocks by size class
@@ -207,9 +213,17 @@ void finish(mela_server *s, Req &q, mela_status st, const std::string &err,
     s->cv_sched.notify_all();
 }
 
+// Makes `device` current on the calling thread.  A server created on a
+// device that does not exist still runs (its requests fail); the failed
+// call's error is cleared so that it does not surface in the caller's next,
+// unrelated HIP call.
+void use_device(int device) {
+    if (hipSetDevice(device) != hipSuccess) (void)hipGetLastError();
+}
+
 void free_all(const std::vector<void *> &ptrs, int device) {
     if (ptrs.empty()) return;
-    (void)hipSetDevice(device);
+    use_device(device);
     for (void *p : ptrs) (void)hipFree(p);
 }
 
@@ -224,7 +238,7 @@ void ensure_loop(mela_server *s) {
         if (q == hipErrorNotReady) return;  // still running: it will take the slots
         (void)hipGetLastError();
     }
-    (void)hipSetDevice(s->device);
+    use_device(s->device);
     hipStream_t st = (hipStream_t)mela_ctx_stream(s->loop_ctx);
     if (launch_serve_loop(st, s->loop_grid, s->slots, s->done, s->tail, s->d_head, kRingCap, kLoopIdleTicks,
                           kLoopLifeTicks) != MELA_OK ||
@@ -268,7 +282,7 @@ bool pool_take(mela_server *s, std::unique_lock<std::mutex> &lk, size_t cw, size
     s->pool_held += need;  // reserved before the allocation
     s->note_peak();
     lk.unlock();
-    (void)hipSetDevice(s->device);
+    use_device(s->device);
     bool ok = true;
     if (!*pw && hipMalloc(pw, cw) != hipSuccess) {
         *pw = nullptr;
@@ -469,12 +483,40 @@ mela_status run_single(mela_ctx *ctx, Req &q) {
     return MELA_OK;
 }
 
+// A process's first launch of a kernel loads its code and a context's first
+// calls grow its scratch; a worker does both before it takes requests (rank
+// and null of a dense 2048 x 2048 matrix on the blocked and on the panel
+// path, and its square), so that early requests do not wait for them.
+// Failures here are ignored: the requests themselves report any error.
+void warm_worker(mela_ctx *ctx) {
+    constexpr size_t kN = 2048;
+    mela_bitmat A{}, Z{}, C{};
+    if (mela_bitmat_alloc(ctx, kN, kN, &A) != MELA_OK) return;
+    if (mela_gen_bernoulli(ctx, 7, &A) == MELA_OK) {
+        for (int v : {MELA_ELIM_AUTO, MELA_ELIM_PANEL}) {
+            size_t r = 0, f = 0;
+            (void)mela_rank_packed(ctx, &A, v, &r);
+            if (mela_null_packed(ctx, &A, v, &Z, &f) == MELA_OK) mela_bitmat_free(ctx, &Z);
+        }
+        if (mela_bitmat_alloc(ctx, kN, kN, &C) == MELA_OK) {
+            (void)mela_matmul_packed(ctx, &A, &A, &C, MELA_ALGO_AUTO);
+            mela_bitmat_free(ctx, &C);
+        }
+    }
+    mela_bitmat_free(ctx, &A);
+}
+
 void worker_main(mela_server *s, int w) {
     mela_ctx *ctx = nullptr;
     mela_status cst = mela_ctx_create(s->device, &ctx);
     std::string cerr = cst == MELA_OK ? "" : mela_last_error();
-    if (ctx) ctx->capture_ok = false;  // other threads issue synchronous HIP calls
+    if (ctx) {
+        ctx->capture_ok = false;  // other threads issue synchronous HIP calls
+        warm_worker(ctx);
+    }
     std::unique_lock<std::mutex> lk(s->mu);
+    s->worker_cache[(size_t)w] = ctx ? ctx_scratch_bytes(ctx) : 0;
+    if (--s->warm_left == 0) s->cv_sched.notify_all();
     for (;;) {
         s->worker_idle[(size_t)w] = 1;
         s->cv_jobs.wait(lk, [&] { return s->stop || !s->jobs.empty() || s->worker_drop[(size_t)w]; });
@@ -596,7 +638,13 @@ mela_status mela_server_create(int device, size_t hbm_budget, int workers, int m
     }
     s->sched = std::thread(scheduler_main, s);
     s->poller = std::thread(poller_main, s);
+    s->warm_left = workers;
     for (int i = 0; i < workers; ++i) s->workers.emplace_back(worker_main, s, i);
+    {
+        // requests are taken once every worker is warm
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_sched.wait(lk, [&] { return s->warm_left == 0; });
+    }
     *out = s;
     return MELA_OK;
 }
@@ -744,7 +792,7 @@ void mela_server_destroy(mela_server *s) {
     for (auto &t : s->workers) t.join();
     s->poller.join();  // returns once the loop's requests are all reported
     loop_shutdown(s);
-    (void)hipSetDevice(s->device);
+    use_device(s->device);
     std::vector<void *> blocks;
     for (auto &kv : s->reqs) {
         Req &q = kv.second;

@@ -75,7 +75,10 @@ def _submit(mela, srv, ctx, req):
         dB = ctx.upload(Bp, nb)
         return srv.submit(mela.OP_MATMUL, dA, dB), (dA, dB)
     op = mela.OP_RANK if kind == "rank" else mela.OP_NULL
-    return srv.submit(op, dA, variant=mela.ELIM_M4RI), (dA,)
+    # AUTO (blocked elimination for large dense matrices, its fallback for
+    # sparse ones) and M4RI (the panel path) alternate
+    variant = mela.ELIM_AUTO if (m + n) % 2 == 0 else mela.ELIM_M4RI
+    return srv.submit(op, dA, variant=variant), (dA,)
 
 
 def _result(ctx, req, res):
@@ -202,3 +205,13 @@ def test_loop_restarts_after_idle(mela, orc):
         for d in inputs:
             ctx.free(d)
         ctx.close()
+
+
+def test_server_on_missing_device_leaves_no_error(mela):
+    """A server on a device index that does not exist fails its requests;
+    closing it leaves no HIP error behind for the caller's next call."""
+    s = mela.Server(device=99, hbm_budget=1 << 30, workers=1)
+    s.close()
+    ctx = mela.Context(0)
+    M = ctx.bitmat(64, 64)  # would raise "invalid device ordinal" from a stale error
+    ctx.free(M)
