@@ -88,20 +88,30 @@ k_mm_transpose(const MMItem *__restrict__ items, int nitems, uint64_t total_bloc
     }
 }
 
-// The persistent product: every workgroup takes queue tiles until none is
-// left (the tile index and the item are workgroup-uniform).
+// Queue tiles a workgroup takes per atomic: a row-XOR tile is a few row
+// loads per wave, less than the take itself (atomic, two barriers, the item
+// search), so takes come in runs of consecutive tiles.
+constexpr int MM_TAKE = 8;
+
+// The persistent product: every workgroup takes runs of queue tiles until
+// none is left (the tile index and the item are workgroup-uniform).  The
+// item of a run's first tile is found by binary search; later tiles of the
+// run step to the next item when they pass its first tile.
 __global__ void __launch_bounds__(256)
 k_mm_batch(const MMItem *__restrict__ items, int nitems, uint64_t total_tiles,
            unsigned long long *__restrict__ counter, const uint64_t *__restrict__ arena) {
     __shared__ unsigned long long s_t;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (;;) {
-        if (threadIdx.x == 0) s_t = atomicAdd(counter, 1ull);
+        if (threadIdx.x == 0) s_t = atomicAdd(counter, (unsigned long long)MM_TAKE);
         __syncthreads();
-        const uint64_t t = s_t;
+        const uint64_t t0 = s_t;
         __syncthreads();  // every thread has read s_t before the next take
-        if (t >= total_tiles) break;
-        const int ii = find_item(nitems, t, [&](int j) { return items[j].tile_begin; });
+        if (t0 >= total_tiles) break;
+        const uint64_t t1 = t0 + MM_TAKE < total_tiles ? t0 + MM_TAKE : total_tiles;
+        int ii = find_item(nitems, t0, [&](int j) { return items[j].tile_begin; });
+        for (uint64_t t = t0; t < t1; ++t) {
+        while (ii + 1 < nitems && items[ii + 1].tile_begin <= t) ++ii;
         const MMItem it = items[ii];
         const uint64_t lt = t - it.tile_begin;
         const uint32_t tx = (uint32_t)(lt % it.tiles_x), ty = (uint32_t)(lt / it.tiles_x);
@@ -113,6 +123,7 @@ k_mm_batch(const MMItem *__restrict__ items, int nitems, uint64_t total_tiles,
             if (row < (int)it.m)
                 dev::rowxor_unit(it.A, it.lda, it.B, it.ldb, it.C, it.ldc, row, (int)(ty * 64) + lane,
                                  (int)((it.k + 63) / 64), (int)((it.n + 63) / 64));
+        }
         }
     }
 }
