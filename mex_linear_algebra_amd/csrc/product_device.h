@@ -113,12 +113,15 @@ __device__ __forceinline__ void andpop_tile(const uint64_t *__restrict__ A, size
 // GE product (matmul_gf2.c:127-186 XORs B_k into the row-block accumulator
 // for every odd A(i,k)), whose cost is nnz(A) row reads instead of m*n*k bit
 // operations, so it is the product of choice for sparse A.  One wave per
-// (row i, 64-word chunk of the row of C): lanes are words, the bits of A_i
-// are walked in scalar registers, RX_BATCH row loads of B in flight together
-// (an empty slot re-reads row 64 kq, a valid row, and masks it).  A_i is read
-// 64 words per vector load.
+// (row i, 64-word chunk of the row of C): lanes are words.  A_i is read 64
+// words per vector load (lane q holds word kb + q); each round takes the
+// lowest set bit of up to RX_BATCH different nonzero words and has their
+// rows of B in flight together, so a sparse row (about one set bit per
+// nonzero word) costs one round trip per RX_BATCH bits, not one per bit.
+// An empty slot re-reads the row of lane 0's lowest bit (or row 64 kb),
+// which lies below k, and masks it.
 constexpr int RX_WAVES = 4;
-constexpr int RX_BATCH = 4;
+constexpr int RX_BATCH = 8;
 
 // Row i of C, word w (lane-varying): XOR of B's word w over the set bits of
 // A_i (i is wave-uniform; all 64 lanes of the wave take part).
@@ -131,32 +134,27 @@ __device__ __forceinline__ void rowxor_unit(const uint64_t *__restrict__ A, size
     const uint64_t *arow = A + (size_t)i * lda;
     const uint64_t *bcol = B + (wok ? w : 0);
     uint64_t c = 0;
-    // 64 words of A_i per vector load (lane q holds word kb + q); the
-    // nonzero ones are broadcast one by one and their bits walked
     for (int kb = 0; kb < kw; kb += 64) {
-        const uint64_t aw = kb + lane < kw ? arow[kb + lane] : 0ull;
-        uint64_t nz = __ballot(aw != 0ull);
-        while (nz) {
-        const int src = __builtin_ctzll(nz);
-        nz &= nz - 1ull;
-        const int kq = kb + src;
-        // readlane returns int: both halves through uint32_t (no sign extension)
-        const uint32_t xlo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)aw, src);
-        const uint32_t xhi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(aw >> 32), src);
-        uint64_t x = ((uint64_t)xhi << 32) | xlo;
-        while (x) {
+        uint64_t x = kb + lane < kw ? arow[kb + lane] : 0ull;
+        for (uint64_t nz = __ballot(x != 0ull); nz; nz = __ballot(x != 0ull)) {
+            const int myb = x ? __builtin_ctzll(x) : 0;  // this word's lowest set bit
             uint64_t v[RX_BATCH], keep[RX_BATCH];
+            uint64_t taken = 0;
 #pragma unroll
             for (int q = 0; q < RX_BATCH; ++q) {
-                const bool has = x != 0ull;
-                const int bit = has ? __builtin_ctzll(x) : 0;
-                x &= x - 1ull;
+                const bool has = nz != 0ull;
+                const int src = has ? __builtin_ctzll(nz) : 0;
+                if (has) {
+                    nz &= nz - 1ull;
+                    taken |= 1ull << src;
+                }
+                const int bit = __builtin_amdgcn_readlane(myb, src);
                 keep[q] = has ? ~0ull : 0ull;
-                v[q] = bcol[(size_t)(kq * 64 + bit) * ldb];
+                v[q] = bcol[(size_t)((kb + src) * 64 + bit) * ldb];
             }
 #pragma unroll
             for (int q = 0; q < RX_BATCH; ++q) c ^= v[q] & keep[q];
-        }
+            if ((taken >> lane) & 1ull) x &= x - 1ull;
         }
     }
     if (wok) C[(size_t)i * ldc + w] = c;
