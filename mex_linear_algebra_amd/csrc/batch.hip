@@ -70,7 +70,8 @@ struct BatchLDS {
     // One region, used in turn per tile: the selected rows' words, their
     // tables, the pivot rows' tables (M4RI) or the pivot rows (GE); the null
     // extraction's column word tables reuse it after the elimination.
-    union {
+    // 16-byte aligned: the sweep reads two words of a table entry at once
+    union alignas(16) {
         uint64_t S[64][TW];      // selected rows' words of the tile
         uint64_t T[16][16][TW];  // 4-bit tables (of the selected rows, then of the pivot rows)
         uint64_t P[64][TW];      // pivot rows of the tile, by pivot bit (GE sweep, 8-bit tables)
@@ -308,11 +309,14 @@ k_rref_loop(const Src src_arg) {
                 __syncthreads();
             }
         }
-        constexpr int RPP = 64 / TW;  // rows per wave pass, TW lanes (words) per row
+        // the sweep's lanes: TW / 2 per row, two consecutive words each
+        // (one 16-byte table read serves both); RPP rows per wave pass
+        constexpr int LPR = TW / 2;
+        constexpr int RPP = 64 / LPR;
         // row passes per wave with loads in flight together (the 512-thread
         // kernel has twice the waves per SIMD and half the registers)
-        constexpr int SWP = TW == K8_TW ? SW_PASSES / 2 : SW_PASSES;
-        const int sub = lane / TW, wl = lane % TW;
+        constexpr int SWP = TW == K8_TW ? SW_PASSES / 4 : SW_PASSES / 2;
+        const int sub = lane / LPR, wp = lane % LPR;
         for (int w0 = g + mk * TW; w0 < wc; w0 += G * TW) {
             // selected rows' words of the tile (loads issued together;
             // slots past rho and words past wc read row 0 and store 0)
@@ -413,16 +417,17 @@ k_rref_loop(const Src src_arg) {
                 }
             }
             __syncthreads();
-            // sweep words > g of the tile; SWP passes' loads together
-            const int w = w0 + wl;
-            const bool wok = w < wc && w > g;
-            const int wsafe = wok ? w : g;
+            // sweep words > g of the tile, words w and w + 1 per lane; SWP
+            // passes' loads together
+            const int w = w0 + 2 * wp;
+            const bool ok0 = w < wc && w > g, ok1 = w + 1 < wc && w + 1 > g;
+            const int ws0 = ok0 ? w : g, ws1 = ok1 ? w + 1 : g;
 #pragma unroll 1
             for (int i0 = wave * RPP + sub; i0 < nupd; i0 += BT_WAVES * RPP * SWP) {
                 // every pass loads unconditionally (entries past the end
                 // re-read row 0 of M) so all loads are in flight together
                 uint64_t *rp[SWP];
-                uint64_t x[SWP], a[SWP];
+                uint64_t x[SWP], a0[SWP], a1[SWP];
 #pragma unroll
                 for (int j = 0; j < SWP; ++j) {
                     const int i = i0 + j * BT_WAVES * RPP;
@@ -431,12 +436,20 @@ k_rref_loop(const Src src_arg) {
 #pragma unroll
                 for (int j = 0; j < SWP; ++j) x[j] = rp[j][g];
 #pragma unroll
-                for (int j = 0; j < SWP; ++j) a[j] = rp[j][wsafe];
+                for (int j = 0; j < SWP; ++j) {
+                    a0[j] = rp[j][ws0];
+                    a1[j] = rp[j][ws1];
+                }
 #pragma unroll
                 for (int j = 0; j < SWP; ++j) {
                     const int i = i0 + j * BT_WAVES * RPP;
-                    x[j] = (i < nupd && wok) ? x[j] & pm : 0ull;
+                    x[j] = (i < nupd && (ok0 || ok1)) ? x[j] & pm : 0ull;
                 }
+                auto acc2 = [&](int j, const uint64_t *e) {
+                    const uint4 q = *reinterpret_cast<const uint4 *>(e);
+                    a0[j] ^= ((uint64_t)q.y << 32) | q.x;
+                    a1[j] ^= ((uint64_t)q.w << 32) | q.z;
+                };
                 if (kb == 4) {
                     // table-major: one lookup of every pass per table, so
                     // the passes' LDS reads overlap (a zero multiplier
@@ -444,22 +457,25 @@ k_rref_loop(const Src src_arg) {
 #pragma unroll
                     for (int t = 0; t < 16; ++t)
 #pragma unroll
-                        for (int j = 0; j < SWP; ++j) a[j] ^= L.u.T[t][(x[j] >> (4 * t)) & 15u][wl];
+                        for (int j = 0; j < SWP; ++j) acc2(j, &L.u.T[t][(x[j] >> (4 * t)) & 15u][2 * wp]);
                 } else if (kb == 8) {
                     if constexpr (TW == K8_TW) {
 #pragma unroll
                         for (int t = 0; t < T8_N; ++t)
 #pragma unroll
-                            for (int j = 0; j < SWP; ++j) a[j] ^= L.u.T8[t][(x[j] >> (8 * t)) & 255u][wl];
+                            for (int j = 0; j < SWP; ++j) acc2(j, &L.u.T8[t][(x[j] >> (8 * t)) & 255u][2 * wp]);
                     }
                 } else {
 #pragma unroll
                     for (int j = 0; j < SWP; ++j)
-                        for (uint64_t xx = x[j]; xx; xx &= xx - 1) a[j] ^= L.u.P[dev::ffs64(xx)][wl];
+                        for (uint64_t xx = x[j]; xx; xx &= xx - 1) acc2(j, &L.u.P[dev::ffs64(xx)][2 * wp]);
                 }
 #pragma unroll
                 for (int j = 0; j < SWP; ++j)
-                    if (x[j]) rp[j][w] = a[j];
+                    if (x[j]) {
+                        if (ok0) rp[j][w] = a0[j];
+                        if (ok1) rp[j][w + 1] = a1[j];
+                    }
             }
             __syncthreads();
         }
