@@ -71,6 +71,12 @@ struct mela_ctx {
     // graph capture of repeated eliminations; off for contexts that share the
     // process with other threads doing synchronous HIP calls (server workers)
     bool capture_ok = true;
+    // a scratch slot that grows keeps its old block here instead of freeing
+    // it: hipFree waits for every stream's work on the device, which a
+    // server worker must not do in the middle of a request (freed by
+    // ctx_release_scratch and at destruction)
+    bool defer_free = false;
+    std::vector<std::pair<void *, size_t>> retired;
     // event pairs around the dominant kernel of each product while profiling
     bool prof_on = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof;

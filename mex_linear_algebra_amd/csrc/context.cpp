@@ -40,16 +40,21 @@ mela_status scratch_get(mela_ctx *ctx, int slot, size_t bytes, void **out) {
         const bool regrow = sc.ptr != nullptr;
         if (regrow) bytes = std::max(bytes, 2 * sc.bytes);
         if (sc.ptr) {
-            // the previous block may still be read by queued work
-            hipError_t e = hipStreamSynchronize(ctx->stream);
-            if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
             // captured eliminations may point into it
             for (auto &g : ctx->elim_graphs) {
                 if (g.exec) (void)hipGraphExecDestroy(g.exec);
                 if (g.graph) (void)hipGraphDestroy(g.graph);
             }
             ctx->elim_graphs.clear();
-            (void)hipFree(sc.ptr);
+            if (ctx->defer_free) {
+                // queued work may still read it; it stays allocated
+                ctx->retired.emplace_back(sc.ptr, sc.bytes);
+            } else {
+                // the previous block may still be read by queued work
+                hipError_t e = hipStreamSynchronize(ctx->stream);
+                if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+                (void)hipFree(sc.ptr);
+            }
             sc.ptr = nullptr;
             sc.bytes = 0;
         }
@@ -75,6 +80,7 @@ size_t ctx_scratch_bytes(mela_ctx *ctx) {
     CtxLock lk(ctx->mu);
     size_t b = 0;
     for (const auto &sc : ctx->scratch) b += sc.bytes;
+    for (const auto &r : ctx->retired) b += r.second;
     return b;
 }
 
@@ -92,6 +98,8 @@ void ctx_release_scratch(mela_ctx *ctx) {
         sc.ptr = nullptr;
         sc.bytes = 0;
     }
+    for (auto &r : ctx->retired) (void)hipFree(r.first);
+    ctx->retired.clear();
 }
 
 void pinned_release(mela_ctx *ctx) {
@@ -204,6 +212,7 @@ void mela_ctx_destroy(mela_ctx *ctx) {
     }
     for (auto &s : ctx->scratch)
         if (s.ptr) (void)hipFree(s.ptr);
+    for (auto &r : ctx->retired) (void)hipFree(r.first);
     pinned_release(ctx);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;

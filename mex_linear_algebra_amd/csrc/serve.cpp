@@ -512,6 +512,7 @@ void worker_main(mela_server *s, int w) {
     std::string cerr = cst == MELA_OK ? "" : mela_last_error();
     if (ctx) {
         ctx->capture_ok = false;  // other threads issue synchronous HIP calls
+        ctx->defer_free = true;   // no device-wide wait inside a request
         warm_worker(ctx);
     }
     std::unique_lock<std::mutex> lk(s->mu);
