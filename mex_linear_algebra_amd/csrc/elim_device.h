@@ -35,6 +35,37 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
 __device__ __forceinline__ int popc64(uint64_t x) { return __popcll(x); }
 __device__ __forceinline__ int ffs64(uint64_t x) { return __ffsll((unsigned long long)x) - 1; }
 
+// XOR of v over the 64 lanes of the wave (every lane receives the result;
+// all 64 lanes must be active).  DPP row shifts give the inclusive XOR of
+// each 16-lane row, two row broadcasts carry rows 0-2 into lane 63, whose
+// value is read out: six VALU steps per half instead of six LDS permutes.
+__device__ __forceinline__ uint32_t wave_xor32(uint32_t v) {
+    v ^= __builtin_amdgcn_update_dpp(0u, v, 0x111, 0xF, 0xF, true);   // row_shr:1
+    v ^= __builtin_amdgcn_update_dpp(0u, v, 0x112, 0xF, 0xF, true);   // row_shr:2
+    v ^= __builtin_amdgcn_update_dpp(0u, v, 0x114, 0xF, 0xF, true);   // row_shr:4
+    v ^= __builtin_amdgcn_update_dpp(0u, v, 0x118, 0xF, 0xF, true);   // row_shr:8
+    v ^= __builtin_amdgcn_update_dpp(0u, v, 0x142, 0xA, 0xF, false);  // row_bcast:15 -> rows 1, 3
+    v ^= __builtin_amdgcn_update_dpp(0u, v, 0x143, 0xC, 0xF, false);  // row_bcast:31 -> rows 2, 3
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
+__device__ __forceinline__ uint64_t wave_xor64(uint64_t v) {
+    const uint32_t lo = wave_xor32((uint32_t)v), hi = wave_xor32((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// OR of v over the 64 lanes of the wave, the same DPP pattern as wave_xor32
+// (all 64 lanes must be active).
+__device__ __forceinline__ uint32_t wave_or32(uint32_t v) {
+    v |= __builtin_amdgcn_update_dpp(0u, v, 0x111, 0xF, 0xF, true);   // row_shr:1
+    v |= __builtin_amdgcn_update_dpp(0u, v, 0x112, 0xF, 0xF, true);   // row_shr:2
+    v |= __builtin_amdgcn_update_dpp(0u, v, 0x114, 0xF, 0xF, true);   // row_shr:4
+    v |= __builtin_amdgcn_update_dpp(0u, v, 0x118, 0xF, 0xF, true);   // row_shr:8
+    v |= __builtin_amdgcn_update_dpp(0u, v, 0x142, 0xA, 0xF, false);  // row_bcast:15 -> rows 1, 3
+    v |= __builtin_amdgcn_update_dpp(0u, v, 0x143, 0xC, 0xF, false);  // row_bcast:31 -> rows 2, 3
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
 // Wave-level Gauss-Jordan over the 64-bit words v (one per lane; lanes with
 // valid == false take no part).  Afterwards pivot lanes hold the RREF basis of
 // span{v} (pbit = pivot bit = lowest set bit of the lane's word), non-pivot
@@ -42,19 +73,23 @@ __device__ __forceinline__ int ffs64(uint64_t x) { return __ffsll((unsigned long
 // lane.  Pivot of column c = the lowest non-pivot lane with bit c set.  The
 // pivot set is kept as a wave-uniform mask so each column costs one ballot,
 // one s_ff1, two (four) v_readlane and an exec-masked XOR.
+// A row XOR never sets a bit in a column that no row has, so only the
+// columns of the rows' OR can take pivots; with fewer than this many of them
+// (a sparse panel word) the elimination visits those columns alone.
+constexpr int GJ_SPARSE_COLS = 48;
+
 template <bool TRACK>
 __device__ __forceinline__ void wave_gj(uint64_t &v, uint64_t &comb, bool valid, bool &piv,
                                         int &pbit, int lane) {
     uint32_t lo = valid ? (uint32_t)v : 0u, hi = valid ? (uint32_t)(v >> 32) : 0u;
     uint32_t clo = (uint32_t)comb, chi = (uint32_t)(comb >> 32);
     uint64_t pivmask = 0;  // wave-uniform
-#pragma unroll
-    for (int c = 0; c < 64; ++c) {
+    auto column = [&](int c) {
         const uint32_t word = c < 32 ? lo : hi;
         const bool has = (word >> (c & 31)) & 1u;
         const uint64_t hb = __ballot(has);
         const uint64_t cand = hb & ~pivmask;
-        if (cand == 0) continue;
+        if (cand == 0) return;
         const int f = ffs64(cand);
         const uint32_t plo = __builtin_amdgcn_readlane(lo, f);
         const uint32_t phi = __builtin_amdgcn_readlane(hi, f);
@@ -72,6 +107,13 @@ __device__ __forceinline__ void wave_gj(uint64_t &v, uint64_t &comb, bool valid,
                 chi ^= pchi;
             }
         }
+    };
+    const uint64_t cols = ((uint64_t)wave_or32(hi) << 32) | wave_or32(lo);
+    if (popc64(cols) >= GJ_SPARSE_COLS) {
+#pragma unroll
+        for (int c = 0; c < 64; ++c) column(c);
+    } else {
+        for (uint64_t cs = cols; cs; cs &= cs - 1) column(ffs64(cs));
     }
     v = ((uint64_t)hi << 32) | lo;
     if (TRACK) comb = ((uint64_t)chi << 32) | clo;
@@ -109,37 +151,6 @@ __host__ __device__ __forceinline__ uint64_t compress_apply(uint64_t x, uint64_t
         x = (x ^ t) | (t >> (1 << i));
     }
     return x;
-}
-
-// XOR of v over the 64 lanes of the wave (every lane receives the result;
-// all 64 lanes must be active).  DPP row shifts give the inclusive XOR of
-// each 16-lane row, two row broadcasts carry rows 0-2 into lane 63, whose
-// value is read out: six VALU steps per half instead of six LDS permutes.
-__device__ __forceinline__ uint32_t wave_xor32(uint32_t v) {
-    v ^= __builtin_amdgcn_update_dpp(0u, v, 0x111, 0xF, 0xF, true);   // row_shr:1
-    v ^= __builtin_amdgcn_update_dpp(0u, v, 0x112, 0xF, 0xF, true);   // row_shr:2
-    v ^= __builtin_amdgcn_update_dpp(0u, v, 0x114, 0xF, 0xF, true);   // row_shr:4
-    v ^= __builtin_amdgcn_update_dpp(0u, v, 0x118, 0xF, 0xF, true);   // row_shr:8
-    v ^= __builtin_amdgcn_update_dpp(0u, v, 0x142, 0xA, 0xF, false);  // row_bcast:15 -> rows 1, 3
-    v ^= __builtin_amdgcn_update_dpp(0u, v, 0x143, 0xC, 0xF, false);  // row_bcast:31 -> rows 2, 3
-    return __builtin_amdgcn_readlane(v, 63);
-}
-
-__device__ __forceinline__ uint64_t wave_xor64(uint64_t v) {
-    const uint32_t lo = wave_xor32((uint32_t)v), hi = wave_xor32((uint32_t)(v >> 32));
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// OR of v over the 64 lanes of the wave, the same DPP pattern as wave_xor32
-// (all 64 lanes must be active).
-__device__ __forceinline__ uint32_t wave_or32(uint32_t v) {
-    v |= __builtin_amdgcn_update_dpp(0u, v, 0x111, 0xF, 0xF, true);   // row_shr:1
-    v |= __builtin_amdgcn_update_dpp(0u, v, 0x112, 0xF, 0xF, true);   // row_shr:2
-    v |= __builtin_amdgcn_update_dpp(0u, v, 0x114, 0xF, 0xF, true);   // row_shr:4
-    v |= __builtin_amdgcn_update_dpp(0u, v, 0x118, 0xF, 0xF, true);   // row_shr:8
-    v |= __builtin_amdgcn_update_dpp(0u, v, 0x142, 0xA, 0xF, false);  // row_bcast:15 -> rows 1, 3
-    v |= __builtin_amdgcn_update_dpp(0u, v, 0x143, 0xC, 0xF, false);  // row_bcast:31 -> rows 2, 3
-    return __builtin_amdgcn_readlane(v, 63);
 }
 
 // Pivots the first 64 active rows must give for the selection to complete
