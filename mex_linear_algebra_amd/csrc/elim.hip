@@ -102,7 +102,8 @@ k_panel_select(const uint64_t *__restrict__ M, size_t ld, uint32_t *__restrict__
     }
     auto rowfn = [&](int q) { return M + (size_t)q * ld; };
     const dev::PanelOut po = dev::wg_panel_select<PS_THREADS>(L, act, n_act, g, rowfn, meta->comb,
-                                                              meta->sel_row, have_pre, pre, actc);
+                                                              meta->sel_row, have_pre, pre, actc,
+                                                              have_pre ? panelw : nullptr);
     const int tid = threadIdx.x;
     // slots past the selection keep a valid row id (the sweep loads every slot)
     if (tid < 64 && tid >= po.rho) meta->sel_row[tid] = 0;
@@ -410,7 +411,7 @@ k_sweep(uint64_t *__restrict__ M, size_t ld, const uint32_t *__restrict__ act,
 #pragma unroll
             for (int j = 0; j < R; ++j) {
                 const int q = q0 + j * RPW;
-                if (q < n_act && q < PS_THREADS) store_through(&panelw[q], a[j]);
+                if (q < n_act) store_through(&panelw[q], a[j]);
             }
         if (q0 + stride < total) {
             load_rows(q0 + stride, false);
@@ -537,7 +538,7 @@ ElimLayout elim_layout(size_t m, size_t n) {
     ElimLayout L;
     const size_t wc = words64(n), rmax = m < n ? m : n;
     L.ld = (wc + 15) & ~(size_t)15;
-    L.meta_bytes = round256(meta_panelw_off(rmax) + PS_THREADS * sizeof(uint64_t));
+    L.meta_bytes = round256(meta_panelw_off(rmax) + std::max<size_t>(m, PS_THREADS) * sizeof(uint64_t));
     L.act_off = L.meta_bytes;
     L.m_off = round256(L.act_off + m * sizeof(uint32_t));
     L.q_off = round256(L.m_off + m * L.ld * 8);
@@ -577,7 +578,7 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
     } else {
         if ((st = scratch_get(ctx, SLOT_ELIM_M, m * ld * 8, &pM)) != MELA_OK) return st;
         if ((st = scratch_get(ctx, SLOT_ELIM_Q, rmax * ld * 8, &pQ)) != MELA_OK) return st;
-        if ((st = scratch_get(ctx, SLOT_ELIM_META, panelw_off + PS_THREADS * sizeof(uint64_t),
+        if ((st = scratch_get(ctx, SLOT_ELIM_META, panelw_off + std::max<size_t>(m, PS_THREADS) * sizeof(uint64_t),
                               &pMeta)) != MELA_OK)
             return st;
         if ((st = scratch_get(ctx, SLOT_ELIM_ACT, m * sizeof(uint32_t), &pAct)) != MELA_OK)

@@ -190,6 +190,8 @@ struct PanelOut {
 //   act      : active list (global or LDS), ACT = its element type
 //   pre      : if have_pre, thread t's panel word of active-list entry t
 //              (t < min(n_act, NT)), already loaded by the caller
+//   prew     : optional panel words of all active-list entries, in list
+//              order (read instead of each row's word g past the first NT)
 //   actc     : optional LDS copy of act (ACT_HEAD + 64 entries) for a global
 //              act: waves 1-4 fill it with act[0 .. ACT_HEAD) and the last 64
 //              entries while wave 0 runs the fast path, so the final
@@ -198,7 +200,8 @@ constexpr int ACT_HEAD = 192;  // positions the fast path can select (64 + 128)
 template <int NT, typename ACT, typename RowFn>
 __device__ PanelOut wg_panel_select(SelectLDS<NT> &L, ACT *act, int n_act, int g, RowFn row,
                                     uint64_t *comb_out, int *selrow_out, bool have_pre = false,
-                                    uint64_t pre = 0, uint32_t *actc = nullptr) {
+                                    uint64_t pre = 0, uint32_t *actc = nullptr,
+                                    const uint64_t *prew = nullptr) {
     // the copy needs waves 1-4 (workgroups of at least 320 threads)
     if (NT < 64 + ACT_HEAD + 64) actc = nullptr;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -281,7 +284,8 @@ __device__ PanelOut wg_panel_select(SelectLDS<NT> &L, ACT *act, int n_act, int g
         uint64_t x = pos < n_act ? xnext : 0ull;
         {
             const int pn = pos + NT;
-            xnext = row((int)act[pn < n_act ? pn : n_act - 1])[g];
+            const int pc = pn < n_act ? pn : n_act - 1;
+            xnext = prew ? prew[pc] : row((int)act[pc])[g];
         }
         const uint64_t pm = L.pm;
         uint64_t y = x;
@@ -424,7 +428,7 @@ __device__ PanelOut wg_panel_select(SelectLDS<NT> &L, ACT *act, int n_act, int g
             // RREF of the selected rows' panel words, tracking combinations
             uint64_t v = 0, comb = 0;
             if (lane < nsel) {
-                v = row(L.selrow[lane])[g];
+                v = prew ? prew[L.selpos[lane]] : row(L.selrow[lane])[g];
                 comb = 1ull << lane;
             }
             bool piv;
