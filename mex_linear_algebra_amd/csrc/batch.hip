@@ -15,8 +15,10 @@
 // Per panel and tile of TW words: the selected rows' words are staged in LDS,
 // the tile's pivot rows are formed from 4-bit tables of them (and stored to
 // Q), the matrix's M4RI tables are built from those, and the sweep runs over
-// the panel's list of rows with a nonzero multiplier, 16 row passes per wave
-// with their loads in flight together, looked up table by table.
+// the panel's list of rows with a nonzero multiplier: each lane takes two
+// consecutive words of a row (one 16-byte table read serves both), several
+// row passes per wave have their loads in flight together, and the lookups
+// go table by table.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -43,7 +45,7 @@ namespace {
 constexpr int MAX_M = 4096;
 constexpr int MAX_N = 8192;
 constexpr int MAX_WC = MAX_N / 64;
-constexpr int SW_PASSES = 16;    // row passes per wave with loads in flight together
+constexpr int SW_PASSES = 16;    // row passes in flight per wave, before the per-kernel cut
 constexpr int MAX_GROUP = 8;     // workgroups per matrix at most
 // a group member that waits this long (wall clock ticks, 100 MHz: 0.5 s)
 // for its partners gives up and flags the batch as failed
