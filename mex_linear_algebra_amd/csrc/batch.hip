@@ -777,6 +777,9 @@ size_t batch_lds_bytes(int tw) {
 // index of a tile width (8, 16, 32 words) in per-width tables
 int tw_slot(int tw) { return tw == K8_TW ? 0 : tw == 16 ? 1 : 2; }
 
+// whether the device granted the 8-word tile kernel its LDS (set once)
+bool g_k8_ok = true;
+
 // Serving-loop workgroups: a quarter of the CUs.  The loop stays resident
 // between requests, so the CUs it does not occupy stay free for the kernels
 // of large requests on the worker streams (an MFMA product workgroup needs a
@@ -974,9 +977,16 @@ mela_status mela_batch_create(mela_ctx *ctx, const mela_bitmat *mats, size_t cou
     static bool attr_set = false;
     std::lock_guard<std::mutex> alk(attr_mu);
     if (!attr_set) {
-        for (int tw : {K8_TW, 16, 32})
+        for (int tw : {16, 32})
             MELA_HIP(hipFuncSetAttribute(batch_kernel(tw), hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)batch_lds_bytes(tw)));
+        // the 8-word tiles' LDS (about 157 KiB) is optional: a device that
+        // does not grant it keeps the other tile widths
+        if (hipFuncSetAttribute(batch_kernel(K8_TW), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)batch_lds_bytes(K8_TW)) != hipSuccess) {
+            (void)hipGetLastError();
+            g_k8_ok = false;
+        }
         attr_set = true;
     }
     mela_batch *b = new mela_batch();
@@ -1070,6 +1080,7 @@ mela_status mela_batch_create(mela_ctx *ctx, const mela_bitmat *mats, size_t cou
         MELA_HIP(hipMemcpyAsync(b->d_inputs, b->inputs.data(), nb * sizeof(void *), hipMemcpyHostToDevice, s));
     }
     for (int tw : {K8_TW, 16, 32}) {
+        if (tw == K8_TW && !g_k8_ok) continue;
         int occ = 0;
         MELA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batch_kernel(tw), batch_threads(tw),
                                                               batch_lds_bytes(tw)));
@@ -1125,6 +1136,8 @@ mela_status mela_batch_set_tile(mela_batch *b, int tile_words) {
     if (!b) return set_error(MELA_ERR_INVALID, "null batch");
     if (tile_words != 8 && tile_words != 16 && tile_words != 32)
         return set_error(MELA_ERR_INVALID, "batch tile width must be 8, 16 or 32 words");
+    if (tile_words == K8_TW && !g_k8_ok)
+        return set_error(MELA_ERR_INVALID, "8-word batch tiles need more LDS per workgroup than this device grants");
     CtxLock lk(b->ctx->mu);
     (void)hipSetDevice(b->ctx->device);
     b->tw = tile_words;
@@ -1223,6 +1236,7 @@ mela_status autotune_locked(mela_batch *b, float *best_ms) {
         for (const auto &c : cands) plans.push_back({tw, c});
     for (const auto &c : cands)
         for (uint32_t rows : {0u, 512u}) {
+            if (!g_k8_ok) break;
             std::vector<int> md = c;
             bool any = false;
             for (size_t i = 0; i < nb; ++i)
