@@ -488,6 +488,8 @@ mela_status run_single(mela_ctx *ctx, Req &q) {
 // and null of a dense 2048 x 2048 matrix on the blocked and on the panel
 // path, and its square), so that early requests do not wait for them.
 // Failures here are ignored: the requests themselves report any error.
+constexpr size_t kWarmBudgetPerWorker = (size_t)256 << 20;
+
 void warm_worker(mela_ctx *ctx) {
     constexpr size_t kN = 2048;
     mela_bitmat A{}, Z{}, C{};
@@ -512,8 +514,10 @@ void worker_main(mela_server *s, int w) {
     std::string cerr = cst == MELA_OK ? "" : mela_last_error();
     if (ctx) {
         ctx->capture_ok = false;  // other threads issue synchronous HIP calls
-        ctx->defer_free = true;   // no device-wide wait inside a request
-        warm_worker(ctx);
+        // the warm-up's scratch (a few MiB) is held from the start, so it
+        // runs only where the budget leaves ample room for it
+        if (s->budget >= (size_t)s->nworkers * kWarmBudgetPerWorker) warm_worker(ctx);
+        ctx->defer_free = true;  // no device-wide wait inside a request
     }
     std::unique_lock<std::mutex> lk(s->mu);
     s->worker_cache[(size_t)w] = ctx ? ctx_scratch_bytes(ctx) : 0;
