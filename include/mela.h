@@ -271,8 +271,9 @@ mela_status mela_batch_set_modes(mela_batch *batch, const int *modes);
 /* update tile width in 64-bit words: 8 (the only width with k = 8 tables),
  * 16 (default) or 32 */
 mela_status mela_batch_set_tile(mela_batch *batch, int tile_words);
-/* the plan the next run uses: tile width, per-matrix modes and workgroups
- * per matrix (each output may be NULL; modes and groups hold count entries) */
+/* the plan the next run uses: tile width, per-matrix modes (as set: a 2
+ * reported with 16- or 32-word tiles runs as 1) and workgroups per matrix
+ * (each output may be NULL; modes and groups hold count entries) */
 mela_status mela_batch_get_plan(mela_batch *batch, int *tile_words, int *modes, int *groups);
 /* per-matrix workgroups (1..8) that eliminate the matrix together: the
  * tiles of each panel are dealt out among them, one device-scope barrier per

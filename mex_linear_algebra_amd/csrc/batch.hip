@@ -1200,7 +1200,14 @@ mela_status autotune_locked(mela_batch *b, float *best_ms) {
     add([&](size_t i) { return dens[i] > 0.05; });                     // dense -> M4RI
     add([&](size_t i) { return dens[i] > 0.05 && b->mats[i].n > 512; });
     add([&](size_t i) { return b->mats[i].n > 1024; });
-    hipEvent_t e0, e1;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    struct EventPair {  // destroyed on every return, early ones included
+        hipEvent_t &a, &b;
+        ~EventPair() {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } events{e0, e1};
     MELA_HIP(hipEventCreate(&e0));
     MELA_HIP(hipEventCreate(&e1));
     // time of the second of two runs (warm); a group wait time-out counts as
@@ -1225,7 +1232,7 @@ mela_status autotune_locked(mela_batch *b, float *best_ms) {
     // search 1: tile width x per-matrix mode assignment.  The 16- and
     // 32-word tiles run M4RI with k = 4; the 8-word tiles also offer k = 8,
     // tried for every M4RI matrix of a candidate and for those of >= 512 rows
-    // (a tile's 8-bit tables cost 64 LDS stores per thread, repaid by 8
+    // (a tile's 8-bit tables cost 32 LDS stores per thread, repaid by 8
     // fewer lookups per row word once a panel updates a few hundred rows).
     struct Plan {
         int tw;
@@ -1278,8 +1285,6 @@ mela_status autotune_locked(mela_batch *b, float *best_ms) {
             }
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     // also drops a graph captured before tuning (old tile width and grid)
     if ((st = set_groups_locked(b, gbest.data())) != MELA_OK) return st;
     if (best_ms) *best_ms = best;
