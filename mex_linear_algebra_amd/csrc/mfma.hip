@@ -619,6 +619,35 @@ k_transpose_tile_packed(const uint64_t *__restrict__ B, size_t ldb, size_t k, si
     p[1] = make_uint4((uint32_t)o[2], (uint32_t)(o[2] >> 32), (uint32_t)o[3], (uint32_t)(o[3] >> 32));
 }
 
+// Plane J of a wave's fragments (mfma_device.h's plane map).
+template <int J, int T>
+__device__ __forceinline__ void planes(const uint4 (&r)[T], pk::v8i (&o)[T]) {
+#pragma unroll
+    for (int t = 0; t < T; ++t) o[t] = pk::plane<J>(r[t]);
+}
+
+// The TR x TR MFMAs of plane J from extracted operands.
+template <int J, int TA, int TB>
+__device__ __forceinline__ void mfmas(const pk::v8i (&fa)[TA], const pk::v8i (&fb)[TB], pk::v16f (&acc)[TA][TB]) {
+#pragma unroll
+    for (int tr = 0; tr < TA; ++tr)
+#pragma unroll
+        for (int tc = 0; tc < TB; ++tc)
+            acc[tr][tc] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+                fa[tr], fb[tc], acc[tr][tc], 4, 4, 0, pk::scale<J>(), 0, pk::scale<J>());
+}
+
+// Scheduling pattern of a region: NM times one MFMA, then NV vector ALU
+// instructions.
+template <int NM, int NV>
+__device__ __forceinline__ void interleave() {
+#pragma unroll
+    for (int i = 0; i < NM; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x002, NV, 0);
+    }
+}
+
 // k_mfma_direct: 256 x 256 tiles, 4 waves in a 2 x 2 grid, each 128 x 128
 // (4 x 4 MFMA tiles, 256 accumulators), one workgroup per CU, the fragments
 // loaded straight into registers.  In the PK layout lane l's
@@ -666,18 +695,47 @@ k_mfma_direct(const uint8_t *__restrict__ A, const uint8_t *__restrict__ B, int 
 #pragma unroll
     for (int j = 0; j < D; ++j) load(j, fa[j], fb[j]);
     __builtin_amdgcn_sched_barrier(0);
-    // the host pads k to whole trips of D stages; the scheduling barriers keep
-    // each refill where it is issued (the scheduler would otherwise sink the
-    // loads to their use D steps later and wait for them there)
+    // Each plane's 16 MFMAs run beside the extraction of the next plane's
+    // operands (even planes in xa/xb, odd ones in qa/qb); the last plane's
+    // run beside the refill of the stage's fragments and the extraction of
+    // the next stage's plane 0, so the matrix core never waits for the
+    // vector ALU at a stage boundary.  The host pads k to whole trips of D
+    // stages; the group barriers pin every instruction class to its slot
+    // (the scheduler would otherwise sink the loads to their use D steps
+    // later and wait for them there).
+    pk::v8i xa[TR], xb[TR], qa[TR], qb[TR];
+    planes<0>(fa[0], xa);
+    planes<0>(fb[0], xb);
+    __builtin_amdgcn_sched_barrier(0);
     for (int st = 0; st < nst; st += D) {
 #pragma unroll
         for (int j = 0; j < D; ++j) {
-            pk::mm<0>(fa[j], fb[j], acc);
-            pk::mm<1>(fa[j], fb[j], acc);
-            pk::mm<2>(fa[j], fb[j], acc);
-            pk::mm<3>(fa[j], fb[j], acc);
+            planes<1>(fa[j], qa);
+            planes<1>(fb[j], qb);
+            mfmas<0>(xa, xb, acc);
+            interleave<16, 2>();
+            __builtin_amdgcn_sched_barrier(0);
+            planes<2>(fa[j], xa);
+            planes<2>(fb[j], xb);
+            mfmas<1>(qa, qb, acc);
+            interleave<16, 2>();
+            __builtin_amdgcn_sched_barrier(0);
+            planes<3>(fa[j], qa);
+            planes<3>(fb[j], qb);
+            mfmas<2>(xa, xb, acc);
+            interleave<16, 4>();
             __builtin_amdgcn_sched_barrier(0);
             load(st + j + D, fa[j], fb[j]);
+            planes<0>(fa[(j + 1) % D], xa);
+            planes<0>(fb[(j + 1) % D], xb);
+            mfmas<3>(qa, qb, acc);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read
+                __builtin_amdgcn_sched_group_barrier(0x002, 1, 0);  // VALU
+            }
+            interleave<8, 4>();
             __builtin_amdgcn_sched_barrier(0);
         }
     }
