@@ -42,7 +42,8 @@ struct Scratch {
 struct ElimGraph {
     size_t m = 0, n = 0, ld = 0;
     int variant = 0, rref = 0;
-    const void *bufs[4] = {nullptr, nullptr, nullptr, nullptr};
+    // the work buffers the captured launches use: M, Q, meta, act, block state
+    const void *bufs[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     int seen = 0;
     // panels (blocks, on the blocked path) the last uncaptured run needed
     // (the first one whose selection found the elimination complete, plus

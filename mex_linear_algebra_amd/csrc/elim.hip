@@ -703,11 +703,13 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
         return read_pivots();
     }
     // replay a captured sequence for a shape seen before (the work buffers are
-    // the context's cached scratch, so their addresses repeat)
+    // the context's cached scratch, so their addresses repeat; a buffer that
+    // moved, e.g. the block state grown by an RREF of the same shape, makes a
+    // new entry)
     ElimGraph *eg = nullptr;
     for (auto &e : ctx->elim_graphs)
         if (e.m == m && e.n == n && e.ld == ld && e.variant == variant && e.rref == (int)rref &&
-            e.bufs[0] == Mw && e.bufs[1] == Q && e.bufs[2] == meta && e.bufs[3] == act)
+            e.bufs[0] == Mw && e.bufs[1] == Q && e.bufs[2] == meta && e.bufs[3] == act && e.bufs[4] == pBlk)
             eg = &e;
     if (!eg) {
         // bounded cache: the oldest shape (and its executable graph) makes room
@@ -720,7 +722,7 @@ mela_status eliminate(mela_ctx *ctx, const uint64_t *A, size_t lda, size_t m, si
         }
         ElimGraph e;
         e.m = m, e.n = n, e.ld = ld, e.variant = variant, e.rref = rref ? 1 : 0;
-        e.bufs[0] = Mw, e.bufs[1] = Q, e.bufs[2] = meta, e.bufs[3] = act;
+        e.bufs[0] = Mw, e.bufs[1] = Q, e.bufs[2] = meta, e.bufs[3] = act, e.bufs[4] = pBlk;
         ctx->elim_graphs.push_back(e);
         eg = &ctx->elim_graphs.back();
     }

@@ -161,3 +161,38 @@ def test_blocked_repeated_calls_replay(ctx, mela, orc):
             assert np.array_equal(Rd[: len(opiv), : M.shape[1]], M[: len(opiv)])
         finally:
             ctx.free(dA)
+
+
+def test_blocked_replay_after_the_block_state_moved(mela, orc):
+    """A shape's rank sequence is captured, then an RREF of the same shape
+    grows the block state (it holds the earlier pivot rows' operands too), so
+    that buffer moves: later rank calls of the shape take a sequence of their
+    own instead of replaying launches on the old block state.  A fresh
+    context, so the RREF is what grows the buffer."""
+    rng = np.random.default_rng(29)
+    A = _dense(rng, 2048, 2048)
+    L = _low_rank(rng, 2048, 2048, 1000)
+    c = mela.Context(0)
+    try:
+        for X in (A, L, A):
+            P = _packed(X)
+            M, opiv = orc.rref_packed(P, 2048, 2048, "gf2")
+            r = len(opiv)
+            print(f"[blocked] moved block state, rank {r}", flush=True)
+            dA = c.upload(P, 2048)
+            try:
+                for _ in range(3):
+                    assert c.rank_packed(dA, mela.ELIM_BLOCKED) == r
+                Rm, piv = c.rref_packed(dA, mela.ELIM_BLOCKED)
+                Rd = c.download(Rm)
+                c.free(Rm)
+                assert list(piv) == list(opiv)
+                assert np.array_equal(Rd[:r, : M.shape[1]], M[:r])
+                for _ in range(3):
+                    assert c.rank_packed(dA, mela.ELIM_BLOCKED) == r
+            finally:
+                c.free(dA)
+        blocked, fallback, _ = c.elim_counts()
+        assert fallback == 0 and blocked == 21
+    finally:
+        c.close()
