@@ -547,36 +547,53 @@ constexpr int PK_SEG = XROWS * PK_STAGE;     // one stage of one 256-row block: 
 #endif
 constexpr int PK_DIRECT = MELA_PK_DIRECT;    // stages in flight in k_mfma_direct
 
-// A (rows x k bits, lds words per row) -> PK layout.  One thread per (row,
-// stage) 32-byte segment; four consecutive lanes take four consecutive
-// stages of one row, so a wave reads 16 runs of 128 contiguous source bytes
-// and writes 4 runs of 16 rows x 32 bytes.  grid.x: groups of 4 stages (the
-// last one partial when nst is not a multiple of 4),
-// grid.y: 64-row groups.  Rows >= rows and words >= kw are written as zeros.
-// count != nullptr: also adds the number of set bits of A to the GATE_SLOTS
-// counts at count (the density gate of the AUTO product; one atomic per
-// workgroup).
+// A (rows x k bits, lds words per row) -> PK layout.  One thread per row of
+// a 256-row block and TP_STAGES consecutive stages: it reads 256 contiguous
+// source bytes (16-byte loads when `vec`: src 16-byte aligned, lds even) and
+// writes its 32 bytes of each stage segment, so a wave's stores are runs of
+// 64 rows x 32 bytes.  (One thread per (row, stage), four lanes per row, ran
+// at 2.5 TB/s against 5.9 TB/s for this form at 32768 x 32768.)  grid.x:
+// groups of TP_STAGES stages (the last one partial when nst is not a
+// multiple), grid.y: 256-row blocks.  Rows >= rows and words >= kw are
+// written as zeros.  count != nullptr: also adds the number of set bits of A
+// to the GATE_SLOTS counts at count (the density gate of the AUTO product;
+// one atomic per workgroup).
+constexpr int TP_STAGES = 8;
 __global__ void __launch_bounds__(256)
-k_tile_packed(const uint64_t *__restrict__ src, size_t lds, size_t rows, size_t kw, size_t nst,
+k_tile_packed(const uint64_t *__restrict__ src, size_t lds, size_t rows, size_t kw, size_t nst, bool vec,
               uint8_t *__restrict__ dst, unsigned long long *__restrict__ count) {
     const int l = threadIdx.x;
-    const size_t s = (size_t)blockIdx.x * 4 + (l & 3);
-    const size_t i = (size_t)blockIdx.y * 64 + (l >> 2);
-    uint64_t v[4] = {0ull, 0ull, 0ull, 0ull};
-    if (s < nst) {
+    const size_t i = (size_t)blockIdx.y * XROWS + l;
+    const size_t s0 = (size_t)blockIdx.x * TP_STAGES;
+    const uint64_t *row = src + i * lds + 4 * s0;
+    uint4 v[2 * TP_STAGES];
+    if (vec && i < rows && 4 * (s0 + TP_STAGES) <= kw) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const size_t w = 4 * s + q;
-            v[q] = (i < rows && w < kw) ? src[i * lds + w] : 0ull;
+        for (int t = 0; t < 2 * TP_STAGES; ++t) v[t] = reinterpret_cast<const uint4 *>(row)[t];
+    } else {
+#pragma unroll
+        for (int t = 0; t < 2 * TP_STAGES; ++t) {
+            const size_t w = 4 * s0 + 2 * t;
+            const uint64_t x = (i < rows && w < kw) ? row[2 * t] : 0ull;
+            const uint64_t y = (i < rows && w + 1 < kw) ? row[2 * t + 1] : 0ull;
+            v[t] = make_uint4((uint32_t)x, (uint32_t)(x >> 32), (uint32_t)y, (uint32_t)(y >> 32));
         }
-        uint4 *p = reinterpret_cast<uint4 *>(dst + ((i / XROWS) * nst + s) * PK_SEG + (i % XROWS) * PK_STAGE);
-        p[0] = make_uint4((uint32_t)v[0], (uint32_t)(v[0] >> 32), (uint32_t)v[1], (uint32_t)(v[1] >> 32));
-        p[1] = make_uint4((uint32_t)v[2], (uint32_t)(v[2] >> 32), (uint32_t)v[3], (uint32_t)(v[3] >> 32));
+    }
+    unsigned long long c = 0;
+#pragma unroll
+    for (int q = 0; q < TP_STAGES; ++q) {
+        if (s0 + q < nst) {
+            uint4 *p = reinterpret_cast<uint4 *>(dst + (blockIdx.y * nst + s0 + q) * PK_SEG + (size_t)l * PK_STAGE);
+            p[0] = v[2 * q];
+            p[1] = v[2 * q + 1];
+        }
+        if (count)
+            c += (unsigned long long)(__popc(v[2 * q].x) + __popc(v[2 * q].y) + __popc(v[2 * q].z) +
+                                      __popc(v[2 * q].w) + __popc(v[2 * q + 1].x) + __popc(v[2 * q + 1].y) +
+                                      __popc(v[2 * q + 1].z) + __popc(v[2 * q + 1].w));
     }
     if (count) {
         __shared__ unsigned long long part[4];
-        unsigned long long c = (unsigned long long)(__popcll(v[0]) + __popcll(v[1]) + __popcll(v[2]) +
-                                                    __popcll(v[3]));
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
         if ((l & 63) == 0) part[l >> 6] = c;
@@ -587,36 +604,60 @@ k_tile_packed(const uint64_t *__restrict__ src, size_t lds, size_t rows, size_t 
     }
 }
 
-// B (k x n bits) -> B^T in PK layout.  Each wave transposes the four 64 x 64
-// bit blocks of one stage (256 k rows) for 64 columns; lane c then holds
-// column 64 jb + c's 256 k bits and writes them as its row's 32 bytes (a
-// wave writes 2 KiB of consecutive output).  grid.x: groups of 4 stages
-// (one per wave), grid.y: 64-column blocks up to the padded width.  Columns
-// >= n and k rows >= k come out as zeros.
+// B (k x n bits) -> B^T in PK layout.  Each wave transposes the 64 x 64 bit
+// blocks of one stage (256 k rows) for four column words: a lane reads 32
+// contiguous bytes of each of its four k rows (16-byte loads when `vec`: B
+// 16-byte aligned, ldb even), and after the in-wave transposes lane c holds
+// column 64 jb + c's 256 k bits of each word jb, which it writes as its
+// row's 32 bytes (a wave writes four runs of 2 KiB).  (One column word per
+// wave ran at 3.1 TB/s against 5.4 TB/s for this form at 32768 x 32768.)
+// grid.x: groups of 4 stages (one per wave), grid.y: 256-column blocks of
+// the padded width.  Columns >= n and k rows >= k come out as zeros.
 __global__ void __launch_bounds__(256)
-k_transpose_tile_packed(const uint64_t *__restrict__ B, size_t ldb, size_t k, size_t n, size_t nst,
+k_transpose_tile_packed(const uint64_t *__restrict__ B, size_t ldb, size_t k, size_t n, size_t nst, bool vec,
                         uint8_t *__restrict__ dst, Gate gate) {
     if (gate_skip(gate, false)) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const size_t s = (size_t)blockIdx.x * 4 + wave;
-    const size_t jb = blockIdx.y;
+    const size_t jb = 4 * (size_t)blockIdx.y;
     if (s >= nst) return;
     const size_t nw = (n + 63) >> 6;
-    uint64_t o[4];
+    uint64_t v[4][4];
 #pragma unroll
     for (int h = 0; h < 4; ++h) {
         const size_t i = (4 * s + h) * 64 + lane;
-        uint64_t v = 0;
-        if (i < k && jb < nw) {
-            v = B[i * ldb + jb];
-            if (jb == nw - 1 && (n & 63)) v &= (1ull << (n & 63)) - 1ull;
+        const uint64_t *row = B + i * ldb + jb;
+        if (vec && i < k && jb + 3 < nw) {
+            const uint4 q0 = reinterpret_cast<const uint4 *>(row)[0];
+            const uint4 q1 = reinterpret_cast<const uint4 *>(row)[1];
+            v[0][h] = ((uint64_t)q0.y << 32) | q0.x;
+            v[1][h] = ((uint64_t)q0.w << 32) | q0.z;
+            v[2][h] = ((uint64_t)q1.y << 32) | q1.x;
+            v[3][h] = ((uint64_t)q1.w << 32) | q1.z;
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) v[c][h] = (i < k && jb + c < nw) ? row[c] : 0ull;
         }
-        o[h] = dev::transpose64(v, lane);
     }
-    const size_t j = jb * 64 + lane;
-    uint4 *p = reinterpret_cast<uint4 *>(dst + ((j / XROWS) * nst + s) * PK_SEG + (j % XROWS) * PK_STAGE);
-    p[0] = make_uint4((uint32_t)o[0], (uint32_t)(o[0] >> 32), (uint32_t)o[1], (uint32_t)(o[1] >> 32));
-    p[1] = make_uint4((uint32_t)o[2], (uint32_t)(o[2] >> 32), (uint32_t)o[3], (uint32_t)(o[3] >> 32));
+    if (n & 63) {
+        const uint64_t lastmask = (1ull << (n & 63)) - 1ull;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (jb + c == nw - 1)
+#pragma unroll
+                for (int h = 0; h < 4; ++h) v[c][h] &= lastmask;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int h = 0; h < 4; ++h) v[c][h] = dev::transpose64(v[c][h], lane);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const size_t j = (jb + c) * 64 + lane;
+        uint4 *p = reinterpret_cast<uint4 *>(dst + ((j / XROWS) * nst + s) * PK_SEG + (j % XROWS) * PK_STAGE);
+        p[0] = make_uint4((uint32_t)v[c][0], (uint32_t)(v[c][0] >> 32), (uint32_t)v[c][1], (uint32_t)(v[c][1] >> 32));
+        p[1] = make_uint4((uint32_t)v[c][2], (uint32_t)(v[c][2] >> 32), (uint32_t)v[c][3], (uint32_t)(v[c][3] >> 32));
+    }
 }
 
 // Plane J of a wave's fragments (mfma_device.h's plane map).
@@ -762,15 +803,17 @@ mela_status matmul_mfma_packed(mela_ctx *ctx, const uint64_t *A, size_t lda, con
     mela_status st;
     if ((st = scratch_get(ctx, SLOT_XA, mp * nst * PK_STAGE, &pa)) != MELA_OK) return st;
     if ((st = scratch_get(ctx, SLOT_XB, np * nst * PK_STAGE, &pb)) != MELA_OK) return st;
-    hipLaunchKernelGGL(k_tile_packed, dim3((unsigned)((nst + 3) / 4), (unsigned)(mp / 64)), dim3(256), 0, s,
-                       A, lda, m, kw, nst, (uint8_t *)pa, count);
+    const bool vec_a = ((uintptr_t)A & 15) == 0 && (lda & 1) == 0;
+    const bool vec_b = ((uintptr_t)B & 15) == 0 && (ldb & 1) == 0;
+    hipLaunchKernelGGL(k_tile_packed, dim3((unsigned)((nst + TP_STAGES - 1) / TP_STAGES), (unsigned)(mp / XROWS)),
+                       dim3(256), 0, s, A, lda, m, kw, nst, vec_a, (uint8_t *)pa, count);
     if (count && sparse_path) {
         // the gate is complete once A is tiled: the row-XOR product goes here
         mela_status rs = sparse_path(gate);
         if (rs != MELA_OK) return rs;
     }
-    hipLaunchKernelGGL(k_transpose_tile_packed, dim3((unsigned)((nst + 3) / 4), (unsigned)(np / 64)), dim3(256), 0,
-                       s, B, ldb, k, n, nst, (uint8_t *)pb, gate);
+    hipLaunchKernelGGL(k_transpose_tile_packed, dim3((unsigned)((nst + 3) / 4), (unsigned)(np / XROWS)), dim3(256),
+                       0, s, B, ldb, k, n, nst, vec_b, (uint8_t *)pb, gate);
     MELA_HIP(hipGetLastError());
     const size_t tx = mp / XROWS, ty = np / XROWS;
     if (tx * ty > (size_t)INT32_MAX) return set_error(MELA_ERR_INVALID, "too many tiles for one launch");

@@ -112,10 +112,12 @@ def test_matmul_packed_algos(mela, ctx, orc, algo, shape):
 
 @pytest.mark.parametrize("algo", [0, 3, 4])
 @pytest.mark.parametrize("shape", [(4100, 1000, 4000), (4096, 4160, 4293), (4352, 64, 4097),
-                                   (4100, 1, 4100), (4096, 129, 4352)])
+                                   (4100, 1, 4100), (4096, 129, 4352), (4100, 2500, 4200)])
 def test_matmul_mfma_large_tiles_exact(mela, ctx, orc, algo, shape):
-    # grids of >= 256 tiles of 256 x 256 take the LDS-ring kernel; ragged m,
-    # k and n exercise the zero padding and the last-word mask
+    # grids of >= 256 tiles of 256 x 256 take the stage-tiled kernels; ragged
+    # m, k and n exercise the zero padding and the last-word mask, odd row
+    # lengths the 8-byte loads of the operand re-layouts, and 2500 x 4200 (40
+    # and 66 words per row) their 16-byte loads with a partial last group
     m, k, n = shape
     print(f"[matmul mfma large tiles] algo={algo} shape={shape}", flush=True)
     Ap = orc.gen_bernoulli_packed(21, m, k)
