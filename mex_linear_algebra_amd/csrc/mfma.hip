@@ -249,6 +249,45 @@ __device__ __forceinline__ void store_tile(Acc (&acc)[TR][TR], int row0, int col
         }
 }
 
+// C tile epilogue of the direct kernel, whose accumulators start at 2^23:
+// a count c < 2^23 then leaves 2^23 + c exactly, with c in the mantissa, so
+// bit 0 of the float is the parity.  Per 64 x 64 quarter of the wave's
+// 128 x 128 (2 x 2 MFMA tiles): each lane packs its 64 parities into one
+// word with one v_alignbit per bit (register r of lane l: row 8(r/4) +
+// 4(l/32) + r%4, column l%32 of its tile), the in-wave 64 x 64 bit transpose
+// gives lane j the 64 lanes' bit j, i.e. 32 columns of two rows, and lanes j
+// and j^16 (the two column tiles) combine their halves into one row's word,
+// which lane j stores.
+template <int TR>
+__device__ __forceinline__ void store_tile_offset(v16f (&acc)[TR][TR], int row0, int col0,
+                                                  uint64_t *__restrict__ C, size_t ldc, int m, int n) {
+    static_assert(TR % 2 == 0, "quarters of 2 x 2 MFMA tiles");
+    const int lane = threadIdx.x & 63;
+    const int nw = (n + 63) >> 6;
+    const uint64_t lastmask = (n & 63) ? ((1ull << (n & 63)) - 1ull) : ~0ull;
+    const int tri = lane >> 5, tci = (lane >> 4) & 1, r = lane & 15;
+#pragma unroll
+    for (int cr = 0; cr < TR / 2; ++cr)
+#pragma unroll
+        for (int cc = 0; cc < TR / 2; ++cc) {
+            uint32_t xh[2] = {0u, 0u};
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int e = 0; e < 16; ++e)
+                        xh[q] = __builtin_amdgcn_alignbit(__float_as_uint(acc[2 * cr + q][2 * cc + h][e]), xh[q], 1);
+            const uint64_t y = dev::transpose64(((uint64_t)xh[1] << 32) | xh[0], lane);
+            const uint64_t yp = dev::shfl_xor64(y, 16);
+            uint64_t word = tci == 0 ? ((y & 0xffffffffull) | (yp << 32)) : ((yp >> 32) | (y & 0xffffffff00000000ull));
+            const int row = row0 + cr * 64 + tri * 32 + 8 * (r >> 2) + (r & 3) + 4 * tci;
+            const int w = (col0 >> 6) + cc;
+            if (w == nw - 1) word &= lastmask;
+            if (row < m && w < nw) C[(size_t)row * ldc + w] = word;
+        }
+}
+
 // ---------------------------------------------------------------------------
 // k_mfma_small: 128 x 128 tiles for grids too small to fill the chip with
 // 256 x 256 tiles.  4 waves in a 2 x 2 grid, each 64 x 64 (2 x 2 MFMA tiles);
@@ -722,7 +761,7 @@ k_mfma_direct(const uint8_t *__restrict__ A, const uint8_t *__restrict__ B, int 
 #pragma unroll
         for (int b = 0; b < TR; ++b)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
+            for (int r = 0; r < 16; ++r) acc[a][b][r] = 8388608.0f;  // 2^23: parity in bit 0
     uint4 fa[D][TR], fb[D][TR];
     // stage st's fragments (past the last stage: the last one again)
     auto load = [&](int st, uint4 (&ra)[TR], uint4 (&rb)[TR]) __attribute__((always_inline)) {
@@ -780,7 +819,7 @@ k_mfma_direct(const uint8_t *__restrict__ A, const uint8_t *__restrict__ B, int 
             __builtin_amdgcn_sched_barrier(0);
         }
     }
-    store_tile<TR>(acc, i0 + wr * WT, j0 + wc * WT, C, ldc, m, n);
+    store_tile_offset<TR>(acc, i0 + wr * WT, j0 + wc * WT, C, ldc, m, n);
 }
 
 }  // namespace
@@ -840,7 +879,8 @@ mela_status matmul_mfma(mela_ctx *ctx, const uint64_t *A, size_t lda, const uint
             const char *e = std::getenv("MELA_MFMA_EXPANDED");
             return e && *e && *e != '0';
         }();
-        if (fp4 && !expanded && t256 >= (size_t)ctx->cu_count)
+        // (k < 2^23: the direct kernel's accumulators start at 2^23)
+        if (fp4 && !expanded && t256 >= (size_t)ctx->cu_count && k < ((size_t)1 << 23))
             return matmul_mfma_packed(ctx, A, lda, B, ldb, C, ldc, m, k, n, gate, count, sparse_path);
     }
     const size_t kw = words64(k);
