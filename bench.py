@@ -48,7 +48,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant product kernel, measured with two
 # rocprofv3 --pmc passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a
 # 1 GiB copy kernel) on this workload; profiles/k_mfma_direct_fp4_32768.md.
-MEASURED_TRAFFIC = {("mfma_fp4", 32768): 6.578e9}
+MEASURED_TRAFFIC = {("mfma_fp4", 32768): 6.611e9}
 
 
 class HipEvents:
