@@ -382,3 +382,43 @@ def test_error_messages(mela):
         mela.mela_null_gf2(A.astype(np.uint16))
     with pytest.raises(mela.MelaError, match="Need two input matrices A,B."):
         mela.mela_matmul_gf2(A, A, A)
+
+
+_EXPANDED_CHILD = r"""
+import sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+import mex_linear_algebra_amd as mela
+from oracle import oracle as orc
+orc.load()
+ctx = mela.Context(0)
+for m, k, n in [(4100, 1000, 4000), (4096, 4160, 4293), (4352, 2500, 4200)]:
+    print(f"[matmul fp4 expanded] {m}x{k}x{n}", flush=True)
+    Ap = orc.gen_bernoulli_packed(31, m, k)
+    Bp = orc.gen_bernoulli_packed(32, k, n)
+    dA, dB = ctx.upload(Ap, k), ctx.upload(Bp, n)
+    dC = ctx.bitmat(m, n)
+    ctx.matmul_packed(dA, dB, dC, 4)
+    got = ctx.download(dC)
+    want = orc.matmul_packed(Ap, m, k, Bp, n, "m4ri")
+    for M in (dA, dB, dC):
+        ctx.free(M)
+    assert np.array_equal(got[:, : want.shape[1]], want), (m, k, n)
+print("expanded ok", flush=True)
+"""
+
+
+def test_matmul_fp4_expanded_kernels_exact():
+    """The expanded-operand fp4 kernels (operands stored as e2m1 nibbles, LDS
+    ring), which grids of >= 256 tiles take when k >= 2^23 (the direct
+    kernel's accumulators start at 2^23), forced with MELA_MFMA_EXPANDED=1 in
+    a child process at sizes the oracle checks in seconds."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, MELA_MFMA_EXPANDED="1")
+    out = subprocess.run([sys.executable, "-c", _EXPANDED_CHILD, root], capture_output=True, text=True,
+                         timeout=240, env=env)
+    print(out.stdout[-2000:], out.stderr[-2000:], flush=True)
+    assert out.returncode == 0 and "expanded ok" in out.stdout
