@@ -737,7 +737,9 @@ __device__ __forceinline__ void interleave() {
 // the two waves that share a row block meet in the L1.  No LDS, no barrier:
 // every wave waits only for its own loads, D stages ahead.  (An LDS ring
 // filled by global_load_lds, one barrier per stage, ran 9.9 ms at 32768^3
-// against 9.0 ms for this form.)
+// against 9.0 ms for this form.)  The accumulators start at 2^23, so the
+// epilogue (store_tile_offset) reads each parity from bit 0 of its float;
+// the host sends k >= 2^23 to the expanded-operand kernels.
 template <int D>
 __global__ void __launch_bounds__(256, 1)
 k_mfma_direct(const uint8_t *__restrict__ A, const uint8_t *__restrict__ B, int nst,
