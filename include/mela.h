@@ -155,7 +155,11 @@ enum {
     MELA_ALGO_ANDPOP = 1, /* AND + parity over a transposed B (v_bitop3) */
     MELA_ALGO_M4RM = 2,   /* Method of Four Russians, Gray-code tables in LDS */
     MELA_ALGO_MFMA_I8 = 3, /* exact int8 MFMA, i32 accumulate, & 1 */
-    MELA_ALGO_MFMA_FP4 = 4, /* exact fp4 (e2m1) MFMA, f32 accumulate, & 1 */
+    MELA_ALGO_MFMA_FP4 = 4, /* exact fp4 (e2m1) MFMA, f32 accumulate, & 1; k < 2^24.
+                               Grids of >= one 256 x 256 tile per CU with k < 2^23 read the
+                               packed bits as fp4 planes in registers; others (or any, with
+                               MELA_MFMA_EXPANDED=1 in the environment) expand the operands
+                               to e2m1 nibbles first */
     MELA_ALGO_ROWXOR = 5  /* C_i ^= B_k for each set bit A(i,k) (matmul_gf2.c:127-186) */
 };
 
