@@ -496,7 +496,8 @@ def bench_batch_matmul(args, mela):
            "products_per_s": round(rate, 1), "Tbitops": round(work / wall / 1e12, 2),
            "plain_one_per_launch_products_per_s": round(plain_rate, 1),
            "plain_sample": len(sidx), "speedup_vs_plain": round(rate / plain_rate, 2),
-           "items_andpop": algos.count(mela.ALGO_ANDPOP), "items_rowxor": algos.count(mela.ALGO_ROWXOR)}
+           "items_andpop": algos.count(mela.ALGO_ANDPOP), "items_rowxor": algos.count(mela.ALGO_ROWXOR),
+           "items_mfma": algos.count(mela.ALGO_MFMA_FP4)}
     b.close()
     for M in dA + dB + dC:
         ctx.free(M)

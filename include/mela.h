@@ -301,8 +301,9 @@ void mela_batch_destroy(mela_batch *batch);
  * matmul_gf2.c:72): C_i = A_i * B_i for thousands of uneven packed device
  * products in one persistent launch.  mela_matmul_batch_create validates the
  * shapes (A_i.cols == B_i.rows, C_i is A_i.rows x B_i.cols), counts the set
- * bits of every A_i and gives each item AND+parity (dense A_i) or the row-XOR
- * product (sparse A_i), whichever the measured kernel rates say is cheaper;
+ * bits of every A_i and gives each item AND+parity, the fp4 matrix cores
+ * (128 x 128 tiles, k < 2^23) or the row-XOR product (sparse A_i), whichever
+ * the measured kernel rates say is cheapest;
  * mela_matmul_batch_run enqueues all products (one hipGraph launch with
  * MELA_BATCH_GRAPH) on the context stream.  The C_i receive their products
  * (words past each row's last column are left as they are); the inputs must
@@ -311,8 +312,8 @@ typedef struct mela_matmul_batch mela_matmul_batch;
 mela_status mela_matmul_batch_create(mela_ctx *ctx, const mela_bitmat *A, const mela_bitmat *B,
                                      const mela_bitmat *C, size_t count, int flags,
                                      mela_matmul_batch **batch);
-/* per-item engine: MELA_ALGO_ANDPOP or MELA_ALGO_ROWXOR (set, or read the
- * planned choice) */
+/* per-item engine: MELA_ALGO_ANDPOP, MELA_ALGO_MFMA_FP4 (k < 2^23) or
+ * MELA_ALGO_ROWXOR (set, or read the planned choice) */
 mela_status mela_matmul_batch_set_algos(mela_matmul_batch *batch, const int *algos);
 mela_status mela_matmul_batch_get_algos(mela_matmul_batch *batch, int *algos);
 mela_status mela_matmul_batch_run(mela_matmul_batch *batch);

@@ -3,20 +3,23 @@
 // matmul_gf2.c:72 / matmul_m4ri.c:140).
 //
 // Planning (mela_matmul_batch_create) counts the set bits of every A_i on the
-// device and gives each item the cheaper of two engines by the measured
-// rates of the single-product kernels: AND+parity over B_i^T (cost m n k bit
-// operations) or the row-XOR product (one B_i-row read per set bit of A_i,
-// the reference's own GE product, matmul_gf2.c:127-186).  Items are queued
-// largest first; each item is cut into output tiles (64 x 256 bits for
-// AND+parity, 4 rows x 64 words for row XOR).  A run is three launches,
-// captured once in a hipGraph: reset of the queue counter, one batched
-// B^T transpose for the AND+parity items, and the persistent product kernel
-// whose workgroups take tiles from the queue until it is empty.
+// device and gives each item the cheapest of three engines by the measured
+// rates of the kernels: AND+parity over B_i^T (cost m n k bit operations),
+// the fp4 matrix cores over A_i and B_i^T (the same count at their rate) or
+// the row-XOR product (one B_i-row read per set bit of A_i, the reference's
+// own GE product, matmul_gf2.c:127-186).  Items are queued largest first;
+// each item is cut into output tiles (64 x 256 bits for AND+parity, 128 x
+// 128 for the matrix cores, 4 rows x 64 words for row XOR).  A run is three
+// launches, captured once in a hipGraph: reset of the queue counter, one
+// batched B^T transpose for the AND+parity and matrix-core items, and the
+// persistent product kernel whose workgroups take tiles from the queue until
+// it is empty.
 #include <algorithm>
 #include <cmath>
 #include <numeric>
 #include <vector>
 
+#include "mfma_device.h"
 #include "product_device.h"
 #include "transpose_device.h"
 
@@ -31,14 +34,16 @@ struct MMItem {
     uint64_t tb_begin;    // first 64 x 64 transpose block (prefix in queue order)
     uint32_t m, k, n;
     uint32_t lda, ldb, ldc, ldbt;
-    uint32_t algo;        // MM_ANDPOP or MM_ROWXOR
+    uint32_t algo;        // MM_ANDPOP, MM_ROWXOR or MM_MFMA
     uint32_t tiles_x;     // tiles along the rows of C
     uint32_t pad;
 };
 
 namespace {
 
-enum { MM_ANDPOP = 0, MM_ROWXOR = 1 };
+enum { MM_ANDPOP = 0, MM_ROWXOR = 1, MM_MFMA = 2 };
+constexpr int MF_TILE = 128;  // rows and columns of C per matrix-core tile
+constexpr uint32_t MF_KMAX = 1u << 23;  // the tile's accumulators start at 2^23
 
 using dev::AP_COLS;
 using dev::AP_ROWS;
@@ -62,7 +67,7 @@ __global__ void k_mm_reset(unsigned long long *counter) {
     if (threadIdx.x == 0) *counter = 0ull;
 }
 
-// B_i^T (n x ldbt words) of every AND+parity item: one wave per 64 x 64 bit
+// B_i^T (n x ldbt words) of every AND+parity and matrix-core item: one wave per 64 x 64 bit
 // block, grid-strided over all items' blocks.
 __global__ void __launch_bounds__(256)
 k_mm_transpose(const MMItem *__restrict__ items, int nitems, uint64_t total_blocks,
@@ -86,6 +91,67 @@ k_mm_transpose(const MMItem *__restrict__ items, int nitems, uint64_t total_bloc
         const uint32_t c = bn * 64 + lane;
         if (c < it.n) arena[it.bt_off + (size_t)c * it.ldbt + bk] = v;
     }
+}
+
+// One 128 x 128 tile of C_i on the fp4 matrix cores: four waves of 64 x 64
+// (2 x 2 MFMA tiles), fragments read straight from the row-major packed rows
+// of A_i and of B_i^T (the arena) with the stage-tiled layout's map (stage
+// s, k half h of a lane: words 4s + 2h and 4s + 2h + 1 of its row), so A and
+// B^T agree on which k each fp4 element holds; the next stage's fragments
+// load before the current stage's MFMAs.  Accumulators start at 2^23 (the
+// planner keeps k < 2^23) and the epilogue packs parities as the direct
+// product kernel does.
+__device__ __forceinline__ void mfma_tile(const MMItem &it, const uint64_t *__restrict__ bt, int row0,
+                                          int col0) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int half = lane >> 5, l32 = lane & 31;
+    const int m = (int)it.m, n = (int)it.n;
+    const int kw = (int)((it.k + 63) / 64), nst = (int)((it.k + 255) / 256);
+    const uint64_t *ar[2], *br[2];
+    bool av[2], bv[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int r = row0 + wr * 64 + 32 * t + l32, c = col0 + wc * 64 + 32 * t + l32;
+        av[t] = r < m;
+        bv[t] = c < n;
+        ar[t] = it.A + (size_t)(av[t] ? r : 0) * it.lda;
+        br[t] = bt + (size_t)(bv[t] ? c : 0) * it.ldbt;
+    }
+    auto frag = [&](const uint64_t *row, bool ok, int s) -> uint4 {
+        const int w = 4 * s + 2 * half;
+        const uint64_t x = ok && w < kw ? row[w] : 0ull;
+        const uint64_t y = ok && w + 1 < kw ? row[w + 1] : 0ull;
+        return make_uint4((uint32_t)x, (uint32_t)(x >> 32), (uint32_t)y, (uint32_t)(y >> 32));
+    };
+    pk::v16f acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[x][y][r] = 8388608.0f;
+    uint4 fa[2], fb[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        fa[t] = frag(ar[t], av[t], 0);
+        fb[t] = frag(br[t], bv[t], 0);
+    }
+    for (int s = 0; s < nst; ++s) {
+        const uint4 ga[2] = {fa[0], fa[1]}, gb[2] = {fb[0], fb[1]};
+        if (s + 1 < nst)
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                fa[t] = frag(ar[t], av[t], s + 1);
+                fb[t] = frag(br[t], bv[t], s + 1);
+            }
+        pk::mm<0>(ga, gb, acc);
+        pk::mm<1>(ga, gb, acc);
+        pk::mm<2>(ga, gb, acc);
+        pk::mm<3>(ga, gb, acc);
+    }
+    pk::store_parity_block(acc[0][0], acc[0][1], acc[1][0], acc[1][1], row0 + wr * 64, col0 + wc * 64, it.C,
+                           it.ldc, m, n);
 }
 
 // Queue tiles a workgroup takes per atomic: a row-XOR tile is a few row
@@ -118,6 +184,8 @@ k_mm_batch(const MMItem *__restrict__ items, int nitems, uint64_t total_tiles,
         if (it.algo == MM_ANDPOP) {
             dev::andpop_tile(it.A, it.lda, arena + it.bt_off, it.ldbt, it.C, it.ldc, (int)it.m, (int)it.n,
                              (int)((it.k + 31) / 32), (int)(tx * AP_ROWS), (int)(ty * AP_COLS));
+        } else if (it.algo == MM_MFMA) {
+            mfma_tile(it, arena + it.bt_off, (int)(tx * MF_TILE), (int)(ty * MF_TILE));
         } else {
             const int row = (int)(tx * RX_WAVES) + wave;
             if (row < (int)it.m)
@@ -168,13 +236,22 @@ using namespace mela;
 
 namespace {
 
-// Estimated seconds of one item under each engine (rates of the single
-// product kernels on MI355X: AND+parity ~1.5e15 bit-op/s over whole 64 x 256
-// tiles, row XOR ~1.6e13 bytes/s of B rows, whole 64-word chunks).
+// Estimated seconds of one item under each engine (rates of the kernels on
+// MI355X: AND+parity ~1.5e15 bit-op/s over whole 64 x 256 tiles, the
+// matrix-core tiles MF_RATE over whole 128 x 128 tiles and 256-bit stages,
+// row XOR ~1.6e13 bytes/s of B rows, whole 64-word chunks).
+constexpr double MF_RATE = 8.0e15;
 double est_andpop(const MMItem &it) {
     const double mp = (double)((it.m + AP_ROWS - 1) / AP_ROWS * AP_ROWS);
     const double np = (double)((it.n + AP_COLS - 1) / AP_COLS * AP_COLS);
     return 2.0 * mp * np * (double)it.k / 1.5e15;
+}
+// the matrix-core tiles: 128 x 128, k in whole 256-bit stages
+double est_mfma(const MMItem &it) {
+    const double mp = (double)((it.m + MF_TILE - 1) / MF_TILE * MF_TILE);
+    const double np = (double)((it.n + MF_TILE - 1) / MF_TILE * MF_TILE);
+    const double kp = (double)((it.k + 255) / 256 * 256);
+    return 2.0 * mp * np * kp / MF_RATE;
 }
 double est_rowxor(const MMItem &it, unsigned long long ones) {
     const double row_bytes = (double)std::max<uint32_t>((it.n + 63) / 64, 64) * 8.0;
@@ -209,6 +286,9 @@ mela_status plan(mela_matmul_batch *b) {
         if (it.algo == MM_ANDPOP) {
             it.tiles_x = (it.m + AP_ROWS - 1) / AP_ROWS;
             cost[i] = est_andpop(it);
+        } else if (it.algo == MM_MFMA) {
+            it.tiles_x = (it.m + MF_TILE - 1) / MF_TILE;
+            cost[i] = est_mfma(it);
         } else {
             it.tiles_x = (it.m + RX_WAVES - 1) / RX_WAVES;
             cost[i] = est_rowxor(it, b->ones[i]);
@@ -226,10 +306,11 @@ mela_status plan(mela_matmul_batch *b) {
         it.tb_begin = tblocks;
         if (it.m && it.n) {
             const uint64_t ty = it.algo == MM_ANDPOP ? (it.n + AP_COLS - 1) / AP_COLS
+                                : it.algo == MM_MFMA ? (it.n + MF_TILE - 1) / MF_TILE
                                                      : ((it.n + 63) / 64 + 63) / 64;
             tiles += (uint64_t)it.tiles_x * ty;
         }
-        if (it.algo == MM_ANDPOP && it.m && it.n) {
+        if ((it.algo == MM_ANDPOP || it.algo == MM_MFMA) && it.m && it.n) {
             it.ldbt = (it.k + 63) / 64;
             it.bt_off = arena;
             arena += (size_t)it.n * it.ldbt;
@@ -348,7 +429,9 @@ mela_status mela_matmul_batch_create(mela_ctx *ctx, const mela_bitmat *A, const 
     }
     for (size_t i = 0; i < nb; ++i) {
         MMItem &it = b->items[i];
-        it.algo = est_rowxor(it, b->ones[i]) < est_andpop(it) ? MM_ROWXOR : MM_ANDPOP;
+        const double ea = est_andpop(it), er = est_rowxor(it, b->ones[i]);
+        const double em = it.k < MF_KMAX ? est_mfma(it) : 1e30;
+        it.algo = er < ea && er < em ? MM_ROWXOR : em < ea ? MM_MFMA : MM_ANDPOP;
     }
     int occ = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)k_mm_batch, 256, 0) != hipSuccess) {
@@ -364,20 +447,26 @@ mela_status mela_matmul_batch_create(mela_ctx *ctx, const mela_bitmat *A, const 
 
 mela_status mela_matmul_batch_set_algos(mela_matmul_batch *b, const int *algos) {
     if (!b || (!b->items.empty() && !algos)) return set_error(MELA_ERR_INVALID, "null argument");
-    for (size_t i = 0; i < b->items.size(); ++i)
-        if (algos[i] != MELA_ALGO_ANDPOP && algos[i] != MELA_ALGO_ROWXOR)
-            return set_error(MELA_ERR_INVALID, "batched products take MELA_ALGO_ANDPOP or MELA_ALGO_ROWXOR");
+    for (size_t i = 0; i < b->items.size(); ++i) {
+        if (algos[i] != MELA_ALGO_ANDPOP && algos[i] != MELA_ALGO_ROWXOR && algos[i] != MELA_ALGO_MFMA_FP4)
+            return set_error(MELA_ERR_INVALID,
+                             "batched products take MELA_ALGO_ANDPOP, MELA_ALGO_ROWXOR or MELA_ALGO_MFMA_FP4");
+        if (algos[i] == MELA_ALGO_MFMA_FP4 && b->items[i].k >= MF_KMAX)
+            return set_error(MELA_ERR_INVALID, "batched matrix-core products need k < 2^23");
+    }
     CtxLock lk(b->ctx->mu);
     MELA_HIP(hipSetDevice(b->ctx->device));
     for (size_t i = 0; i < b->items.size(); ++i)
-        b->items[i].algo = algos[i] == MELA_ALGO_ROWXOR ? MM_ROWXOR : MM_ANDPOP;
+        b->items[i].algo = algos[i] == MELA_ALGO_ROWXOR ? MM_ROWXOR : algos[i] == MELA_ALGO_MFMA_FP4 ? MM_MFMA : MM_ANDPOP;
     return plan(b);
 }
 
 mela_status mela_matmul_batch_get_algos(mela_matmul_batch *b, int *algos) {
     if (!b || (!b->items.empty() && !algos)) return set_error(MELA_ERR_INVALID, "null argument");
     for (size_t i = 0; i < b->items.size(); ++i)
-        algos[i] = b->items[i].algo == MM_ROWXOR ? MELA_ALGO_ROWXOR : MELA_ALGO_ANDPOP;
+        algos[i] = b->items[i].algo == MM_ROWXOR ? MELA_ALGO_ROWXOR
+                   : b->items[i].algo == MM_MFMA ? MELA_ALGO_MFMA_FP4
+                                                 : MELA_ALGO_ANDPOP;
     return MELA_OK;
 }
 

@@ -250,42 +250,17 @@ __device__ __forceinline__ void store_tile(Acc (&acc)[TR][TR], int row0, int col
 }
 
 // C tile epilogue of the direct kernel, whose accumulators start at 2^23:
-// a count c < 2^23 then leaves 2^23 + c exactly, with c in the mantissa, so
-// bit 0 of the float is the parity.  Per 64 x 64 quarter of the wave's
-// 128 x 128 (2 x 2 MFMA tiles): each lane packs its 64 parities into one
-// word with one v_alignbit per bit (register r of lane l: row 8(r/4) +
-// 4(l/32) + r%4, column l%32 of its tile), the in-wave 64 x 64 bit transpose
-// gives lane j the 64 lanes' bit j, i.e. 32 columns of two rows, and lanes j
-// and j^16 (the two column tiles) combine their halves into one row's word,
-// which lane j stores.
+// the wave's 128 x 128 as four 64 x 64 parity blocks (pk::store_parity_block).
 template <int TR>
 __device__ __forceinline__ void store_tile_offset(v16f (&acc)[TR][TR], int row0, int col0,
                                                   uint64_t *__restrict__ C, size_t ldc, int m, int n) {
     static_assert(TR % 2 == 0, "quarters of 2 x 2 MFMA tiles");
-    const int lane = threadIdx.x & 63;
-    const int nw = (n + 63) >> 6;
-    const uint64_t lastmask = (n & 63) ? ((1ull << (n & 63)) - 1ull) : ~0ull;
-    const int tri = lane >> 5, tci = (lane >> 4) & 1, r = lane & 15;
 #pragma unroll
     for (int cr = 0; cr < TR / 2; ++cr)
 #pragma unroll
-        for (int cc = 0; cc < TR / 2; ++cc) {
-            uint32_t xh[2] = {0u, 0u};
-#pragma unroll
-            for (int q = 0; q < 2; ++q)
-#pragma unroll
-                for (int h = 0; h < 2; ++h)
-#pragma unroll
-                    for (int e = 0; e < 16; ++e)
-                        xh[q] = __builtin_amdgcn_alignbit(__float_as_uint(acc[2 * cr + q][2 * cc + h][e]), xh[q], 1);
-            const uint64_t y = dev::transpose64(((uint64_t)xh[1] << 32) | xh[0], lane);
-            const uint64_t yp = dev::shfl_xor64(y, 16);
-            uint64_t word = tci == 0 ? ((y & 0xffffffffull) | (yp << 32)) : ((yp >> 32) | (y & 0xffffffff00000000ull));
-            const int row = row0 + cr * 64 + tri * 32 + 8 * (r >> 2) + (r & 3) + 4 * tci;
-            const int w = (col0 >> 6) + cc;
-            if (w == nw - 1) word &= lastmask;
-            if (row < m && w < nw) C[(size_t)row * ldc + w] = word;
-        }
+        for (int cc = 0; cc < TR / 2; ++cc)
+            pk::store_parity_block(acc[2 * cr][2 * cc], acc[2 * cr][2 * cc + 1], acc[2 * cr + 1][2 * cc],
+                                   acc[2 * cr + 1][2 * cc + 1], row0 + cr * 64, col0 + cc * 64, C, ldc, m, n);
 }
 
 // ---------------------------------------------------------------------------

@@ -16,6 +16,7 @@
 #pragma once
 
 #include "common.h"
+#include "transpose_device.h"
 
 namespace mela {
 namespace pk {
@@ -53,6 +54,39 @@ __device__ __forceinline__ void mm(const uint4 (&ra)[TA], const uint4 (&rb)[TB],
         for (int tc = 0; tc < TB; ++tc)
             acc[tr][tc] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
                 fa[tr], fb[tc], acc[tr][tc], 4, 4, 0, scale<J>(), 0, scale<J>());
+}
+
+// Parity words of one wave's 64 x 64 block of C (2 x 2 MFMA tiles a[q][h]
+// whose accumulators started at 2^23: a count c < 2^23 leaves 2^23 + c
+// exactly, so bit 0 of the float is the parity).  Each lane packs its 64
+// parities into one word with one v_alignbit per bit (register r of lane l:
+// row 8(r/4) + 4(l/32) + r%4, column l%32 of its tile), the in-wave 64 x 64
+// bit transpose gives lane j the 64 lanes' bit j, i.e. 32 columns of two
+// rows, and lanes j and j^16 (the two column tiles) combine their halves
+// into one row's word, which lane j stores: row row0 + its row, word
+// col0 / 64 of C, bits past column n cleared, nothing past row m or word nw.
+__device__ __forceinline__ void store_parity_block(const v16f &a00, const v16f &a01, const v16f &a10,
+                                                   const v16f &a11, int row0, int col0,
+                                                   uint64_t *__restrict__ C, size_t ldc, int m, int n) {
+    const int lane = threadIdx.x & 63;
+    const int nw = (n + 63) >> 6;
+    uint32_t xh[2] = {0u, 0u};
+#pragma unroll
+    for (int e = 0; e < 16; ++e) xh[0] = __builtin_amdgcn_alignbit(__float_as_uint(a00[e]), xh[0], 1);
+#pragma unroll
+    for (int e = 0; e < 16; ++e) xh[0] = __builtin_amdgcn_alignbit(__float_as_uint(a01[e]), xh[0], 1);
+#pragma unroll
+    for (int e = 0; e < 16; ++e) xh[1] = __builtin_amdgcn_alignbit(__float_as_uint(a10[e]), xh[1], 1);
+#pragma unroll
+    for (int e = 0; e < 16; ++e) xh[1] = __builtin_amdgcn_alignbit(__float_as_uint(a11[e]), xh[1], 1);
+    const uint64_t y = dev::transpose64(((uint64_t)xh[1] << 32) | xh[0], lane);
+    const uint64_t yp = dev::shfl_xor64(y, 16);
+    const int tri = lane >> 5, tci = (lane >> 4) & 1, r = lane & 15;
+    uint64_t word = tci == 0 ? ((y & 0xffffffffull) | (yp << 32)) : ((yp >> 32) | (y & 0xffffffff00000000ull));
+    const int row = row0 + tri * 32 + 8 * (r >> 2) + (r & 3) + 4 * tci;
+    const int w = col0 >> 6;
+    if (w == nw - 1 && (n & 63)) word &= (1ull << (n & 63)) - 1ull;
+    if (row < m && w < nw) C[(size_t)row * ldc + w] = word;
 }
 
 // Stage-tiled packed layout (PK): rows padded to 256; block b of 256 rows,

@@ -1,7 +1,8 @@
 """Batched uneven products in one persistent launch (mela_matmul_batch_*),
 item by item against the oracle's product (mela_matmul_gf2.c:110-208
-restated): the planned per-item engines, each engine forced on every item,
-and replays of the captured graph with other work in between."""
+restated): the planned per-item engines (AND+parity, row XOR, fp4 matrix-core
+tiles), each engine forced on every item, and replays of the captured graph
+with other work in between."""
 import numpy as np
 import pytest
 
@@ -55,7 +56,8 @@ def test_batch_matmul_matches_oracle(mela, orc):
     try:
         b = mela.MatmulBatch(ctx, dA, dB, dC)
         algos = b.algos()
-        assert set(algos) == {mela.ALGO_ANDPOP, mela.ALGO_ROWXOR}
+        assert set(algos) <= {mela.ALGO_ANDPOP, mela.ALGO_ROWXOR, mela.ALGO_MFMA_FP4}
+        assert mela.ALGO_ROWXOR in algos and mela.ALGO_MFMA_FP4 in algos
         for rep in range(3):
             print(f"[batch matmul] planned, replay {rep}", flush=True)
             for C in dC:  # clear the outputs between replays
@@ -67,7 +69,7 @@ def test_batch_matmul_matches_oracle(mela, orc):
             ctx.free(x)
             ctx.sync()
             _check(ctx, orc, items, dC, f"planned replay {rep}")
-        for forced in (mela.ALGO_ANDPOP, mela.ALGO_ROWXOR):
+        for forced in (mela.ALGO_ANDPOP, mela.ALGO_ROWXOR, mela.ALGO_MFMA_FP4):
             print(f"[batch matmul] forced {forced}", flush=True)
             b.set_algos([forced] * len(items))
             for C in dC:
@@ -85,7 +87,7 @@ def test_batch_matmul_matches_oracle(mela, orc):
 
 def test_batch_matmul_plans_by_density(mela):
     """A clearly sparse A (column weight 3) gets the row-XOR product, a dense
-    one the AND+parity tiles."""
+    one the matrix-core tiles; engines outside the three are refused."""
     from mex_linear_algebra_amd import workloads as W
     ctx = mela.Context(0)
     rng = np.random.default_rng(11)
@@ -96,7 +98,9 @@ def test_batch_matmul_plans_by_density(mela):
             mats.append((ctx.upload(A, k), ctx.bitmat(k, n), ctx.bitmat(m, n)))
             ctx.gen_bernoulli(5, mats[-1][1])
         b = mela.MatmulBatch(ctx, [x[0] for x in mats], [x[1] for x in mats], [x[2] for x in mats])
-        assert b.algos() == [mela.ALGO_ROWXOR, mela.ALGO_ANDPOP]
+        assert b.algos() == [mela.ALGO_ROWXOR, mela.ALGO_MFMA_FP4]
+        with pytest.raises(mela.MelaError, match="batched products take"):
+            b.set_algos([mela.ALGO_M4RM, mela.ALGO_ANDPOP])
         b.close()
     finally:
         for x in mats:
