@@ -31,7 +31,7 @@ struct MMItem {
     uint64_t *C;
     uint64_t bt_off;      // words: B^T in the arena (AND+parity items)
     uint64_t tile_begin;  // first queue tile of the item (prefix in queue order)
-    uint64_t tb_begin;    // first 64 x 64 transpose block (prefix in queue order)
+    uint64_t tb_begin;    // first transpose block, 64 k-rows x 4 column words (prefix in queue order)
     uint32_t m, k, n;
     uint32_t lda, ldb, ldc, ldbt;
     uint32_t algo;        // MM_ANDPOP, MM_ROWXOR or MM_MFMA
@@ -67,8 +67,10 @@ __global__ void k_mm_reset(unsigned long long *counter) {
     if (threadIdx.x == 0) *counter = 0ull;
 }
 
-// B_i^T (n x ldbt words) of every AND+parity and matrix-core item: one wave per 64 x 64 bit
-// block, grid-strided over all items' blocks.
+// B_i^T (n x ldbt words) of every AND+parity and matrix-core item: one wave
+// per 64 k-rows x 4 column words (lane = k row, four 64 x 64 bit transposes;
+// one column word per wave ran 3x longer on its strided 8-byte loads),
+// grid-strided over all items' blocks.
 __global__ void __launch_bounds__(256)
 k_mm_transpose(const MMItem *__restrict__ items, int nitems, uint64_t total_blocks,
                uint64_t *__restrict__ arena) {
@@ -79,17 +81,22 @@ k_mm_transpose(const MMItem *__restrict__ items, int nitems, uint64_t total_bloc
         const MMItem it = items[ii];
         const uint64_t lb = b - it.tb_begin;
         const uint32_t nbk = (it.k + 63) / 64;
-        const uint32_t bk = (uint32_t)(lb % nbk), bn = (uint32_t)(lb / nbk);
+        const uint32_t bk = (uint32_t)(lb % nbk), bn0 = 4 * (uint32_t)(lb / nbk);
         const uint32_t nw = (it.n + 63) / 64;
         const uint32_t i = bk * 64 + lane;
-        uint64_t v = 0;
-        if (i < it.k) {
-            v = it.B[(size_t)i * it.ldb + bn];
-            if (bn == nw - 1 && (it.n & 63)) v &= (1ull << (it.n & 63)) - 1ull;
+        uint64_t v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t bn = bn0 + q;
+            v[q] = i < it.k && bn < nw ? it.B[(size_t)i * it.ldb + bn] : 0ull;
+            if (bn == nw - 1 && (it.n & 63)) v[q] &= (1ull << (it.n & 63)) - 1ull;
         }
-        v = dev::transpose64(v, lane);
-        const uint32_t c = bn * 64 + lane;
-        if (c < it.n) arena[it.bt_off + (size_t)c * it.ldbt + bk] = v;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint64_t t = dev::transpose64(v[q], lane);
+            const uint32_t c = (bn0 + q) * 64 + lane;
+            if (c < it.n) arena[it.bt_off + (size_t)c * it.ldbt + bk] = t;
+        }
     }
 }
 
@@ -314,7 +321,7 @@ mela_status plan(mela_matmul_batch *b) {
             it.ldbt = (it.k + 63) / 64;
             it.bt_off = arena;
             arena += (size_t)it.n * it.ldbt;
-            tblocks += (uint64_t)((it.k + 63) / 64) * ((it.n + 63) / 64);
+            tblocks += (uint64_t)((it.k + 63) / 64) * (((it.n + 63) / 64 + 3) / 4);
         }
         q[p] = it;
     }
