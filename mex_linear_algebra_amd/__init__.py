@@ -483,11 +483,13 @@ class MatmulBatch:
         self.handle = h
 
     def algos(self):
+        """Engine of each item: ALGO_MFMA_FP4, ALGO_ANDPOP or ALGO_ROWXOR."""
         arr = (C.c_int * max(self.count, 1))()
         _check(self.lib.mela_matmul_batch_get_algos(self.handle, arr))
         return [arr[i] for i in range(self.count)]
 
     def set_algos(self, algos):
+        """Force each item's engine (ALGO_MFMA_FP4 needs k < 2^23)."""
         arr = (C.c_int * max(self.count, 1))(*[int(a) for a in algos])
         _check(self.lib.mela_matmul_batch_set_algos(self.handle, arr))
 
