@@ -31,7 +31,7 @@ struct MMItem {
     uint64_t *C;
     uint64_t bt_off;      // words: B^T in the arena (AND+parity items)
     uint64_t tile_begin;  // first queue tile of the item (prefix in queue order)
-    uint64_t tb_begin;    // first transpose block, 64 k-rows x 4 column words (prefix in queue order)
+    uint64_t tb_begin;    // first transpose block, 64 k-rows x MT_WORDS column words (prefix in queue order)
     uint32_t m, k, n;
     uint32_t lda, ldb, ldc, ldbt;
     uint32_t algo;        // MM_ANDPOP, MM_ROWXOR or MM_MFMA
@@ -44,6 +44,7 @@ namespace {
 enum { MM_ANDPOP = 0, MM_ROWXOR = 1, MM_MFMA = 2 };
 constexpr int MF_TILE = 128;  // rows and columns of C per matrix-core tile
 constexpr uint32_t MF_KMAX = 1u << 23;  // the tile's accumulators start at 2^23
+constexpr int MT_WORDS = 4;             // column words per wave of the B^T transpose (8: no faster)
 
 using dev::AP_COLS;
 using dev::AP_ROWS;
@@ -68,8 +69,9 @@ __global__ void k_mm_reset(unsigned long long *counter) {
 }
 
 // B_i^T (n x ldbt words) of every AND+parity and matrix-core item: one wave
-// per 64 k-rows x 4 column words (lane = k row, four 64 x 64 bit transposes;
-// one column word per wave ran 3x longer on its strided 8-byte loads),
+// per 64 k-rows x MT_WORDS column words (lane = k row, one 64 x 64 bit
+// transpose per word; one column word per wave ran 3x longer on its strided
+// 8-byte loads),
 // grid-strided over all items' blocks.
 __global__ void __launch_bounds__(256)
 k_mm_transpose(const MMItem *__restrict__ items, int nitems, uint64_t total_blocks,
@@ -81,18 +83,18 @@ k_mm_transpose(const MMItem *__restrict__ items, int nitems, uint64_t total_bloc
         const MMItem it = items[ii];
         const uint64_t lb = b - it.tb_begin;
         const uint32_t nbk = (it.k + 63) / 64;
-        const uint32_t bk = (uint32_t)(lb % nbk), bn0 = 4 * (uint32_t)(lb / nbk);
+        const uint32_t bk = (uint32_t)(lb % nbk), bn0 = MT_WORDS * (uint32_t)(lb / nbk);
         const uint32_t nw = (it.n + 63) / 64;
         const uint32_t i = bk * 64 + lane;
-        uint64_t v[4];
+        uint64_t v[MT_WORDS];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < MT_WORDS; ++q) {
             const uint32_t bn = bn0 + q;
             v[q] = i < it.k && bn < nw ? it.B[(size_t)i * it.ldb + bn] : 0ull;
             if (bn == nw - 1 && (it.n & 63)) v[q] &= (1ull << (it.n & 63)) - 1ull;
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < MT_WORDS; ++q) {
             const uint64_t t = dev::transpose64(v[q], lane);
             const uint32_t c = (bn0 + q) * 64 + lane;
             if (c < it.n) arena[it.bt_off + (size_t)c * it.ldbt + bk] = t;
@@ -321,7 +323,7 @@ mela_status plan(mela_matmul_batch *b) {
             it.ldbt = (it.k + 63) / 64;
             it.bt_off = arena;
             arena += (size_t)it.n * it.ldbt;
-            tblocks += (uint64_t)((it.k + 63) / 64) * (((it.n + 63) / 64 + 3) / 4);
+            tblocks += (uint64_t)((it.k + 63) / 64) * (((it.n + 63) / 64 + MT_WORDS - 1) / MT_WORDS);
         }
         q[p] = it;
     }
