@@ -301,9 +301,9 @@ void mela_batch_destroy(mela_batch *batch);
  * matmul_gf2.c:72): C_i = A_i * B_i for thousands of uneven packed device
  * products in one persistent launch.  mela_matmul_batch_create validates the
  * shapes (A_i.cols == B_i.rows, C_i is A_i.rows x B_i.cols), counts the set
- * bits of every A_i and gives each item AND+parity, the fp4 matrix cores
- * (128 x 128 tiles, k < 2^23) or the row-XOR product (sparse A_i), whichever
- * the measured kernel rates say is cheapest;
+ * bits of every A_i and gives each item the fp4 matrix cores (128 x 128
+ * tiles; AND+parity when k >= 2^23) or the row-XOR product (sparse A_i),
+ * whichever the measured kernel rates say is cheaper;
  * mela_matmul_batch_run enqueues all products (one hipGraph launch with
  * MELA_BATCH_GRAPH) on the context stream.  The C_i receive their products
  * (words past each row's last column are left as they are); the inputs must

@@ -3,11 +3,11 @@
 // matmul_gf2.c:72 / matmul_m4ri.c:140).
 //
 // Planning (mela_matmul_batch_create) counts the set bits of every A_i on the
-// device and gives each item the cheapest of three engines by the measured
-// rates of the kernels: AND+parity over B_i^T (cost m n k bit operations),
-// the fp4 matrix cores over A_i and B_i^T (the same count at their rate) or
-// the row-XOR product (one B_i-row read per set bit of A_i, the reference's
-// own GE product, matmul_gf2.c:127-186).  Items are queued largest first;
+// device and gives each item the cheaper of two engines by the measured
+// rates of the kernels: the fp4 matrix cores over A_i and B_i^T (cost m n k
+// bit operations at their rate; AND+parity over B_i^T when k >= 2^23) or the
+// row-XOR product (one B_i-row read per set bit of A_i, the reference's own
+// GE product, matmul_gf2.c:127-186).  Items are queued largest first;
 // each item is cut into output tiles (64 x 256 bits for AND+parity, 128 x
 // 128 for the matrix cores, 4 rows x 64 words for row XOR).  A run is three
 // launches, captured once in a hipGraph: reset of the queue counter, one
@@ -438,9 +438,12 @@ mela_status mela_matmul_batch_create(mela_ctx *ctx, const mela_bitmat *A, const 
     }
     for (size_t i = 0; i < nb; ++i) {
         MMItem &it = b->items[i];
-        const double ea = est_andpop(it), er = est_rowxor(it, b->ones[i]);
-        const double em = it.k < MF_KMAX ? est_mfma(it) : 1e30;
-        it.algo = er < ea && er < em ? MM_ROWXOR : em < ea ? MM_MFMA : MM_ANDPOP;
+        // the matrix cores beat AND+parity on every item shape measured
+        // (forcing the planner's AND+parity items onto them only gained), so
+        // the choice is theirs (AND+parity past their k limit) or row XOR
+        const bool mf = it.k < MF_KMAX;
+        const double ed = mf ? est_mfma(it) : est_andpop(it);
+        it.algo = est_rowxor(it, b->ones[i]) < ed ? MM_ROWXOR : mf ? MM_MFMA : MM_ANDPOP;
     }
     int occ = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)k_mm_batch, 256, 0) != hipSuccess) {
